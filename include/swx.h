@@ -1,0 +1,169 @@
+/*
+ * swx.h — C ABI of the MI355X (gfx950) backend for the SeismicWaves.jl
+ * finite-difference forward / adjoint-gradient path.
+ *
+ * The reference dispatches each time step through a backend *module*
+ * returned by select_backend (src/models/backend_selection.jl:2-40, GPU
+ * modules registered by ext/SeismicWaves_AMDGPUBackendExt.jl:15-26).  Its
+ * drivers call, once per step, backend.forward_onestep_CPML!,
+ * backend.adjoint_onestep_CPML!, backend.prescale_residuals! and
+ * backend.correlate_gradient[_m0|_m1]! (acou_forward.jl:49-58,
+ * acou_gradient.jl:33-82 and :145-175), and keep the field state and the
+ * LinearCheckpointer (src/utils/checkpointers.jl) in the wave simulation.
+ *
+ * This library replaces that whole per-shot loop for a *batch* of shots:
+ * one plan owns the device-resident state of all shots (fields, C-PML memory
+ * variables, checkpoints, replay buffers), swx_forward runs the forward time
+ * loop of every shot, swx_adjoint replays from the GPU-resident checkpoints,
+ * runs the adjoint loop and correlates the gradient.  The host keeps what the
+ * reference keeps on the host: checks, C-PML coefficients (cpmlcoeffs.jl),
+ * source scaling (possrcrec_scaletf), the misfit and its adjoint source
+ * (L2Misfit.jl) and the gradient post-processing (mute, chain rule).
+ *
+ * Conventions
+ *   - Arrays are column-major with x (dimension 1) fastest, exactly the
+ *     memory layout of the Julia arrays the reference hands to its backends.
+ *   - Grid indices are 1-based Int64 (Julia Matrix{Int} from
+ *     find_nearest_grid_points, src/utils/utils.jl:47-58).
+ *   - Scalars of type T are float (SWX_F32) or double (SWX_F64) as the
+ *     plan's dtype says; `const void*` arguments point at host memory of T.
+ *   - Every function returns SWX_OK (0) or an swx_status; the message of the
+ *     last failure on the calling thread is returned by swx_last_error().
+ *     All validation happens before any kernel is launched, mirroring the
+ *     reference's Julia-side @assert checks.
+ *   - A plan is bound to one device and one HIP stream; a plan is not
+ *     thread-safe, distinct plans are.
+ */
+#ifndef SWX_H
+#define SWX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWX_ABI_VERSION 1
+
+typedef enum {
+  SWX_OK = 0,
+  SWX_EINVAL = 1,       /* invalid argument (reference: AssertionError)     */
+  SWX_ENOMEM = 2,       /* device memory budget exceeded                    */
+  SWX_EHIP = 3,         /* HIP runtime error                                */
+  SWX_ESTATE = 4,       /* call out of order (e.g. adjoint before forward)  */
+  SWX_EUNSUPPORTED = 5  /* physics/dimension not built into this library    */
+} swx_status;
+
+typedef enum {
+  SWX_ACOUSTIC_CD = 1,  /* AcousticCDCPMLWaveSimulation, acou_models.jl:68   */
+  SWX_ACOUSTIC_VD = 2,  /* AcousticVDStaggeredCPMLWaveSimulation, :311     */
+  SWX_ELASTIC_ISO = 3   /* ElasticIsoCPMLWaveSimulation, ela_models.jl:177  */
+} swx_physics;
+
+typedef enum { SWX_F32 = 1, SWX_F64 = 2 } swx_dtype;
+
+typedef struct swx_plan swx_plan;
+
+/* Plan description: the static part of InputParametersAcoustic
+ * (acou_params.jl:9-35), CPMLBoundaryConditionParameters (bdc_params.jl:
+ * 16-27) and GradParameters.check_freq (genparameters.jl:63-78), plus the
+ * batch capacity. */
+typedef struct {
+  int32_t physics;       /* swx_physics                                     */
+  int32_t dtype;         /* swx_dtype                                       */
+  int32_t ndim;          /* 2 or 3                                          */
+  int32_t halo;          /* C-PML thickness in grid points (>= 0)           */
+  int64_t n[3];          /* grid points per dimension, x first              */
+  double h[3];           /* grid spacing per dimension                      */
+  double dt;             /* time step                                       */
+  int32_t freetop;       /* free surface on the last axis                   */
+  int32_t max_shots;     /* batch capacity                                  */
+  int32_t nt_max;        /* largest number of time steps of any shot        */
+  int32_t check_freq;    /* checkpoint interval; 1 = keep every time step   */
+  int32_t gradient;      /* 1: allocate adjoint state and checkpoints       */
+  int32_t device;        /* HIP device ordinal                              */
+  double mem_budget;     /* bytes of HBM the plan may use, 0 = 90% of free  */
+} swx_plan_desc;
+
+/* One shot of scalar sources / scalar receivers (ScalarShot, shot.jl:9-16).
+ * `srctf` is the scaled source term of possrcrec_scaletf (acou_forward.jl:
+ * 6-20 for CD, :67-81 for VD), nt x nsrc column-major.  `cpml[d]` are the
+ * coefficient vectors of CPMLCoefficientsAxis for axis d (cpmlcoeffs.jl:2-16)
+ * as computed by init_bdc! for this shot's dominant frequency:
+ * a (2(halo+1)), a_h (2halo), b (2(halo+1)), b_h (2halo).                  */
+typedef struct {
+  int32_t nt;
+  int32_t nsrc;
+  const int64_t *src_idx;   /* nsrc x ndim, 1-based                         */
+  const void *srctf;        /* nt x nsrc, T                                  */
+  int32_t nrec;
+  const int64_t *rec_idx;   /* nrec x ndim, 1-based                         */
+  const void *cpml[3][4];   /* [axis][a, a_h, b, b_h], T                     */
+} swx_shot_desc;
+
+typedef struct {
+  int64_t bytes_state;       /* fields + C-PML memory variables             */
+  int64_t bytes_checkpoints; /* checkpoint storage                          */
+  int64_t bytes_replay;      /* replay buffer                               */
+  int32_t segment;           /* replay segment length actually used         */
+  int32_t nshots;            /* shots currently set                         */
+  double ms_forward;         /* device time of the last swx_forward         */
+  double ms_adjoint;         /* device time of the last swx_adjoint         */
+  double ms_stencil;         /* summed device time of the stencil launches
+                                of the last call with kernel timing on      */
+  int64_t n_stencil;         /* number of those launches                    */
+} swx_plan_info;
+
+/* Version of this ABI (SWX_ABI_VERSION). No reference counterpart. */
+int swx_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *swx_last_error(void);
+
+/* Replaces the allocation done by the WaveSimulation constructors
+ * (acou_models.jl:90-201, :333-451) for `max_shots` shots at once. */
+int swx_plan_create(const swx_plan_desc *desc, swx_plan **plan);
+
+int swx_plan_destroy(swx_plan *plan);
+
+/* Replaces precompute_fact! uploads (acou_models.jl:60, :292-300): the
+ * material factor fields consumed by the kernels, host arrays of T.
+ *   SWX_ACOUSTIC_CD: fields[0] = fact = dt^2 vp^2            (n)
+ *   SWX_ACOUSTIC_VD: fields[0] = fact_m0 = vp^2 rho dt       (n)
+ *                    fields[1] = fact_m1_stag[1] (x)         (nx-1, nz)
+ *                    fields[2] = fact_m1_stag[2] (z)         (nx, nz-1) */
+int swx_set_model(swx_plan *plan, int nfields, const void *const *fields);
+
+/* Replaces the per-shot uploads of swforward_1shot!/swgradient_1shot!
+ * (acou_forward.jl:38-41) for a batch of shots.  Validates positions. */
+int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots);
+
+/* Forward time loop of every shot (acou_forward.jl:49-58 per shot).
+ * traces: host T array, the shots' seismograms (nt_s x nrec_s, column-major)
+ * concatenated in shot order; may be NULL.  With save_state != 0 (gradient
+ * plans) the checkpoints for swx_adjoint stay resident on the device. */
+int swx_forward(swx_plan *plan, void *traces, int save_state);
+
+/* Adjoint time loop with replay from the device checkpoints and gradient
+ * correlation (acou_gradient.jl:51-84 for CD, :145-175 for VD).
+ * adjsrc: host T array, -dchi/du per shot (nt_s x nrec_s) concatenated; the
+ *   prescale by the material factor at the receivers is done here.
+ * grad: host T output.  Raw gradient fields as the reference's grid fields
+ *   hold them after the adjoint loop, already summed over shots unless
+ *   per_shot != 0 (then nshots consecutive copies):
+ *   SWX_ACOUSTIC_CD: grad_vp (n)
+ *   SWX_ACOUSTIC_VD: grad_m0 (n), grad_m1_stag x (nx-1,nz), z (nx,nz-1) */
+int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot);
+
+/* Sizes, segment length and device timings of the last calls. */
+int swx_plan_get_info(swx_plan *plan, swx_plan_info *info);
+
+/* Record HIP events around every stencil launch of subsequent calls
+ * (0 = off).  Used by the benchmark to time the dominant kernel. */
+int swx_set_kernel_timing(swx_plan *plan, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWX_H */

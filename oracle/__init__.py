@@ -1,0 +1,338 @@
+"""CPU oracle for the SeismicWaves.jl finite-difference forward/adjoint path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker: tests/, the smoke()
+entry of __graft_entry__ and the cpu_baseline leg of bench.py may import it;
+the product library (seismicwaves.jl_amd) never does.
+
+It has two halves, both restating the reference from source:
+
+* host-side preparation in numpy (C-PML coefficients, nearest-grid-point
+  lookup, source scaling, L2 misfit and its adjoint source, gradient
+  post-processing), each function citing the reference file:line it follows;
+* the time loops in C (swx_oracle.c / oracle_impl.h, built by oracle/Makefile
+  into liboracle.so), called through ctypes.
+
+Parity pinning of every function is listed in `oracle/PINNING.md`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, no GPU)."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        _LIB = ctypes.CDLL(so)
+    return _LIB
+
+
+# ---------------------------------------------------------------------------
+# ctypes structures mirroring swx_oracle.c
+# ---------------------------------------------------------------------------
+class Geom(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64 * 3), ("h", ctypes.c_double * 3),
+                ("dt", ctypes.c_double), ("halo", ctypes.c_int), ("freetop", ctypes.c_int)]
+
+
+class _Cpml(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_void_p), ("a_h", ctypes.c_void_p),
+                ("b", ctypes.c_void_p), ("b_h", ctypes.c_void_p)]
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _geom(n, h, dt, halo, freetop=0):
+    g = Geom()
+    for d in range(3):
+        g.n[d] = int(n[d]) if d < len(n) else 1
+        g.h[d] = float(h[d]) if d < len(h) else 1.0
+    g.dt = float(dt)
+    g.halo = int(halo)
+    g.freetop = int(freetop)
+    return g
+
+
+def _cpml_struct(coeffs, dtype):
+    arrs = [np.ascontiguousarray(coeffs[k], dtype=dtype) for k in ("a", "a_h", "b", "b_h")]
+    st = _Cpml(*[_ptr(a) for a in arrs])
+    return st, arrs
+
+
+# ---------------------------------------------------------------------------
+# Host-side restatements (numpy)
+# ---------------------------------------------------------------------------
+def rickerstf(t, t0, f0):
+    """src/utils/utils.jl:6 (Ricker wavelet)."""
+    arg = (np.pi * f0 * (t - t0)) ** 2
+    return (1.0 - 2.0 * arg) * np.exp(-arg)
+
+
+def gaussderivstf(t, t0, f0):
+    """src/utils/utils.jl:13."""
+    return (t - t0) * np.exp(-(np.pi * f0 * (t - t0)) ** 2)
+
+
+def calc_kab_cpml_axis(halo, dt, npower, d0, alpha_max, onwhere):
+    """calc_Kab_CPML_axis, src/models/cpmlcoeffs.jl:43-100 (K_max = nothing)."""
+    size = halo
+    if onwhere == "halfgrd":
+        shift = 0.5
+    elif onwhere == "ongrd":
+        shift = 0.0
+        size += 1
+    else:
+        raise ValueError("Wrong onwhere parameter!")
+    dist = np.linspace(0 + shift, size + shift - 1, size) if size > 1 else np.array([0 + shift] * size)
+    if halo != 0:
+        nleft = dist[::-1] / halo
+        nright = dist / halo
+    else:
+        nleft = dist[::-1]
+        nright = dist
+
+    def side(nd):
+        d = d0 * nd ** npower
+        alpha = alpha_max * (1.0 - nd)
+        b = np.exp(-(d / 1.0 + alpha) * dt)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            a = d * (b - 1.0) / (1.0 * (d + 1.0 * alpha))
+        return a, b
+
+    al, bl = side(nleft)
+    ar, br = side(nright)
+    return al, ar, bl, br
+
+
+def cpml_coefficients_axis(vel_max, dt, halo, rcoef, thickness, f0, dtype=np.float64):
+    """compute_CPML_coefficientsAxis!, src/models/cpmlcoeffs.jl:18-41.
+
+    Scalar inputs are first rounded to `dtype` as the reference converts them
+    to T; the profile itself is evaluated in Float64 (LinRange) and rounded.
+    """
+    T = dtype
+    alpha_max = T(np.pi * T(f0))
+    npower = T(2.0)
+    d0 = T(-(npower + 1) * T(vel_max) * np.log(T(rcoef)) / (2.0 * T(thickness))) if halo != 0 else T(0.0)
+    a_l, a_r, b_l, b_r = calc_kab_cpml_axis(halo, float(T(dt)), float(npower), float(d0), float(alpha_max), "ongrd")
+    ah_l, ah_r, bh_l, bh_r = calc_kab_cpml_axis(halo, float(T(dt)), float(npower), float(d0), float(alpha_max), "halfgrd")
+    return {
+        "a": np.concatenate([a_l, a_r]).astype(T),
+        "a_h": np.concatenate([ah_l, ah_r]).astype(T),
+        "b": np.concatenate([b_l, b_r]).astype(T),
+        "b_h": np.concatenate([bh_l, bh_r]).astype(T),
+    }
+
+
+def init_bdc(vp, dt, spacing, halo, rcoef, freeboundtop, f0, dtype=np.float64):
+    """init_bdc! for CPML, src/models/acoustic/acou_init_bc.jl:7-40."""
+    vel_max = float(np.max(vp))
+    out = []
+    for h in spacing:
+        out.append(cpml_coefficients_axis(vel_max, dt, halo, rcoef, h * halo, f0, dtype))
+    if freeboundtop and len(out) >= 1:
+        last = out[-1]
+        for k, v in (("a", 0.0), ("a_h", 0.0), ("b", 1.0), ("b_h", 1.0)):
+            last[k][: len(last[k]) // 2] = v
+    return out
+
+
+def find_nearest_grid_points(positions, spacing):
+    """src/utils/utils.jl:47-58: round(pos/h + 1, RoundNearestTiesUp), 1-based."""
+    positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
+    idx = np.floor(positions / np.asarray(spacing, dtype=np.float64)[None, :] + 1 + 0.5)
+    return idx.astype(np.int64)
+
+
+def scale_srctf_cd(tf, spacing, dt, vp, src_idx, dtype=np.float64):
+    """possrcrec_scaletf for CD, src/models/acoustic/acou_forward.jl:6-20."""
+    T = dtype
+    tf = np.asarray(tf, dtype=T)
+    out = tf / T(np.prod(np.asarray(spacing, dtype=T))) * (T(dt) ** 2)
+    vp = np.asarray(vp, dtype=T)
+    for s in range(out.shape[1]):
+        out[:, s] *= vp[tuple(src_idx[s] - 1)] ** 2
+    return out.astype(T)
+
+
+def l2_misfit(syn, obs, invcov=None, windows=()):
+    """calcmisfit, src/inversion/misfits/L2Misfit.jl:24-42."""
+    res = np.asarray(syn, dtype=np.float64) - np.asarray(obs, dtype=np.float64)
+    if windows:
+        mask = np.zeros(res.shape[0])
+        for a, b in windows:
+            mask[a - 1:b] = 1.0
+        res = mask[:, None] * res
+    if invcov is None:
+        return 0.5 * float(np.sum(res * res))
+    return 0.5 * float(np.sum(res * (np.asarray(invcov) @ res)))
+
+
+def l2_dchi_du(syn, obs, invcov=None, windows=()):
+    """∂χ_∂u, src/inversion/misfits/L2Misfit.jl:64-77."""
+    res = np.asarray(syn, dtype=np.float64) - np.asarray(obs, dtype=np.float64)
+    if windows:
+        mask = np.zeros(res.shape[0])
+        for a, b in windows:
+            mask[a - 1:b] = 1.0
+        res = mask[:, None] * res
+    if invcov is None:
+        return res
+    return np.asarray(invcov) @ res
+
+
+def mute_around_points(grad, points, spacing, radius):
+    """mutearoundmultiplepoints!, src/utils/mute_grad.jl:49-126."""
+    if radius == 0:
+        return grad
+    grad = grad.copy()
+    n = grad.shape
+    N = len(n)
+    extent = [(n[d] - 1) * spacing[d] for d in range(N)]
+    rmax = radius * max(spacing)
+    for pt in np.atleast_2d(points):
+        lo, hi = [], []
+        for d in range(N):
+            if not (0.0 <= pt[d] <= extent[d]):
+                raise ValueError("mutearoundpoint!(): The point lies outside the grid")
+            ijk = int(np.floor(np.floor(pt[d] / spacing[d]))) + 1
+            lo.append(ijk - radius)
+            hi.append(ijk + radius)
+        for idx in np.ndindex(*[hi[d] - lo[d] + 1 for d in range(N)]):
+            cell = [lo[d] + idx[d] for d in range(N)]
+            if not all(1 <= cell[d] <= n[d] for d in range(N)):
+                continue
+            xyz = [(cell[d] - 1) * spacing[d] for d in range(N)]
+            r = np.sqrt(sum((pt[d] - xyz[d]) ** 2 for d in range(N)))
+            if r <= rmax:
+                grad[tuple(c - 1 for c in cell)] *= r / rmax
+    return grad
+
+
+# ---------------------------------------------------------------------------
+# 2D acoustic constant density: time loops in C
+# ---------------------------------------------------------------------------
+def _fn(name, dtype):
+    return getattr(lib(), name + ("_f64" if dtype == np.float64 else "_f32"))
+
+
+def _f_order(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def cd2d_forward(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, dtype=np.float64):
+    """Run swforward_1shot! (acou_forward.jl:22-62) on prepared inputs.
+
+    fact: (nx, nz) = dt^2 vp^2; cpml: list of two coefficient dicts (x, z);
+    src_idx/rec_idx: 1-based (n, 2) integer arrays; srctf: scaled (nt, nsrc).
+    Returns traces (nt, nrec).
+    """
+    fact = _f_order(fact, dtype)
+    nx, nz = fact.shape
+    g = _geom((nx, nz), spacing, dt, halo)
+    cx, kx = _cpml_struct(cpml[0], dtype)
+    cz, kz = _cpml_struct(cpml[1], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    rc = _fn("oracle_cd2d_forward", dtype)(
+        ctypes.byref(g), _ptr(fact), ctypes.byref(cx), ctypes.byref(cz), ctypes.c_int(nt),
+        ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(traces))
+    assert rc == 0
+    return np.ascontiguousarray(traces)
+
+
+def cd2d_gradient(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, adjsrc,
+                  check_freq, dtype=np.float64):
+    """Run swgradient_1shot! (acou_gradient.jl:4-95) up to the raw grad_vp
+    field (before muting and the 2/vp^3 rescale).  adjsrc = -dchi/du.
+    Returns (raw_grad (nx, nz), traces (nt, nrec))."""
+    fact = _f_order(fact, dtype)
+    nx, nz = fact.shape
+    g = _geom((nx, nz), spacing, dt, halo)
+    cx, kx = _cpml_struct(cpml[0], dtype)
+    cz, kz = _cpml_struct(cpml[1], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    grad = np.zeros((nx, nz), dtype=dtype, order="F")
+    rc = _fn("oracle_cd2d_gradient", dtype)(
+        ctypes.byref(g), _ptr(fact), ctypes.byref(cx), ctypes.byref(cz), ctypes.c_int(nt),
+        ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec),
+        _ptr(adj), ctypes.c_int(check_freq), _ptr(grad), _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return np.array(grad), np.ascontiguousarray(traces)
+
+
+# ---------------------------------------------------------------------------
+# Whole-API restatements (numpy inputs) used by the parity tests
+# ---------------------------------------------------------------------------
+def acoustic_cd_forward(vp, spacing, dt, nt, halo, rcoef, freetop, shots, dtype=np.float64):
+    """run_swforward! (src/apis/forward.jl:161-199) for CD 2D.
+
+    shots: list of dicts {src_pos (nsrc,2), srctf (nt,nsrc), f0, rec_pos (nrec,2)}.
+    Returns a list of seismograms (nt, nrec)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2                                  # acou_models.jl:60
+    out = []
+    for sh in shots:
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_cd(sh["srctf"], spacing, dt, vp, si, T)
+        out.append(cd2d_forward(fact, spacing, dt, halo, cpml, nt, si, stf, ri, T))
+    return out
+
+
+def acoustic_cd_gradient(vp, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1,
+                         mute_radius_src=0, mute_radius_rec=0, dtype=np.float64):
+    """run_swgradient! (src/apis/gradient.jl:93-136) for CD 2D with L2 misfits.
+
+    Each shot dict also carries `observed` (nt, nrec) and optionally `invcov`.
+    Returns (gradient dict {"vp": ...}, total misfit, list of seismograms)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2
+    tot = np.zeros(vp.shape, dtype=T)
+    misfit = 0.0
+    seis = []
+    for sh in shots:
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_cd(sh["srctf"], spacing, dt, vp, si, T)
+        syn = cd2d_forward(fact, spacing, dt, halo, cpml, nt, si, stf, ri, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))          # acou_gradient.jl:48
+        g, tr = cd2d_gradient(fact, spacing, dt, halo, cpml, nt, si, stf, ri, adj.astype(T), check_freq, T)
+        g = mute_around_points(g, sh["src_pos"], spacing, mute_radius_src)
+        g = mute_around_points(g, sh["rec_pos"], spacing, mute_radius_rec)
+        g = (T(2.0) / vp ** 3) * g                                        # acou_gradient.jl:93
+        tot += g
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return {"vp": tot}, misfit, seis

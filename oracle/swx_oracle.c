@@ -1,0 +1,34 @@
+/*
+ * FP64 oracle (and same-precision FP32 CPU baseline) for the SeismicWaves.jl
+ * finite-difference path.  TEST INFRASTRUCTURE: loaded only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
+ * product library.  Build: oracle/Makefile (gcc -O3 -fopenmp).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+typedef struct {
+  int64_t n[3];   /* grid size, x (fast axis) first                         */
+  double h[3];    /* grid spacing                                           */
+  double dt;
+  int halo;       /* C-PML thickness in cells                               */
+  int freetop;    /* free surface on the last axis (elastic only)           */
+} oracle_geom;
+
+#define real double
+#define SFX(x) x##_f64
+typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
+#include "oracle_impl.h"
+#undef real
+#undef SFX
+
+#define real float
+#define SFX(x) x##_f32
+typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
+#include "oracle_impl.h"
+#undef real
+#undef SFX
+
+int oracle_abi_version(void) { return 1; }

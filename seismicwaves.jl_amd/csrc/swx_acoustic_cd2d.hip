@@ -1,0 +1,539 @@
+// 2D acoustic constant-density engine: fused C-PML time step, point
+// (source/receiver) kernel, checkpoint/replay driver and gradient
+// correlation for a batch of shots.
+//
+// Reference semantics (paths relative to the reference root):
+//   step       src/models/acoustic/backends/shared/acoustic2D_xPU.jl:1-169
+//   stencils   src/utils/fdgen.jl:137-193 (order 2)
+//   driver     src/models/acoustic/acou_gradient.jl:4-95
+//   correlate  src/models/acoustic/backends/shared/correlate_gradient_xPU.jl:1-10
+//
+// Device layout (x fastest, exactly the Julia layout), one slab per shot:
+//   p      [3][ns][nx*nz]  rotating time levels, p^k lives in slot k mod 3
+//                          (the reference swaps pold/pcur/pnew, :122-124)
+//   psi_x  [2][ns][2h*nz]  double buffered by step parity so a cell can
+//   psi_z  [2][ns][nx*2h]  recompute its left neighbour's new value
+//   xi_x   [ns][(2h+2)*nz], xi_z [ns][nx*(2h+2)]
+//   D      [S][ns][nx*nz]  p^{it-2} - 2 p^{it-1} + p^{it} of the segment
+//                          being replayed (what correlate_gradient! needs)
+//   ckpt   [K-2][...]      p^{c-1}, p^c, psi, xi at c = k*S
+//   grad   [ns][nx*nz]     per-shot raw gradient (reduced in shot order)
+#include <algorithm>
+
+#include "swx_internal.hpp"
+
+namespace swx {
+namespace {
+
+constexpr int TX = 64;  // one wavefront along the fast axis
+constexpr int TZ = 4;
+
+template <typename T>
+struct Cd2dStepArgs {
+  int nx, nz, h;
+  int64_t N;                 // nx*nz
+  T idx, idz, idx2, idz2;
+  const T *__restrict__ fact;
+  const T *pold;             // shot 0 slab of the three time levels
+  const T *pcur;
+  T *pnew;
+  const T *psix_o; T *psix_n; int64_t spx;   // psi_x old/new, per-shot stride
+  const T *psiz_o; T *psiz_n; int64_t spz;
+  T *xix; int64_t sxx;
+  T *xiz; int64_t sxz;
+  const T *cpml; int cstride;                // per-shot packed coefficients
+  const int *nt_shot;
+  int it;                    // time step index (1-based)
+  int act_min;               // shot active iff it <= nt_s and nt_s > act_min
+  // D store: dmode 0 none, 1 forward last segment, 2 replay slot it-cbase-1
+  T *dbuf; int64_t dslot; int dmode; int S; int cbase;
+  // correlation of the current (adjoint) field with D slot it-cbase
+  T *grad; int corr; int corr_lim;
+};
+
+// Packed coefficient block of one shot: for axis x then z,
+//   a (2h+2) | a_h (2h) | b (2h+2) | b_h (2h)
+template <typename T>
+struct AxisCoef {
+  const T *a, *ah, *b, *bh;
+};
+template <typename T>
+__device__ __forceinline__ AxisCoef<T> axis_coef(const T *base, int h, int axis) {
+  const T *p = base + (int64_t)axis * (8 * h + 4);
+  return AxisCoef<T>{p, p + (2 * h + 2), p + (4 * h + 2), p + (6 * h + 4)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int i = blockIdx.x * TX + threadIdx.x;
+  const int j = blockIdx.y * TZ + threadIdx.y;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (i >= nx || j >= nz) return;
+  const int64_t so = (int64_t)s * a.N;
+  const int64_t c = (int64_t)j * nx + i;
+  const T *pc_ = a.pcur + so;
+  const T pc = pc_[c];
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0);
+  const AxisCoef<T> cz = axis_coef(cb, h, 1);
+
+  // psi_x: cell i owns strip index kx (update_psi_x!, acoustic2D_xPU.jl:1-12)
+  int kx = -1;
+  if (i < h) kx = i;
+  else if (i >= nx - h - 1 && i <= nx - 2) kx = i + 1 - nx + 2 * h;
+  T psx = 0;
+  if (kx >= 0) {
+    const int64_t q = (int64_t)s * a.spx + (int64_t)j * (2 * h) + kx;
+    const T d = (pc_[c + 1] - pc) * a.idx;
+    psx = cx.bh[kx] * a.psix_o[q] + cx.ah[kx] * d;
+    a.psix_n[q] = psx;
+  }
+  // psi_z (update_psi_y!, acoustic2D_xPU.jl:14-25)
+  int kz = -1;
+  if (j < h) kz = j;
+  else if (j >= nz - h - 1 && j <= nz - 2) kz = j + 1 - nz + 2 * h;
+  T psz = 0;
+  if (kz >= 0) {
+    const int64_t q = (int64_t)s * a.spz + (int64_t)kz * nx + i;
+    const T d = (pc_[c + nx] - pc) * a.idz;
+    psz = cz.bh[kz] * a.psiz_o[q] + cz.ah[kz] * d;
+    a.psiz_n[q] = psz;
+  }
+
+  const bool interior = (i >= 1 && i <= nx - 2 && j >= 1 && j <= nz - 2);
+  const T *po_ = a.pold + so;
+  T *pn_ = a.pnew + so;
+  T pnv = 0;
+  if (interior) {
+    // update_p_CPML! (acoustic2D_xPU.jl:27-46)
+    const T pl = pc_[c - 1], pr = pc_[c + 1], pd = pc_[c - nx], pu = pc_[c + nx];
+    const T d2x = (pl - T(2) * pc + pr) * a.idx2;
+    const T d2z = (pd - T(2) * pc + pu) * a.idz2;
+    T lx = d2x, lz = d2z;
+    if (i < h || i >= nx - h) {
+      // left neighbour's new psi, recomputed from the old buffer
+      const int kn = kx - 1;
+      const int64_t qn = (int64_t)s * a.spx + (int64_t)j * (2 * h) + kn;
+      const T psn = cx.bh[kn] * a.psix_o[qn] + cx.ah[kn] * ((pc - pl) * a.idx);
+      const T dpsi = (psx - psn) * a.idx;
+      const int64_t qx = (int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + kx;
+      const T xi = cx.b[kx] * a.xix[qx] + cx.a[kx] * (d2x + dpsi);
+      a.xix[qx] = xi;
+      lx = d2x + dpsi + xi;
+    }
+    if (j < h || j >= nz - h) {
+      const int kn = kz - 1;
+      const int64_t qn = (int64_t)s * a.spz + (int64_t)kn * nx + i;
+      const T psn = cz.bh[kn] * a.psiz_o[qn] + cz.ah[kn] * ((pc - pd) * a.idz);
+      const T dpsi = (psz - psn) * a.idz;
+      const int64_t qz = (int64_t)s * a.sxz + (int64_t)kz * nx + i;
+      const T xi = cz.b[kz] * a.xiz[qz] + cz.a[kz] * (d2z + dpsi);
+      a.xiz[qz] = xi;
+      lz = d2z + dpsi + xi;
+    }
+    pnv = T(2) * pc - po_[c] + a.fact[c] * (lx + lz);
+    pn_[c] = pnv;
+  }
+  if (a.dmode) {
+    int slot = -1;
+    if (a.dmode == 1) {
+      const int seg0 = ((nts - 1) / a.S) * a.S;
+      if (a.it > seg0) slot = a.it - seg0 - 1;
+    } else {
+      slot = a.it - a.cbase - 1;
+    }
+    if (slot >= 0) {
+      const T pn = interior ? pnv : pn_[c];
+      a.dbuf[(int64_t)slot * a.dslot + so + c] = (po_[c] - T(2) * pc) + pn;
+    }
+  }
+  if (a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim) {
+    // correlate_gradient! of the previous adjoint step (adjcur = pcur here)
+    const int slot = a.it - a.cbase;
+    a.grad[so + c] += pc * a.dbuf[(int64_t)slot * a.dslot + so + c];
+  }
+}
+
+template <typename T>
+struct Cd2dPointArgs {
+  PointArgsView pt;
+  int64_t N;
+  const T *srctf;
+  T *traces;        // forward: record (nullable)
+  const T *res;     // adjoint: residuals (nullable)
+  const T *fact;    // prescale of the residuals at the receivers
+  T *pnew; const T *pcur; const T *pold;
+  int it, act_min;
+  T *dbuf; int64_t dslot; int dmode; int S; int cbase;
+};
+
+// inject_sources! + record_receivers! (acoustic2D_xPU.jl:48-62) for the
+// forward/replay step, or the residual injection of adjoint_onestep_CPML!
+// with prescale_residuals! folded in (acoustic2D_xPU.jl:64-76, :161-162).
+template <typename T>
+__global__ __launch_bounds__(256) void cd2d_point_kernel(Cd2dPointArgs<T> a) {
+  const int s = blockIdx.x;
+  const int nts = a.pt.nt[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int64_t so = (int64_t)s * a.N;
+  T *pn = a.pnew + so;
+  const int tid = threadIdx.x;
+  if (a.res == nullptr) {
+    int slot = -1;
+    if (a.dmode == 1) {
+      const int seg0 = ((nts - 1) / a.S) * a.S;
+      if (a.it > seg0) slot = a.it - seg0 - 1;
+    } else if (a.dmode == 2) {
+      slot = a.it - a.cbase - 1;
+    }
+    const int u0 = a.pt.usrc_off[s], u1 = a.pt.usrc_off[s + 1];
+    const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
+    for (int u = u0 + tid; u < u1; u += blockDim.x) {
+      const int64_t cell = a.pt.usrc_cell[u];
+      T v = pn[cell];
+      for (int q = a.pt.usrc_ptr[u]; q < a.pt.usrc_ptr[u + 1]; q++) v += tf[(int64_t)nts * a.pt.usrc_list[q]];
+      pn[cell] = v;
+      if (slot >= 0) a.dbuf[(int64_t)slot * a.dslot + so + cell] = (a.pold[so + cell] - T(2) * a.pcur[so + cell]) + v;
+    }
+    if (a.traces) {
+      __syncthreads();
+      const int r0 = a.pt.rec_off[s], r1 = a.pt.rec_off[s + 1];
+      T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
+      for (int r = r0 + tid; r < r1; r += blockDim.x) tr[(int64_t)nts * (r - r0)] = pn[a.pt.rec_cell[r]];
+    }
+  } else {
+    const int u0 = a.pt.urec_off[s], u1 = a.pt.urec_off[s + 1];
+    const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
+    for (int u = u0 + tid; u < u1; u += blockDim.x) {
+      const int64_t cell = a.pt.urec_cell[u];
+      const T f = a.fact[cell];
+      T v = pn[cell];
+      for (int q = a.pt.urec_ptr[u]; q < a.pt.urec_ptr[u + 1]; q++) v += rs[(int64_t)nts * a.pt.urec_list[q]] * f;
+      pn[cell] = v;
+    }
+  }
+}
+
+// grad += adjcur * D for the first step of a segment (the other steps are
+// fused into the adjoint stencil).
+template <typename T>
+__global__ void cd2d_correlate_kernel(T *grad, const T *adj, const T *d, int64_t N, const int *nt_shot, int cbase) {
+  const int s = blockIdx.y;
+  if (nt_shot[s] <= cbase) return;
+  const int64_t so = (int64_t)s * N;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x)
+    grad[so + q] += adj[so + q] * d[so + q];
+}
+
+// Sum of the per-shot gradients in shot order times 1/dt^2, or a scaled copy
+// per shot.
+template <typename T>
+__global__ void reduce_shots_kernel(T *out, const T *grad, int64_t N, int ns, T scale, int per_shot) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    if (per_shot) {
+      for (int s = 0; s < ns; s++) out[(int64_t)s * N + q] = grad[(int64_t)s * N + q] * scale;
+    } else {
+      T acc = 0;
+      for (int s = 0; s < ns; s++) acc += grad[(int64_t)s * N + q] * scale;
+      out[q] = acc;
+    }
+  }
+}
+
+template <typename T>
+class Cd2dEngine final : public Engine {
+ public:
+  explicit Cd2dEngine(PlanCore &core) : c_(core) {
+    const swx_plan_desc &d = c_.d;
+    nx_ = (int)d.n[0];
+    nz_ = (int)d.n[1];
+    h_ = d.halo;
+    ns_ = d.max_shots;
+    N_ = (int64_t)nx_ * nz_;
+    spx_ = (int64_t)2 * h_ * nz_;
+    spz_ = (int64_t)nx_ * 2 * h_;
+    sxx_ = (int64_t)(2 * h_ + 2) * nz_;
+    sxz_ = (int64_t)nx_ * (2 * h_ + 2);
+    cstride_ = 2 * (8 * h_ + 4);
+    const size_t e = sizeof(T);
+    fact_.alloc(N_ * e);
+    p_.alloc(3 * ns_ * N_ * e);
+    psix_.alloc(std::max<int64_t>(2 * ns_ * spx_, 1) * e);
+    psiz_.alloc(std::max<int64_t>(2 * ns_ * spz_, 1) * e);
+    xix_.alloc(ns_ * sxx_ * e);
+    xiz_.alloc(ns_ * sxz_ * e);
+    cpml_.alloc((int64_t)ns_ * cstride_ * e);
+    state_bytes_ = fact_.bytes() + p_.bytes() + psix_.bytes() + psiz_.bytes() + xix_.bytes() + xiz_.bytes();
+    if (d.gradient) {
+      a_.alloc(3 * ns_ * N_ * e);
+      apsix_.alloc(psix_.bytes());
+      apsiz_.alloc(psiz_.bytes());
+      axix_.alloc(xix_.bytes());
+      axiz_.alloc(xiz_.bytes());
+      grad_.alloc(ns_ * N_ * e);
+      gout_.alloc(ns_ * N_ * e);
+      state_bytes_ += a_.bytes() + apsix_.bytes() + apsiz_.bytes() + axix_.bytes() +
+                      axiz_.bytes() + grad_.bytes() + gout_.bytes();
+      const int64_t slot = ns_ * N_ * (int64_t)e;
+      ck_bytes_one_ = (2 * N_ + spx_ + spz_ + sxx_ + sxz_) * ns_ * (int64_t)e;
+      int64_t avail = (int64_t)c_.budget - state_bytes_;
+      S_ = choose_segment(d.nt_max, d.check_freq, slot, ck_bytes_one_, avail);
+      K_ = (int)ceil_div(d.nt_max, S_);
+      dbuf_.alloc((int64_t)S_ * slot);
+      nck_ = std::max(K_ - 2, 0);
+      ck_.alloc(std::max<int64_t>(nck_ * ck_bytes_one_, 1));
+    }
+    zero_forward_state();
+  }
+
+  void set_model(int nfields, const void *const *fields) override {
+    SWX_REQUIRE(nfields == 1 && fields[0], SWX_EINVAL, "acoustic CD expects one model field (fact)");
+    SWX_HIP(hipMemcpyAsync(fact_.get(), fields[0], N_ * sizeof(T), hipMemcpyHostToDevice, c_.stream));
+    SWX_HIP(hipStreamSynchronize(c_.stream));
+    model_set_ = true;
+  }
+
+  void on_shots_set(int nshots, const swx_shot_desc *shots) override {
+    std::vector<T> hc((size_t)nshots * cstride_);
+    const int la = 2 * h_ + 2, lh = 2 * h_;
+    for (int s = 0; s < nshots; s++) {
+      T *dst = hc.data() + (size_t)s * cstride_;
+      for (int ax = 0; ax < 2; ax++) {
+        const int len[4] = {la, lh, la, lh};
+        T *p = dst + ax * (8 * h_ + 4);
+        for (int k = 0; k < 4; k++) {
+          const T *src = static_cast<const T *>(shots[s].cpml[ax][k]);
+          SWX_REQUIRE(src || len[k] == 0, SWX_EINVAL, "missing C-PML coefficient array");
+          if (len[k]) std::memcpy(p, src, sizeof(T) * len[k]);
+          p += len[k];
+        }
+      }
+      SWX_REQUIRE(shots[s].nt <= c_.d.nt_max, SWX_EINVAL, "shot nt exceeds nt_max");
+    }
+    SWX_HIP(hipMemcpyAsync(cpml_.get(), hc.data(), hc.size() * sizeof(T), hipMemcpyHostToDevice, c_.stream));
+    SWX_HIP(hipStreamSynchronize(c_.stream));
+    nsh_ = nshots;
+    nt_ = c_.pts.nt;
+    ntmax_ = *std::max_element(nt_.begin(), nt_.end());
+  }
+
+  void forward(bool save) override {
+    SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
+    zero_forward_state();
+    T *tr = c_.pts.d_traces.as<T>();
+    for (int it = 1; it <= ntmax_; it++) {
+      step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0);
+      point_fwd(it, 0, tr, save ? 1 : 0, 0);
+      if (save && it % S_ == 0) {
+        const int k = it / S_;
+        if (k >= 1 && k <= K_ - 2) save_ckpt(k, it);
+      }
+    }
+  }
+
+  void adjoint(bool per_shot) override {
+    (void)per_shot;
+    const int64_t e = sizeof(T);
+    SWX_HIP(hipMemsetAsync(a_.get(), 0, a_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(apsix_.get(), 0, apsix_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(apsiz_.get(), 0, apsiz_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(axix_.get(), 0, axix_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(axiz_.get(), 0, axiz_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
+    const int Kmax = (int)ceil_div(ntmax_, S_);
+    for (int k = Kmax - 1; k >= 0; k--) {
+      const int cb = k * S_;
+      // replay for shots whose segment k is not their last one
+      bool need = false;
+      for (int s = 0; s < nsh_; s++) need |= nt_[s] > cb + S_;
+      if (need) {
+        restore_ckpt(k, cb);
+        for (int it = cb + 1; it <= cb + S_; it++) {
+          step_fwd(it, cb + S_, 2, cb);
+          point_fwd(it, cb + S_, nullptr, 2, cb);
+        }
+      }
+      const int top = std::min(cb + S_, ntmax_);
+      for (int it = top; it >= cb + 1; it--) {
+        step_adj(it, cb);
+        point_adj(it);
+      }
+      // correlation of the first step of the segment
+      dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 1024), nsh_);
+      cd2d_correlate_kernel<T><<<g, 256, 0, c_.stream>>>(grad_.as<T>(), a_.as<T>() + mod3(cb + 1) * ns_ * N_,
+                                                          dbuf_.as<T>(), N_, c_.pts.d_nt.as<int>(), cb);
+      SWX_HIP(hipGetLastError());
+    }
+    const T scale = T(1.0 / (c_.d.dt * c_.d.dt));
+    dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 4096));
+    reduce_shots_kernel<T><<<g, 256, 0, c_.stream>>>(gout_.as<T>(), grad_.as<T>(), N_, nsh_, scale, per_shot ? 1 : 0);
+    SWX_HIP(hipGetLastError());
+    (void)e;
+  }
+
+  void copy_grad(void *host, bool per_shot) override {
+    const size_t bytes = (per_shot ? (size_t)nsh_ : 1) * N_ * sizeof(T);
+    SWX_HIP(hipMemcpy(host, gout_.get(), bytes, hipMemcpyDeviceToHost));
+  }
+
+  int64_t bytes_state() const override { return state_bytes_; }
+  int64_t bytes_checkpoints() const override { return ck_.bytes(); }
+  int64_t bytes_replay() const override { return dbuf_.bytes(); }
+  int segment() const override { return S_; }
+
+ private:
+  T *P(int k) const { return p_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
+  T *A(int k) const { return a_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
+
+  void zero_forward_state() {
+    SWX_HIP(hipMemsetAsync(p_.get(), 0, p_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(psix_.get(), 0, psix_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(psiz_.get(), 0, psiz_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(xix_.get(), 0, xix_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(xiz_.get(), 0, xiz_.bytes(), c_.stream));
+  }
+
+  Cd2dStepArgs<T> base_args(int it) const {
+    Cd2dStepArgs<T> a{};
+    a.nx = nx_; a.nz = nz_; a.h = h_; a.N = N_;
+    a.idx = T(1.0 / c_.d.h[0]);
+    a.idz = T(1.0 / c_.d.h[1]);
+    a.idx2 = a.idx * a.idx;
+    a.idz2 = a.idz * a.idz;
+    a.fact = fact_.as<T>();
+    a.spx = spx_; a.spz = spz_; a.sxx = sxx_; a.sxz = sxz_;
+    a.cpml = cpml_.as<T>(); a.cstride = cstride_;
+    a.nt_shot = c_.pts.d_nt.as<int>();
+    a.it = it;
+    a.dbuf = dbuf_.as<T>(); a.dslot = (int64_t)ns_ * N_; a.S = S_ > 0 ? S_ : 1;
+    return a;
+  }
+
+  void launch(const Cd2dStepArgs<T> &a) {
+    dim3 blk(TX, TZ);
+    dim3 grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+    c_.ktimer.begin(c_.stream);
+    cd2d_step_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    c_.ktimer.end(c_.stream);
+    SWX_HIP(hipGetLastError());
+  }
+
+  void step_fwd(int it, int act_min, int dmode, int cbase) {
+    Cd2dStepArgs<T> a = base_args(it);
+    a.pold = P(it - 2); a.pcur = P(it - 1); a.pnew = P(it);
+    a.psix_o = psix_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spx_;
+    a.psix_n = psix_.as<T>() + (int64_t)(it & 1) * ns_ * spx_;
+    a.psiz_o = psiz_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spz_;
+    a.psiz_n = psiz_.as<T>() + (int64_t)(it & 1) * ns_ * spz_;
+    a.xix = xix_.as<T>(); a.xiz = xiz_.as<T>();
+    a.act_min = act_min;
+    a.dmode = c_.d.gradient ? dmode : 0;
+    a.cbase = cbase;
+    a.corr = 0;
+    launch(a);
+  }
+
+  void step_adj(int it, int cbase) {
+    Cd2dStepArgs<T> a = base_args(it);
+    a.pold = A(it + 2); a.pcur = A(it + 1); a.pnew = A(it);
+    a.psix_o = apsix_.as<T>() + (int64_t)((it + 1) & 1) * ns_ * spx_;
+    a.psix_n = apsix_.as<T>() + (int64_t)(it & 1) * ns_ * spx_;
+    a.psiz_o = apsiz_.as<T>() + (int64_t)((it + 1) & 1) * ns_ * spz_;
+    a.psiz_n = apsiz_.as<T>() + (int64_t)(it & 1) * ns_ * spz_;
+    a.xix = axix_.as<T>(); a.xiz = axiz_.as<T>();
+    a.act_min = 0;
+    a.dmode = 0;
+    a.cbase = cbase;
+    a.grad = grad_.as<T>();
+    a.corr = 1;
+    a.corr_lim = cbase + S_;
+    launch(a);
+  }
+
+  void point_fwd(int it, int act_min, T *traces, int dmode, int cbase) {
+    Cd2dPointArgs<T> a{};
+    a.pt = view(c_.pts);
+    a.N = N_;
+    a.srctf = c_.pts.d_srctf.as<T>();
+    a.traces = traces;
+    a.res = nullptr;
+    a.fact = fact_.as<T>();
+    a.pnew = P(it); a.pcur = P(it - 1); a.pold = P(it - 2);
+    a.it = it; a.act_min = act_min;
+    a.dbuf = dbuf_.as<T>(); a.dslot = (int64_t)ns_ * N_;
+    a.dmode = c_.d.gradient ? dmode : 0; a.S = S_ > 0 ? S_ : 1; a.cbase = cbase;
+    cd2d_point_kernel<T><<<nsh_, 256, 0, c_.stream>>>(a);
+    SWX_HIP(hipGetLastError());
+  }
+
+  void point_adj(int it) {
+    Cd2dPointArgs<T> a{};
+    a.pt = view(c_.pts);
+    a.N = N_;
+    a.res = c_.pts.d_res.as<T>();
+    a.fact = fact_.as<T>();
+    a.pnew = A(it); a.pcur = A(it + 1); a.pold = A(it + 2);
+    a.it = it; a.act_min = 0;
+    cd2d_point_kernel<T><<<nsh_, 256, 0, c_.stream>>>(a);
+    SWX_HIP(hipGetLastError());
+  }
+
+  // checkpoint layout (all shots): p^{c-1} | p^c | psi_x | psi_z | xi_x | xi_z
+  void ckpt_copy(int k, int cb, bool save) {
+    const int64_t e = sizeof(T);
+    char *base = ck_.as<char>() + (int64_t)(k - 1) * ck_bytes_one_;
+    const int64_t fb = (int64_t)ns_ * N_ * e;
+    struct Part { void *dev; int64_t bytes; };
+    Part parts[6] = {
+        {P(cb - 1), fb},
+        {P(cb), fb},
+        {psix_.as<T>() + (int64_t)(cb & 1) * ns_ * spx_, ns_ * spx_ * e},
+        {psiz_.as<T>() + (int64_t)(cb & 1) * ns_ * spz_, ns_ * spz_ * e},
+        {xix_.get(), ns_ * sxx_ * e},
+        {xiz_.get(), ns_ * sxz_ * e},
+    };
+    int64_t off = 0;
+    for (auto &pt : parts) {
+      if (pt.bytes) {
+        if (save)
+          SWX_HIP(hipMemcpyAsync(base + off, pt.dev, pt.bytes, hipMemcpyDeviceToDevice, c_.stream));
+        else
+          SWX_HIP(hipMemcpyAsync(pt.dev, base + off, pt.bytes, hipMemcpyDeviceToDevice, c_.stream));
+      }
+      off += pt.bytes;
+    }
+  }
+  void save_ckpt(int k, int it) { ckpt_copy(k, it, true); }
+  void restore_ckpt(int k, int cb) {
+    if (k == 0) {
+      zero_forward_state();
+    } else {
+      ckpt_copy(k, cb, false);
+    }
+  }
+
+  PlanCore &c_;
+  int nx_, nz_, h_, ns_;
+  int64_t N_, spx_, spz_, sxx_, sxz_;
+  int cstride_;
+  DevBuf fact_, p_, psix_, psiz_, xix_, xiz_, cpml_;
+  DevBuf a_, apsix_, apsiz_, axix_, axiz_, grad_, gout_;
+  DevBuf dbuf_, ck_;
+  int64_t ck_bytes_one_ = 0, state_bytes_ = 0;
+  int S_ = 1, K_ = 1, nck_ = 0;
+  int nsh_ = 0, ntmax_ = 0;
+  std::vector<int> nt_;
+  bool model_set_ = false;
+};
+
+}  // namespace
+
+std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core) {
+  if (core.d.dtype == SWX_F32) return std::make_unique<Cd2dEngine<float>>(core);
+  return std::make_unique<Cd2dEngine<double>>(core);
+}
+
+}  // namespace swx

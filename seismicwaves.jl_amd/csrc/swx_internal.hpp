@@ -1,0 +1,183 @@
+// Internal declarations shared by the translation units of libswx.so.
+// Not part of the public ABI (that is include/swx.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "swx.h"
+
+namespace swx {
+
+// Error carrying an swx_status; converted to a return code at the C boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define SWX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::swx::Error(SWX_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SWX_REQUIRE(cond, code, msg)                 \
+  do {                                               \
+    if (!(cond)) throw ::swx::Error((code), (msg));  \
+  } while (0)
+
+// Owning device allocation (hipMalloc / hipFree).
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(&p_, bytes);
+    if (e != hipSuccess) {
+      p_ = nullptr;
+      throw Error(SWX_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n_ = bytes;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(p_); }
+  void *get() const { return p_; }
+  size_t bytes() const { return n_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Per-batch source/receiver tables shared by the acoustic engines.
+// Sources and receivers that fall on the same grid cell are grouped so that
+// one thread adds them in the reference's serial order (inject_sources!,
+// acoustic2D_xPU.jl:48-54, runs over sources in index order).
+struct PointTables {
+  int nshots = 0;
+  std::vector<int> nt, nsrc, nrec;
+  std::vector<int64_t> off_tf, off_tr;    // offsets into srctf / traces
+  int64_t total_tf = 0, total_tr = 0;
+  // device arrays
+  DevBuf d_nt;        // int32 [ns]
+  DevBuf d_off_tf;    // int64 [ns]
+  DevBuf d_off_tr;    // int64 [ns]
+  DevBuf d_usrc_off;  // int32 [ns+1]  unique source cells per shot
+  DevBuf d_usrc_cell; // int64 [*]
+  DevBuf d_usrc_ptr;  // int32 [*+1]  into d_usrc_list (global)
+  DevBuf d_usrc_list; // int32 [*]    source column within its shot
+  DevBuf d_urec_off;  // same for receivers (adjoint injection)
+  DevBuf d_urec_cell;
+  DevBuf d_urec_ptr;
+  DevBuf d_urec_list;
+  DevBuf d_rec_off;   // int32 [ns+1] receivers per shot (recording)
+  DevBuf d_rec_cell;  // int64 [*]
+  DevBuf d_srctf;     // T [total_tf]
+  DevBuf d_traces;    // T [total_tr]
+  DevBuf d_res;       // T [total_tr] (adjoint sources)
+  std::vector<int64_t> h_rec_cell;  // host copies (for validation/debug)
+};
+
+struct PointArgsView {
+  const int *nt;
+  const int64_t *off_tf, *off_tr;
+  const int *usrc_off; const int64_t *usrc_cell; const int *usrc_ptr; const int *usrc_list;
+  const int *urec_off; const int64_t *urec_cell; const int *urec_ptr; const int *urec_list;
+  const int *rec_off; const int64_t *rec_cell;
+};
+
+inline PointArgsView view(const PointTables &t) {
+  return PointArgsView{t.d_nt.as<int>(), t.d_off_tf.as<int64_t>(), t.d_off_tr.as<int64_t>(),
+                       t.d_usrc_off.as<int>(), t.d_usrc_cell.as<int64_t>(), t.d_usrc_ptr.as<int>(),
+                       t.d_usrc_list.as<int>(), t.d_urec_off.as<int>(), t.d_urec_cell.as<int64_t>(),
+                       t.d_urec_ptr.as<int>(), t.d_urec_list.as<int>(), t.d_rec_off.as<int>(),
+                       t.d_rec_cell.as<int64_t>()};
+}
+
+// Builds PointTables from the ABI shot descriptors.  `grid` are the point
+// counts per axis of the field sources and receivers live on; indices are
+// validated against it (check_positions, src/utils/checks.jl:55-70).
+void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize, int nshots,
+                        const swx_shot_desc *shots, hipStream_t stream);
+
+// Event-pair timer for the dominant kernel.
+class KernelTimer {
+ public:
+  ~KernelTimer();
+  void begin(hipStream_t s);
+  void end(hipStream_t s);
+  // Sums elapsed times of recorded pairs (stream must be synchronized).
+  double collect_ms(int64_t *count);
+  void reset() { used_ = 0; }
+  bool enabled = false;
+
+ private:
+  std::vector<hipEvent_t> ev_;
+  size_t used_ = 0;
+};
+
+// Engine interface: one implementation per physics x dimension x dtype.
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  virtual void set_model(int nfields, const void *const *fields) = 0;
+  // Receives the shot descriptors after the point tables were built (the
+  // engine uploads the per-shot C-PML coefficients).
+  virtual void on_shots_set(int nshots, const swx_shot_desc *shots) = 0;
+  virtual void forward(bool save_state) = 0;
+  // Runs the adjoint loop on the device (residuals already in pts.d_res).
+  virtual void adjoint(bool per_shot) = 0;
+  // Copies the gradient fields of the last adjoint() to host memory.
+  virtual void copy_grad(void *grad_host, bool per_shot) = 0;
+  virtual int64_t bytes_state() const = 0;
+  virtual int64_t bytes_checkpoints() const = 0;
+  virtual int64_t bytes_replay() const = 0;
+  virtual int segment() const = 0;
+};
+
+struct PlanCore {
+  swx_plan_desc d{};
+  hipStream_t stream = nullptr;
+  int esize = 4;
+  PointTables pts;
+  KernelTimer ktimer;
+  bool forward_done = false;
+  double ms_forward = 0, ms_adjoint = 0, ms_stencil = 0;
+  int64_t n_stencil = 0;
+  size_t budget = 0;  // bytes
+};
+
+std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core);
+std::unique_ptr<Engine> make_acoustic_vd2d(PlanCore &core);
+
+// Helpers
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int mod3(int k) { return ((k % 3) + 3) % 3; }
+
+// Chooses the replay segment length S for linear checkpointing given the
+// bytes per replay slot (all shots) and per checkpoint (all shots), the
+// user check_freq (1 = keep every step) and the memory still available.
+int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail);
+
+}  // namespace swx

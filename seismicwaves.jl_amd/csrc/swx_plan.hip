@@ -1,0 +1,354 @@
+// C ABI entry points of libswx.so (declared in include/swx.h): plan life
+// cycle, validation, shot tables and dispatch to the physics engines.
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <numeric>
+
+#include "swx_internal.hpp"
+
+struct swx_plan {
+  swx::PlanCore core;
+  std::unique_ptr<swx::Engine> engine;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SWX_OK;
+  } catch (const swx::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SWX_EINVAL;
+  }
+}
+}  // namespace
+
+namespace swx {
+
+KernelTimer::~KernelTimer() {
+  for (auto e : ev_) (void)hipEventDestroy(e);
+}
+
+void KernelTimer::begin(hipStream_t s) {
+  if (!enabled) return;
+  if (used_ + 2 > ev_.size()) {
+    for (int k = 0; k < 2; k++) {
+      hipEvent_t e;
+      SWX_HIP(hipEventCreate(&e));
+      ev_.push_back(e);
+    }
+  }
+  SWX_HIP(hipEventRecord(ev_[used_], s));
+}
+
+void KernelTimer::end(hipStream_t s) {
+  if (!enabled) return;
+  SWX_HIP(hipEventRecord(ev_[used_ + 1], s));
+  used_ += 2;
+}
+
+double KernelTimer::collect_ms(int64_t *count) {
+  double tot = 0;
+  for (size_t k = 0; k + 1 < used_; k += 2) {
+    float ms = 0;
+    SWX_HIP(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
+    tot += ms;
+  }
+  if (count) *count = (int64_t)(used_ / 2);
+  used_ = 0;
+  return tot;
+}
+
+int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail) {
+  auto mem = [&](int64_t S) {
+    int64_t K = ceil_div(nt_max, S);
+    return S * slot_bytes + std::max<int64_t>(K - 2, 0) * ckpt_bytes;
+  };
+  int64_t S = check_freq <= 1 ? nt_max : std::min<int64_t>(check_freq, nt_max);
+  if (mem(S) <= avail) return (int)S;
+  // Not enough memory for the requested interval: take the interval with the
+  // smallest footprint (replay is deterministic, so results do not change).
+  int64_t best = 1, bestm = mem(1);
+  for (int64_t s = 1; s <= nt_max; s++) {
+    int64_t m = mem(s);
+    if (m < bestm) { bestm = m; best = s; }
+  }
+  if (bestm > avail)
+    throw Error(SWX_ENOMEM, "checkpoint storage does not fit the device memory budget (need " +
+                                std::to_string(bestm) + " bytes, have " + std::to_string(avail) + ")");
+  return (int)best;
+}
+
+void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize, int nshots,
+                        const swx_shot_desc *shots, hipStream_t stream) {
+  t.nshots = nshots;
+  t.nt.assign(nshots, 0);
+  t.nsrc.assign(nshots, 0);
+  t.nrec.assign(nshots, 0);
+  t.off_tf.assign(nshots, 0);
+  t.off_tr.assign(nshots, 0);
+  std::vector<int> usrc_off(1, 0), usrc_ptr(1, 0), usrc_list, urec_off(1, 0), urec_ptr(1, 0), urec_list,
+      rec_off(1, 0);
+  std::vector<int64_t> usrc_cell, urec_cell, rec_cell;
+  int64_t otf = 0, otr = 0;
+  auto linear = [&](const int64_t *idx, int n, int k, const char *what, int s) {
+    int64_t lin = 0, stride = 1;
+    for (int d = 0; d < ndim; d++) {
+      int64_t v = idx[k + (int64_t)n * d];
+      if (v < 1 || v > grid[d])
+        throw Error(SWX_EINVAL, std::string(what) + " position of shot " + std::to_string(s + 1) +
+                                    " is not inside the grid!");
+      lin += (v - 1) * stride;
+      stride *= grid[d];
+    }
+    return lin;
+  };
+  for (int s = 0; s < nshots; s++) {
+    const swx_shot_desc &sh = shots[s];
+    SWX_REQUIRE(sh.nt > 0, SWX_EINVAL, "Number of timesteps must be positive!");
+    SWX_REQUIRE(sh.nsrc > 0, SWX_EINVAL, "There must be at least one source!");
+    SWX_REQUIRE(sh.nrec > 0, SWX_EINVAL, "There must be at least one receiver!");
+    SWX_REQUIRE(sh.src_idx && sh.rec_idx && sh.srctf, SWX_EINVAL, "null source/receiver array");
+    t.nt[s] = sh.nt;
+    t.nsrc[s] = sh.nsrc;
+    t.nrec[s] = sh.nrec;
+    t.off_tf[s] = otf;
+    t.off_tr[s] = otr;
+    otf += (int64_t)sh.nt * sh.nsrc;
+    otr += (int64_t)sh.nt * sh.nrec;
+    // group sources by cell, keeping index order inside a group
+    std::map<int64_t, std::vector<int>> g;
+    std::vector<int64_t> order;
+    for (int k = 0; k < sh.nsrc; k++) {
+      int64_t c = linear(sh.src_idx, sh.nsrc, k, "Source", s);
+      if (!g.count(c)) order.push_back(c);
+      g[c].push_back(k);
+    }
+    for (int64_t c : order) {
+      usrc_cell.push_back(c);
+      for (int k : g[c]) usrc_list.push_back(k);
+      usrc_ptr.push_back((int)usrc_list.size());
+    }
+    usrc_off.push_back((int)usrc_cell.size());
+    std::map<int64_t, std::vector<int>> gr;
+    std::vector<int64_t> rorder;
+    for (int k = 0; k < sh.nrec; k++) {
+      int64_t c = linear(sh.rec_idx, sh.nrec, k, "Receiver", s);
+      rec_cell.push_back(c);
+      if (!gr.count(c)) rorder.push_back(c);
+      gr[c].push_back(k);
+    }
+    for (int64_t c : rorder) {
+      urec_cell.push_back(c);
+      for (int k : gr[c]) urec_list.push_back(k);
+      urec_ptr.push_back((int)urec_list.size());
+    }
+    urec_off.push_back((int)urec_cell.size());
+    rec_off.push_back((int)rec_cell.size());
+  }
+  t.total_tf = otf;
+  t.total_tr = otr;
+  t.h_rec_cell = rec_cell;
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+    b.alloc(std::max<size_t>(bytes, 16));
+    if (bytes) SWX_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, stream));
+  };
+  up(t.d_nt, t.nt.data(), sizeof(int) * nshots);
+  up(t.d_off_tf, t.off_tf.data(), sizeof(int64_t) * nshots);
+  up(t.d_off_tr, t.off_tr.data(), sizeof(int64_t) * nshots);
+  up(t.d_usrc_off, usrc_off.data(), sizeof(int) * usrc_off.size());
+  up(t.d_usrc_cell, usrc_cell.data(), sizeof(int64_t) * usrc_cell.size());
+  up(t.d_usrc_ptr, usrc_ptr.data(), sizeof(int) * usrc_ptr.size());
+  up(t.d_usrc_list, usrc_list.data(), sizeof(int) * usrc_list.size());
+  up(t.d_urec_off, urec_off.data(), sizeof(int) * urec_off.size());
+  up(t.d_urec_cell, urec_cell.data(), sizeof(int64_t) * urec_cell.size());
+  up(t.d_urec_ptr, urec_ptr.data(), sizeof(int) * urec_ptr.size());
+  up(t.d_urec_list, urec_list.data(), sizeof(int) * urec_list.size());
+  up(t.d_rec_off, rec_off.data(), sizeof(int) * rec_off.size());
+  up(t.d_rec_cell, rec_cell.data(), sizeof(int64_t) * rec_cell.size());
+  // source time functions: concatenated per shot
+  std::vector<char> tf((size_t)otf * esize);
+  for (int s = 0; s < nshots; s++)
+    std::memcpy(tf.data() + (size_t)t.off_tf[s] * esize, shots[s].srctf,
+                (size_t)shots[s].nt * shots[s].nsrc * esize);
+  up(t.d_srctf, tf.data(), tf.size());
+  t.d_traces.alloc(std::max<size_t>((size_t)otr * esize, 16));
+  t.d_res.alloc(std::max<size_t>((size_t)otr * esize, 16));
+  SWX_HIP(hipMemsetAsync(t.d_traces.get(), 0, (size_t)otr * esize, stream));
+  SWX_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace swx
+
+using namespace swx;
+
+extern "C" {
+
+int swx_abi_version(void) { return SWX_ABI_VERSION; }
+
+const char *swx_last_error(void) { return g_last_error.c_str(); }
+
+int swx_plan_create(const swx_plan_desc *desc, swx_plan **out) {
+  return guarded([&] {
+    SWX_REQUIRE(desc && out, SWX_EINVAL, "null argument");
+    *out = nullptr;
+    const swx_plan_desc &d = *desc;
+    SWX_REQUIRE(d.dtype == SWX_F32 || d.dtype == SWX_F64, SWX_EINVAL, "dtype must be SWX_F32 or SWX_F64");
+    SWX_REQUIRE(d.ndim == 2 || d.ndim == 3, SWX_EINVAL, "Dimensionality must be 2 or 3!");
+    for (int k = 0; k < d.ndim; k++) {
+      SWX_REQUIRE(d.n[k] > 0, SWX_EINVAL, "All numbers of grid points must be positive!");
+      SWX_REQUIRE(d.h[k] > 0, SWX_EINVAL, "All grid spacings must be positive!");
+    }
+    SWX_REQUIRE(d.dt > 0, SWX_EINVAL, "Timestep size must be positive!");
+    SWX_REQUIRE(d.nt_max > 0, SWX_EINVAL, "Number of timesteps must be positive!");
+    SWX_REQUIRE(d.halo >= 0, SWX_EINVAL, "CPML halo size must be non-negative!");
+    SWX_REQUIRE(d.max_shots > 0, SWX_EINVAL, "max_shots must be positive");
+    SWX_REQUIRE(d.check_freq >= 1, SWX_EINVAL, "check_freq must be positive");
+    for (int k = 0; k < d.ndim; k++)
+      SWX_REQUIRE(d.n[k] >= 2 * d.halo + 3, SWX_EINVAL,
+                  "Number grid points in the dimensions with C-PML boundaries must be at least 2*halo+3 = " +
+                      std::to_string(2 * d.halo + 3) + "!");
+    int ndev = 0;
+    SWX_HIP(hipGetDeviceCount(&ndev));
+    SWX_REQUIRE(d.device >= 0 && d.device < ndev, SWX_EINVAL, "invalid device ordinal");
+    SWX_HIP(hipSetDevice(d.device));
+    auto p = std::make_unique<swx_plan>();
+    p->core.d = d;
+    p->core.esize = d.dtype == SWX_F32 ? 4 : 8;
+    SWX_HIP(hipStreamCreateWithFlags(&p->core.stream, hipStreamNonBlocking));
+    size_t fr = 0, tot = 0;
+    SWX_HIP(hipMemGetInfo(&fr, &tot));
+    p->core.budget = d.mem_budget > 0 ? (size_t)d.mem_budget : (size_t)(0.9 * (double)fr);
+    if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 2) {
+      p->engine = make_acoustic_cd2d(p->core);
+    } else if (d.physics == SWX_ACOUSTIC_VD && d.ndim == 2) {
+      p->engine = make_acoustic_vd2d(p->core);
+    } else {
+      throw Error(SWX_EUNSUPPORTED, "physics/dimension combination not supported by this build");
+    }
+    *out = p.release();
+  });
+}
+
+int swx_plan_destroy(swx_plan *plan) {
+  return guarded([&] {
+    if (!plan) return;
+    (void)hipSetDevice(plan->core.d.device);
+    (void)hipStreamSynchronize(plan->core.stream);
+    plan->engine.reset();
+    hipStream_t s = plan->core.stream;
+    delete plan;
+    if (s) (void)hipStreamDestroy(s);
+  });
+}
+
+int swx_set_model(swx_plan *plan, int nfields, const void *const *fields) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && fields, SWX_EINVAL, "null argument");
+    SWX_HIP(hipSetDevice(plan->core.d.device));
+    plan->engine->set_model(nfields, fields);
+  });
+}
+
+int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && shots, SWX_EINVAL, "null argument");
+    SWX_REQUIRE(nshots >= 1 && nshots <= plan->core.d.max_shots, SWX_EINVAL,
+                "number of shots must be in 1..max_shots");
+    const swx_plan_desc &d = plan->core.d;
+    for (int s = 0; s < nshots; s++)
+      SWX_REQUIRE(shots[s].nt >= 1 && shots[s].nt <= d.nt_max, SWX_EINVAL,
+                  "shot nt must be in 1..nt_max");
+    SWX_HIP(hipSetDevice(d.device));
+    build_point_tables(plan->core.pts, d.ndim, d.n, plan->core.esize, nshots, shots, plan->core.stream);
+    plan->core.forward_done = false;
+    plan->engine->on_shots_set(nshots, shots);
+  });
+}
+
+int swx_forward(swx_plan *plan, void *traces, int save_state) {
+  return guarded([&] {
+    SWX_REQUIRE(plan, SWX_EINVAL, "null plan");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(c.pts.nshots > 0, SWX_ESTATE, "swx_set_shots must be called before swx_forward");
+    SWX_REQUIRE(!save_state || c.d.gradient, SWX_ESTATE, "plan was not created for gradients");
+    SWX_HIP(hipSetDevice(c.d.device));
+    hipEvent_t e0, e1;
+    SWX_HIP(hipEventCreate(&e0));
+    SWX_HIP(hipEventCreate(&e1));
+    c.ktimer.reset();
+    SWX_HIP(hipEventRecord(e0, c.stream));
+    plan->engine->forward(save_state != 0);
+    SWX_HIP(hipEventRecord(e1, c.stream));
+    SWX_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.ms_forward = ms;
+    if (c.ktimer.enabled) c.ms_stencil = c.ktimer.collect_ms(&c.n_stencil);
+    if (traces)
+      SWX_HIP(hipMemcpy(traces, c.pts.d_traces.get(), (size_t)c.pts.total_tr * c.esize, hipMemcpyDeviceToHost));
+    c.forward_done = save_state != 0;
+  });
+}
+
+int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && adjsrc && grad, SWX_EINVAL, "null argument");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(c.d.gradient, SWX_ESTATE, "plan was not created for gradients");
+    SWX_REQUIRE(c.forward_done, SWX_ESTATE, "swx_forward(save_state=1) must precede swx_adjoint");
+    SWX_HIP(hipSetDevice(c.d.device));
+    SWX_HIP(hipMemcpyAsync(c.pts.d_res.get(), adjsrc, (size_t)c.pts.total_tr * c.esize, hipMemcpyHostToDevice,
+                           c.stream));
+    hipEvent_t e0, e1;
+    SWX_HIP(hipEventCreate(&e0));
+    SWX_HIP(hipEventCreate(&e1));
+    c.ktimer.reset();
+    SWX_HIP(hipEventRecord(e0, c.stream));
+    plan->engine->adjoint(per_shot != 0);
+    SWX_HIP(hipEventRecord(e1, c.stream));
+    SWX_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c.ms_adjoint = ms;
+    if (c.ktimer.enabled) c.ms_stencil = c.ktimer.collect_ms(&c.n_stencil);
+    plan->engine->copy_grad(grad, per_shot != 0);
+  });
+}
+
+int swx_plan_get_info(swx_plan *plan, swx_plan_info *info) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && info, SWX_EINVAL, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->bytes_state = plan->engine->bytes_state();
+    info->bytes_checkpoints = plan->engine->bytes_checkpoints();
+    info->bytes_replay = plan->engine->bytes_replay();
+    info->segment = plan->engine->segment();
+    info->nshots = plan->core.pts.nshots;
+    info->ms_forward = plan->core.ms_forward;
+    info->ms_adjoint = plan->core.ms_adjoint;
+    info->ms_stencil = plan->core.ms_stencil;
+    info->n_stencil = plan->core.n_stencil;
+  });
+}
+
+int swx_set_kernel_timing(swx_plan *plan, int enable) {
+  return guarded([&] {
+    SWX_REQUIRE(plan, SWX_EINVAL, "null plan");
+    plan->core.ktimer.enabled = enable != 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+"""ctypes binding of libswx.so (C ABI declared in include/swx.h).
+
+The product path always runs through this library: there is no CPU
+fallback.  If the shared library is missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libswx.so")
+
+SWX_OK, SWX_EINVAL, SWX_ENOMEM, SWX_EHIP, SWX_ESTATE, SWX_EUNSUPPORTED = range(6)
+SWX_ACOUSTIC_CD, SWX_ACOUSTIC_VD, SWX_ELASTIC_ISO = 1, 2, 3
+SWX_F32, SWX_F64 = 1, 2
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTS = (
+    "swx_abi_version", "swx_last_error", "swx_plan_create", "swx_plan_destroy",
+    "swx_set_model", "swx_set_shots", "swx_forward", "swx_adjoint",
+    "swx_plan_get_info", "swx_set_kernel_timing",
+)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int32), ("dtype", ctypes.c_int32), ("ndim", ctypes.c_int32),
+        ("halo", ctypes.c_int32), ("n", ctypes.c_int64 * 3), ("h", ctypes.c_double * 3),
+        ("dt", ctypes.c_double), ("freetop", ctypes.c_int32), ("max_shots", ctypes.c_int32),
+        ("nt_max", ctypes.c_int32), ("check_freq", ctypes.c_int32), ("gradient", ctypes.c_int32),
+        ("device", ctypes.c_int32), ("mem_budget", ctypes.c_double),
+    ]
+
+
+class ShotDesc(ctypes.Structure):
+    _fields_ = [
+        ("nt", ctypes.c_int32), ("nsrc", ctypes.c_int32), ("src_idx", ctypes.c_void_p),
+        ("srctf", ctypes.c_void_p), ("nrec", ctypes.c_int32), ("rec_idx", ctypes.c_void_p),
+        ("cpml", (ctypes.c_void_p * 4) * 3),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("bytes_state", ctypes.c_int64), ("bytes_checkpoints", ctypes.c_int64),
+        ("bytes_replay", ctypes.c_int64), ("segment", ctypes.c_int32), ("nshots", ctypes.c_int32),
+        ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
+        ("ms_stencil", ctypes.c_double), ("n_stencil", ctypes.c_int64),
+    ]
+
+
+class SwxError(RuntimeError):
+    """Failure reported by libswx (HIP error or out-of-order call)."""
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libswx.so once; raise ImportError if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIBPATH
+    if not os.path.exists(p):
+        raise ImportError(f"libswx.so not found at {p}: build it with `python seismicwaves.jl_amd/build.py`")
+    lib = ctypes.CDLL(p)
+    vp = ctypes.c_void_p
+    lib.swx_abi_version.restype = ctypes.c_int
+    lib.swx_last_error.restype = ctypes.c_char_p
+    lib.swx_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(vp)]
+    lib.swx_plan_destroy.argtypes = [vp]
+    lib.swx_set_model.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.swx_set_shots.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ShotDesc)]
+    lib.swx_forward.argtypes = [vp, vp, ctypes.c_int]
+    lib.swx_adjoint.argtypes = [vp, vp, vp, ctypes.c_int]
+    lib.swx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
+    lib.swx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    for name in EXPORTS:
+        getattr(lib, name).restype = ctypes.c_int if name != "swx_last_error" else ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc == SWX_OK:
+        return
+    msg = load().swx_last_error().decode(errors="replace")
+    if rc == SWX_EINVAL:
+        raise AssertionError(msg)           # the reference's @assert failures
+    if rc == SWX_ENOMEM:
+        raise MemoryError(msg)
+    if rc == SWX_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise SwxError(f"libswx error {rc}: {msg}")
+
+
+def ptr(a: np.ndarray | None):
+    return ctypes.c_void_p(0) if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Plan:
+    """Owner of one swx_plan (device state for a batch of shots)."""
+
+    def __init__(self, *, physics, dtype, n, h, dt, halo, freetop, max_shots, nt_max,
+                 check_freq=1, gradient=False, device=0, mem_budget=0.0):
+        lib = load()
+        d = PlanDesc()
+        d.physics = physics
+        d.dtype = SWX_F32 if np.dtype(dtype) == np.float32 else SWX_F64
+        d.ndim = len(n)
+        d.halo = int(halo)
+        for k in range(3):
+            d.n[k] = int(n[k]) if k < len(n) else 1
+            d.h[k] = float(h[k]) if k < len(h) else 1.0
+        d.dt = float(dt)
+        d.freetop = int(bool(freetop))
+        d.max_shots = int(max_shots)
+        d.nt_max = int(nt_max)
+        d.check_freq = int(check_freq)
+        d.gradient = int(bool(gradient))
+        d.device = int(device)
+        d.mem_budget = float(mem_budget)
+        self.dtype = np.dtype(dtype)
+        self.ndim = len(n)
+        self._h = ctypes.c_void_p()
+        check(lib.swx_plan_create(ctypes.byref(d), ctypes.byref(self._h)))
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            load().swx_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, fields):
+        arrs = [np.asfortranarray(f, dtype=self.dtype) for f in fields]
+        ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        check(load().swx_set_model(self._h, len(arrs), ptrs))
+
+    def set_shots(self, shots):
+        """shots: list of dicts with nt, src_idx (nsrc, ndim) 1-based, srctf
+        (nt, nsrc), rec_idx (nrec, ndim), cpml [axis] -> dict a, a_h, b, b_h."""
+        keep = []
+        descs = (ShotDesc * len(shots))()
+        for k, sh in enumerate(shots):
+            si = np.asfortranarray(sh["src_idx"], dtype=np.int64)
+            ri = np.asfortranarray(sh["rec_idx"], dtype=np.int64)
+            tf = np.asfortranarray(sh["srctf"], dtype=self.dtype)
+            keep += [si, ri, tf]
+            dd = descs[k]
+            dd.nt = int(sh["nt"])
+            dd.nsrc = si.shape[0]
+            dd.src_idx = si.ctypes.data
+            dd.srctf = tf.ctypes.data
+            dd.nrec = ri.shape[0]
+            dd.rec_idx = ri.ctypes.data
+            for ax, co in enumerate(sh["cpml"]):
+                for j, key in enumerate(("a", "a_h", "b", "b_h")):
+                    arr = np.ascontiguousarray(co[key], dtype=self.dtype)
+                    keep.append(arr)
+                    dd.cpml[ax][j] = arr.ctypes.data if arr.size else None
+        check(load().swx_set_shots(self._h, len(shots), descs))
+        self._keep = keep
+
+    def forward(self, traces: np.ndarray | None, save_state: bool):
+        check(load().swx_forward(self._h, ptr(traces), int(save_state)))
+
+    def adjoint(self, adjsrc: np.ndarray, grad: np.ndarray, per_shot: bool):
+        check(load().swx_adjoint(self._h, ptr(adjsrc), ptr(grad), int(per_shot)))
+
+    def info(self) -> PlanInfo:
+        inf = PlanInfo()
+        check(load().swx_plan_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def set_kernel_timing(self, on: bool):
+        check(load().swx_set_kernel_timing(self._h, int(on)))

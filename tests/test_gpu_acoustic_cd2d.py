@@ -1,0 +1,204 @@
+"""2D acoustic constant density on the GPU, through the C ABI, against the
+FP64 oracle: seismograms and gradients as relative L2 error, <= 1e-10 for
+the FP64 GPU mode and <= 1e-4 for FP32."""
+import numpy as np
+import pytest
+
+import seismicwaves as sw
+from seismicwaves import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-10, np.float32: 1e-4}
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_shots(pos_list, tf, rec_list, f0, nts, dtype):
+    shots = []
+    for pos, recp, nt in zip(pos_list, rec_list, nts):
+        srcs = sw.ScalarSources(np.atleast_2d(pos), tf[:nt].astype(dtype), f0)
+        shots.append(sw.ScalarShot(srcs, sw.ScalarReceivers(np.atleast_2d(recp), nt, dtype=dtype)))
+    return shots
+
+
+def oracle_shots(shots):
+    return [dict(src_pos=s.srcs.positions.astype(np.float64), srctf=s.srcs.tf.astype(np.float64),
+                 f0=float(s.srcs.domfreq), rec_pos=s.recs.positions.astype(np.float64)) for s in shots]
+
+
+# ---------------------------------------------------------------------------
+# Forward: the plain forward configuration (301 x 201, nt = 2000)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_forward_gaussian_anomaly(oracle_mod, dtype):
+    nx, nz, h, dt, nt, f0 = 301, 201, (10.0, 10.0), 1e-3, 2000, 10.0
+    print(f"[cd2d forward 301x201 nt=2000 {np.dtype(dtype).name}]", flush=True)
+    vp = syn.gaussian_anomaly(nx, nz, h)
+    tf = syn.ricker_tf(nt, dt, f0, 2 / f0)[:, None]
+    rec = np.stack([30.0 * np.arange(101), np.full(101, 20.0)], 1)
+    params = sw.InputParametersAcoustic(nt, dtype(dt), (nx, nz), h,
+                                        sw.CPMLBoundaryConditionParameters(halo=20, rcoef=1e-4, freeboundtop=False))
+    shots = make_shots([[1500.0, 20.0]], tf, [rec], f0, [nt], dtype)
+    sw.swforward(params, sw.VpAcousticCDMaterialProperties(vp.astype(dtype)), shots, runparams=sw.RunParameters())
+    ref = oracle_mod.acoustic_cd_forward(vp, h, dt, nt, 20, 1e-4, False, oracle_shots(shots))[0]
+    err = rel_l2(shots[0].recs.seismograms, ref)
+    print(f"rel-L2 seismograms = {err:.3e}")
+    assert np.max(np.abs(ref)) > 0
+    assert err <= TOL[dtype]
+
+
+# ---------------------------------------------------------------------------
+# Gradient: batched shots of uneven length with checkpoint replay
+# ---------------------------------------------------------------------------
+def _grad_problem(dtype, nts=(420, 500, 333), freetop=True, seed=7, n=(150, 110)):
+    nx, nz = n
+    h, dt, f0, halo = (10.0, 10.0), 1e-3, 8.0, 12
+    rng = np.random.default_rng(seed)
+    vtrue = syn.layered(nx, nz, h, rng, vmin=1800, vmax=3000)
+    v0 = syn.gaussian_smooth(vtrue, 4.0)
+    ntmax = max(nts)
+    tf = syn.ricker_tf(ntmax, dt, f0, 1.2 / f0)[:, None]
+    xs = rng.uniform(halo * h[0], (nx - 1 - halo) * h[0], len(nts))
+    recs = [syn.surface_receivers(37 + 5 * k, nx, h, halo, 30.0) for k in range(len(nts))]
+    shots = make_shots([[x, 20.0] for x in xs], tf, recs, f0, nts, dtype)
+    params = sw.InputParametersAcoustic(ntmax, dtype(dt), (nx, nz), h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=freetop))
+    return dict(params=params, vtrue=vtrue, v0=v0, shots=shots, h=h, dt=dt, halo=halo, freetop=freetop, nts=nts)
+
+
+def _observed(oracle_mod, p):
+    out = []
+    for sh, nt in zip(p["shots"], p["nts"]):
+        o = oracle_shots([sh])
+        out.append(oracle_mod.acoustic_cd_forward(p["vtrue"], p["h"], p["dt"], nt, p["halo"], 1e-4, p["freetop"], o)[0])
+    return out
+
+
+def _oracle_gradient(oracle_mod, p, obs, check_freq):
+    tot, mis, seis = None, 0.0, []
+    for sh, nt, ob in zip(p["shots"], p["nts"], obs):
+        o = oracle_shots([sh])
+        o[0]["observed"] = ob
+        g, m, s = oracle_mod.acoustic_cd_gradient(p["v0"], p["h"], p["dt"], nt, p["halo"], 1e-4, p["freetop"], o,
+                                                  check_freq=min(check_freq, nt - 1))
+        tot = g["vp"] if tot is None else tot + g["vp"]
+        mis += m
+        seis.append(s[0])
+    return tot, mis, seis
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gradient_batched_uneven_shots(oracle_mod, dtype):
+    print(f"[cd2d gradient batched 3 shots {np.dtype(dtype).name}]", flush=True)
+    p = _grad_problem(dtype)
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o.astype(dtype)) for o in obs]
+    grad, misfit = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"].astype(dtype)), p["shots"], mis,
+                                 runparams=sw.RunParameters(),
+                                 gradparams=sw.GradParameters(check_freq=50, compute_misfit=True))
+    gref, mref, sref = _oracle_gradient(oracle_mod, p, obs, 50)
+    for sh, s in zip(p["shots"], sref):
+        assert rel_l2(sh.recs.seismograms, s) <= TOL[dtype]
+    err = rel_l2(grad["vp"], gref)
+    print(f"rel-L2 gradient = {err:.3e}; misfit {misfit} vs {mref}")
+    assert np.max(np.abs(gref)) > 0
+    assert err <= TOL[dtype]
+    assert abs(misfit - mref) <= TOL[dtype] * abs(mref)
+
+
+def test_gradient_checkpoint_intervals_agree_bitwise(oracle_mod):
+    """Store-all (check_freq=1), check_freq=50 and a ragged check_freq=7
+    give bit-identical gradients (the replay is deterministic)."""
+    p = _grad_problem(np.float32, nts=(300, 257))
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o.astype(np.float32)) for o in obs]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"].astype(np.float32))
+    outs = []
+    for cf in (1, 50, 7):
+        outs.append(sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                                  gradparams=sw.GradParameters(check_freq=cf))["vp"])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_batched_equals_sequential():
+    """One batched launch over three shots = three single-shot runs."""
+    p = _grad_problem(np.float32)
+    rng = np.random.default_rng(1)
+    mis = [sw.L2Misfit(rng.standard_normal(sh.recs.seismograms.shape).astype(np.float32) * 1e-3) for sh in p["shots"]]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"].astype(np.float32))
+    gp = sw.GradParameters(check_freq=40)
+    gb = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(), gradparams=gp)["vp"]
+    seis_b = [sh.recs.seismograms.copy() for sh in p["shots"]]
+    gs = np.zeros_like(gb)
+    for k, sh in enumerate(p["shots"]):
+        gs += sw.swgradient(p["params"], mp, [sh], [mis[k]], runparams=sw.RunParameters(), gradparams=gp)["vp"]
+        assert np.array_equal(sh.recs.seismograms, seis_b[k])
+    np.testing.assert_allclose(gb, gs, rtol=1e-5, atol=1e-6 * np.max(np.abs(gs)))
+
+
+def test_gradient_mute_per_shot(oracle_mod):
+    p = _grad_problem(np.float64, nts=(300, 280))
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o) for o in obs]
+    gp = sw.GradParameters(check_freq=30, mute_radius_src=5, mute_radius_rec=2)
+    grad = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"]), p["shots"], mis,
+                         runparams=sw.RunParameters(), gradparams=gp)["vp"]
+    tot = None
+    for sh, nt, ob in zip(p["shots"], p["nts"], obs):
+        o = oracle_shots([sh])
+        o[0]["observed"] = ob
+        g, _, _ = oracle_mod.acoustic_cd_gradient(p["v0"], p["h"], p["dt"], nt, p["halo"], 1e-4, p["freetop"], o,
+                                                  check_freq=30, mute_radius_src=5, mute_radius_rec=2)
+        tot = g["vp"] if tot is None else tot + g["vp"]
+    assert rel_l2(grad, tot) <= 1e-10
+
+
+@pytest.mark.parametrize("case", ["colocated", "edge_receivers", "halo0"])
+def test_point_edge_cases(oracle_mod, case):
+    """Several sources on one cell (serial injection order), receivers on
+    the outermost row (edge values are never updated by the stencil and
+    accumulate the adjoint injection) and a grid without C-PML."""
+    dtype = np.float64
+    nx, nz, h, dt, nt, f0 = 90, 70, (10.0, 10.0), 1e-3, 260, 9.0
+    halo = 0 if case == "halo0" else 10
+    rng = np.random.default_rng(11)
+    vtrue = 2000 + 300 * rng.random((nx, nz))
+    v0 = np.full((nx, nz), 2100.0)
+    tf = np.stack([syn.ricker_tf(nt, dt, f0, 1.2 / f0), 0.5 * syn.ricker_tf(nt, dt, f0, 1.5 / f0)], 1)
+    if case == "colocated":
+        src = np.array([[450.0, 300.0], [450.0, 300.0]])
+        rec = np.array([[200.0, 400.0], [200.0, 400.0], [600.0, 420.0]])
+    elif case == "edge_receivers":
+        src = np.array([[450.0, 300.0], [300.0, 200.0]])
+        rec = np.stack([np.arange(15, 75, 6) * 10.0, np.zeros(10)], 1)
+        rec = np.vstack([rec, [[0.0, 350.0], [890.0, 350.0]]])
+    else:
+        src = np.array([[450.0, 300.0], [300.0, 200.0]])
+        rec = np.stack([np.arange(15, 75, 6) * 10.0, np.full(10, 500.0)], 1)
+    shot = dict(src_pos=src, srctf=tf, f0=f0, rec_pos=rec)
+    obs = oracle_mod.acoustic_cd_forward(vtrue, h, dt, nt, halo, 1e-4, True, [shot])[0]
+    shot["observed"] = obs
+    gref, mref, sref = oracle_mod.acoustic_cd_gradient(v0, h, dt, nt, halo, 1e-4, True, [shot], check_freq=1)
+    params = sw.InputParametersAcoustic(nt, dt, (nx, nz), h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=True))
+    shots = [sw.ScalarShot(sw.ScalarSources(src, tf, f0), sw.ScalarReceivers(rec, nt))]
+    g = sw.swgradient(params, sw.VpAcousticCDMaterialProperties(v0), shots, [sw.L2Misfit(obs)],
+                      runparams=sw.RunParameters(), gradparams=sw.GradParameters(check_freq=1))
+    assert rel_l2(shots[0].recs.seismograms, sref[0]) <= 1e-10
+    assert rel_l2(g["vp"], gref["vp"]) <= 1e-10
+
+
+def test_misfit_api_matches_gradient_misfit(oracle_mod):
+    p = _grad_problem(np.float64, nts=(300, 300))
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o) for o in obs]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"])
+    m1 = sw.swmisfit(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters())
+    _, m2 = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(compute_misfit=True, check_freq=20))
+    assert np.isclose(m1, m2, rtol=1e-12)
