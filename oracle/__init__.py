@@ -336,3 +336,156 @@ def acoustic_cd_gradient(vp, spacing, dt, nt, halo, rcoef, freetop, shots, check
         misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
         seis.append(tr)
     return {"vp": tot}, misfit, seis
+
+
+# ---------------------------------------------------------------------------
+# 2D acoustic variable density (staggered)
+# ---------------------------------------------------------------------------
+def interp_stag(m, dim, method="arithmetic"):
+    """ArithmeticAverageInterpolation / HarmonicAverageInterpolation at the
+    half points along `dim` (1-based), src/utils/interpolations.jl:35-43."""
+    ax = dim - 1
+    n = m.shape[ax]
+    lo = np.take(m, np.arange(n - 1), axis=ax)
+    hi = np.take(m, np.arange(1, n), axis=ax)
+    if method == "arithmetic":
+        return 0.5 * lo + 0.5 * hi
+    return 1.0 / (0.5 / lo + 0.5 / hi)
+
+
+def back_interp(m, g, dim, method="arithmetic"):
+    """back_interp, interpolations.jl:15-28 with ∂f∂m :45-47."""
+    ax = dim - 1
+    n = m.shape[ax]
+    out = np.zeros(m.shape)
+    lo = np.take(m, np.arange(n - 1), axis=ax)
+    hi = np.take(m, np.arange(1, n), axis=ax)
+    if method == "arithmetic":
+        wl = wh = 0.5
+    else:
+        itp = interp_stag(m, dim, method)
+        wl, wh = itp ** 2 / lo ** 2 / 2, itp ** 2 / hi ** 2 / 2
+    idx_lo = [slice(None)] * m.ndim
+    idx_hi = [slice(None)] * m.ndim
+    idx_lo[ax] = slice(0, n - 1)
+    idx_hi[ax] = slice(1, n)
+    out[tuple(idx_lo)] += g * wl
+    out[tuple(idx_hi)] += g * wh
+    return out
+
+
+def vd_model_fields(vp, rho, dt, method="arithmetic", dtype=np.float64):
+    """precompute_fact! for VD, acou_models.jl:292-300."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    rho = np.asarray(rho, dtype=T)
+    m0 = vp ** 2 * rho * T(dt)
+    inv = T(1) / rho
+    return m0, (interp_stag(inv, 1, method) * T(dt)).astype(T), (interp_stag(inv, 2, method) * T(dt)).astype(T)
+
+
+def scale_srctf_vd(tf, spacing, dt, vp, rho, src_idx, dtype=np.float64):
+    """possrcrec_scaletf for VD, acou_forward.jl:67-81."""
+    T = dtype
+    out = np.asarray(tf, dtype=T) / T(np.prod(np.asarray(spacing, dtype=T))) * T(dt)
+    for s in range(out.shape[1]):
+        ix = tuple(src_idx[s] - 1)
+        out[:, s] *= T(vp[ix]) ** 2 * T(rho[ix])
+    return out.astype(T)
+
+
+def vd2d_forward(m0, m1x, m1z, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, dtype=np.float64):
+    m0 = _f_order(m0, dtype)
+    m1x = _f_order(m1x, dtype)
+    m1z = _f_order(m1z, dtype)
+    nx, nz = m0.shape
+    g = _geom((nx, nz), spacing, dt, halo)
+    cx, kx = _cpml_struct(cpml[0], dtype)
+    cz, kz = _cpml_struct(cpml[1], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    rc = _fn("oracle_vd2d_forward", dtype)(
+        ctypes.byref(g), _ptr(m0), _ptr(m1x), _ptr(m1z), ctypes.byref(cx), ctypes.byref(cz), ctypes.c_int(nt),
+        ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(traces))
+    assert rc == 0
+    return np.ascontiguousarray(traces)
+
+
+def vd2d_gradient(m0, m1x, m1z, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, adjsrc, check_freq,
+                  dtype=np.float64):
+    """Raw grad_m0, grad_m1_stag (x, z) and the traces of swgradient_1shot! (VD)."""
+    m0 = _f_order(m0, dtype)
+    m1x = _f_order(m1x, dtype)
+    m1z = _f_order(m1z, dtype)
+    nx, nz = m0.shape
+    g = _geom((nx, nz), spacing, dt, halo)
+    cx, kx = _cpml_struct(cpml[0], dtype)
+    cz, kz = _cpml_struct(cpml[1], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    g0 = np.zeros((nx, nz), dtype=dtype, order="F")
+    g1x = np.zeros((nx - 1, nz), dtype=dtype, order="F")
+    g1z = np.zeros((nx, nz - 1), dtype=dtype, order="F")
+    rc = _fn("oracle_vd2d_gradient", dtype)(
+        ctypes.byref(g), _ptr(m0), _ptr(m1x), _ptr(m1z), ctypes.byref(cx), ctypes.byref(cz), ctypes.c_int(nt),
+        ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(adj),
+        ctypes.c_int(check_freq), _ptr(g0), _ptr(g1x), _ptr(g1z), _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return (np.array(g0), np.array(g1x), np.array(g1z)), np.ascontiguousarray(traces)
+
+
+def acoustic_vd_forward(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, method="arithmetic",
+                        dtype=np.float64):
+    """run_swforward! for VD 2D; returns the seismograms of each shot."""
+    T = dtype
+    m0, m1x, m1z = vd_model_fields(vp, rho, dt, method, T)
+    out = []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_vd(sh["srctf"][:nt_s], spacing, dt, vp, rho, si, T)
+        out.append(vd2d_forward(m0, m1x, m1z, spacing, dt, halo, cpml, nt_s, si, stf, ri, T))
+    return out
+
+
+def acoustic_vd_gradient(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1,
+                         mute_radius_src=0, mute_radius_rec=0, method="arithmetic", dtype=np.float64):
+    """run_swgradient! for VD 2D (gradient.jl:93-136, acou_gradient.jl:222-327).
+    Returns ({"vp", "rho"}, misfit, seismograms)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    rho = np.asarray(rho, dtype=T)
+    m0, m1x, m1z = vd_model_fields(vp, rho, dt, method, T)
+    gvp = np.zeros(vp.shape, dtype=T)
+    grho = np.zeros(vp.shape, dtype=T)
+    misfit, seis = 0.0, []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_vd(sh["srctf"][:nt_s], spacing, dt, vp, rho, si, T)
+        syn = vd2d_forward(m0, m1x, m1z, spacing, dt, halo, cpml, nt_s, si, stf, ri, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
+        (g0, g1x, g1z), tr = vd2d_gradient(m0, m1x, m1z, spacing, dt, halo, cpml, nt_s, si, stf, ri,
+                                           adj.astype(T), min(check_freq, nt_s - 1), T)
+        inv = 1.0 / rho.astype(np.float64)
+        g1 = back_interp(inv, g1x, 1, method) + back_interp(inv, g1z, 2, method)
+        g0 = g0.astype(np.float64)
+        g0 = mute_around_points(mute_around_points(g0, sh["src_pos"], spacing, mute_radius_src),
+                                sh["rec_pos"], spacing, mute_radius_rec)
+        g1 = mute_around_points(mute_around_points(g1, sh["src_pos"], spacing, mute_radius_src),
+                                sh["rec_pos"], spacing, mute_radius_rec)
+        gvp += (-2.0 * g0 / (vp ** 3 * rho)).astype(T)
+        grho += (-g0 / (vp ** 2 * rho ** 2) - g1 / rho).astype(T)
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return {"vp": gvp, "rho": grho}, misfit, seis

@@ -332,3 +332,260 @@ int SFX(oracle_cd2d_gradient)(const oracle_geom *g, const real *fact,
   SFX(cd2d_state_free)(&a);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* 2D acoustic variable density (staggered pressure / velocity)             */
+/* ------------------------------------------------------------------------ */
+
+/* 4th-order staggered first derivative of fdgen.jl:65-135 with order=4,
+ * deriv=1 (weights 1/24, -9/8, 9/8, -1/24 at offsets -1, 0, 1, 2 from the
+ * base index `b`, terms outside 1..n dropped as the generated boundary
+ * branches do).  `at(k)` reads the 1-based element k along the axis.        */
+#define D4_TERM(k, w) (((k) >= 1 && (k) <= n) ? (w) * at(k) : (real)0)
+
+/* derivative along x of A (n1 x n2, column-major) at (b, j) */
+static real SFX(d4x)(const real *A, int64_t n1, int64_t n, int64_t b, int64_t j) {
+#define at(k) A2(A, n1, (k), j)
+  const real c1 = (real)(1.0 / 24.0), c2 = (real)(-9.0 / 8.0);
+  if (b - 1 >= 1 && b + 2 <= n) return c1 * at(b - 1) + c2 * at(b) + (-c2) * at(b + 1) + (-c1) * at(b + 2);
+  return D4_TERM(b - 1, c1) + D4_TERM(b, c2) + D4_TERM(b + 1, -c2) + D4_TERM(b + 2, -c1);
+#undef at
+}
+/* derivative along z of A (n1 x n) at (i, b) */
+static real SFX(d4z)(const real *A, int64_t n1, int64_t n, int64_t i, int64_t b) {
+#define at(k) A2(A, n1, i, (k))
+  const real c1 = (real)(1.0 / 24.0), c2 = (real)(-9.0 / 8.0);
+  if (b - 1 >= 1 && b + 2 <= n) return c1 * at(b - 1) + c2 * at(b) + (-c2) * at(b + 1) + (-c1) * at(b + 2);
+  return D4_TERM(b - 1, c1) + D4_TERM(b, c2) + D4_TERM(b + 1, -c2) + D4_TERM(b + 2, -c1);
+#undef at
+}
+
+typedef struct {
+  real *p, *vx, *vz;              /* p: nx x nz, vx: (nx-1) x nz, vz: nx x (nz-1) */
+  real *psi_x, *psi_z, *xi_x, *xi_z;
+} SFX(vd2d_state);
+
+/* update_p_CPML! (acoustic2D_VD_xPU.jl:17-37): p -= m0 (d~x vx + d~z vz) */
+static void SFX(vd2d_update_p)(const oracle_geom *g, const real *m0, SFX(vd2d_state) *s,
+                               const SFX(cpml) *cx, const SFX(cpml) *cz) {
+  const int64_t nx = g->n[0], nz = g->n[1];
+  const int halo = g->halo;
+  const int64_t nxx = 2 * (halo + 1);
+  const real _dx = (real)(1.0 / g->h[0]), _dz = (real)(1.0 / g->h[1]);
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 2; j <= nz - 1; j++) {
+    for (int64_t i = 2; i <= nx - 1; i++) {
+      /* @d~x(vx, a_x, b_x, xi_x, order=4, I=(i-1,j), halfgrid=false): idim = i */
+      real dvx = SFX(d4x)(s->vx, nx - 1, nx - 1, i - 1, j) * _dx;
+      if (i <= halo + 1) {
+        A2(s->xi_x, nxx, i, j) = cx->b[i - 1] * A2(s->xi_x, nxx, i, j) + cx->a[i - 1] * dvx;
+        dvx = dvx + A2(s->xi_x, nxx, i, j);
+      } else if (i >= nx - halo) {
+        int64_t k = i - (nx - halo) + 1 + (halo + 1);
+        A2(s->xi_x, nxx, k, j) = cx->b[k - 1] * A2(s->xi_x, nxx, k, j) + cx->a[k - 1] * dvx;
+        dvx = dvx + A2(s->xi_x, nxx, k, j);
+      }
+      real dvz = SFX(d4z)(s->vz, nx, nz - 1, i, j - 1) * _dz;
+      if (j <= halo + 1) {
+        A2(s->xi_z, nx, i, j) = cz->b[j - 1] * A2(s->xi_z, nx, i, j) + cz->a[j - 1] * dvz;
+        dvz = dvz + A2(s->xi_z, nx, i, j);
+      } else if (j >= nz - halo) {
+        int64_t k = j - (nz - halo) + 1 + (halo + 1);
+        A2(s->xi_z, nx, i, k) = cz->b[k - 1] * A2(s->xi_z, nx, i, k) + cz->a[k - 1] * dvz;
+        dvz = dvz + A2(s->xi_z, nx, i, k);
+      }
+      A2(s->p, nx, i, j) -= A2(m0, nx, i, j) * (dvx + dvz);
+    }
+  }
+}
+
+/* update_vx_CPML! and update_vy_CPML! (acoustic2D_VD_xPU.jl:39-75) */
+static void SFX(vd2d_update_v)(const oracle_geom *g, const real *m1x, const real *m1z, SFX(vd2d_state) *s,
+                               const SFX(cpml) *cx, const SFX(cpml) *cz) {
+  const int64_t nx = g->n[0], nz = g->n[1];
+  const int halo = g->halo;
+  const int64_t npx = 2 * halo;
+  const real _dx = (real)(1.0 / g->h[0]), _dz = (real)(1.0 / g->h[1]);
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 1; j <= nz; j++) {
+    for (int64_t i = 1; i <= nx - 1; i++) {
+      real d = SFX(d4x)(s->p, nx, nx, i, j) * _dx;
+      if (i <= halo) {
+        A2(s->psi_x, npx, i, j) = cx->b_h[i - 1] * A2(s->psi_x, npx, i, j) + cx->a_h[i - 1] * d;
+        d = d + A2(s->psi_x, npx, i, j);
+      } else if (i >= nx - halo) {
+        int64_t k = i - (nx - halo) + 1 + halo;
+        A2(s->psi_x, npx, k, j) = cx->b_h[k - 1] * A2(s->psi_x, npx, k, j) + cx->a_h[k - 1] * d;
+        d = d + A2(s->psi_x, npx, k, j);
+      }
+      A2(s->vx, nx - 1, i, j) -= A2(m1x, nx - 1, i, j) * d;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 1; j <= nz - 1; j++) {
+    for (int64_t i = 1; i <= nx; i++) {
+      real d = SFX(d4z)(s->p, nx, nz, i, j) * _dz;
+      if (j <= halo) {
+        A2(s->psi_z, nx, i, j) = cz->b_h[j - 1] * A2(s->psi_z, nx, i, j) + cz->a_h[j - 1] * d;
+        d = d + A2(s->psi_z, nx, i, j);
+      } else if (j >= nz - halo) {
+        int64_t k = j - (nz - halo) + 1 + halo;
+        A2(s->psi_z, nx, i, k) = cz->b_h[k - 1] * A2(s->psi_z, nx, i, k) + cz->a_h[k - 1] * d;
+        d = d + A2(s->psi_z, nx, i, k);
+      }
+      A2(s->vz, nx, i, j) -= A2(m1z, nx, i, j) * d;
+    }
+  }
+}
+
+static void SFX(vd2d_inject)(const oracle_geom *g, real *p, int n, const int64_t *idx, const real *tf, int nt,
+                             int it, const real *scale_field) {
+  const int64_t nx = g->n[0];
+  for (int s = 0; s < n; s++) {
+    int64_t i = idx[s], j = idx[s + n];
+    real v = tf[(it - 1) + (size_t)nt * s];
+    if (scale_field) v = v * A2(scale_field, nx, i, j);
+    A2(p, nx, i, j) += v;
+  }
+}
+
+/* forward_onestep_CPML! (acoustic2D_VD_xPU.jl:91-137) */
+static void SFX(vd2d_fwd_step)(const oracle_geom *g, const real *m0, const real *m1x, const real *m1z,
+                               SFX(vd2d_state) *s, const SFX(cpml) *cx, const SFX(cpml) *cz, int nsrc,
+                               const int64_t *src, const real *srctf, int nt, int it, int nrec,
+                               const int64_t *rec, real *traces) {
+  const int64_t nx = g->n[0];
+  SFX(vd2d_update_p)(g, m0, s, cx, cz);
+  SFX(vd2d_inject)(g, s->p, nsrc, src, srctf, nt, it, NULL);
+  SFX(vd2d_update_v)(g, m1x, m1z, s, cx, cz);
+  if (traces)
+    for (int r = 0; r < nrec; r++) traces[(it - 1) + (size_t)nt * r] = A2(s->p, nx, rec[r], rec[r + nrec]);
+}
+
+/* adjoint_onestep_CPML! (acoustic2D_VD_xPU.jl:139-178); the residual is
+ * prescaled by fact_m0 at the receivers (prescale_residuals!, :77-89) */
+static void SFX(vd2d_adj_step)(const oracle_geom *g, const real *m0, const real *m1x, const real *m1z,
+                               SFX(vd2d_state) *s, const SFX(cpml) *cx, const SFX(cpml) *cz, int nrec,
+                               const int64_t *rec, const real *res, int nt, int it) {
+  SFX(vd2d_update_v)(g, m1x, m1z, s, cx, cz);
+  SFX(vd2d_update_p)(g, m0, s, cx, cz);
+  SFX(vd2d_inject)(g, s->p, nrec, rec, res, nt, it, NULL);
+}
+
+static void SFX(vd2d_state_alloc)(const oracle_geom *g, SFX(vd2d_state) *s) {
+  const int64_t nx = g->n[0], nz = g->n[1];
+  const int h = g->halo;
+  s->p = (real *)calloc((size_t)nx * nz, sizeof(real));
+  s->vx = (real *)calloc((size_t)(nx - 1) * nz, sizeof(real));
+  s->vz = (real *)calloc((size_t)nx * (nz - 1), sizeof(real));
+  s->psi_x = (real *)calloc((size_t)(2 * h) * nz + 1, sizeof(real));
+  s->psi_z = (real *)calloc((size_t)nx * (2 * h) + 1, sizeof(real));
+  s->xi_x = (real *)calloc((size_t)(2 * h + 2) * nz, sizeof(real));
+  s->xi_z = (real *)calloc((size_t)nx * (2 * h + 2), sizeof(real));
+}
+static void SFX(vd2d_state_free)(SFX(vd2d_state) *s) {
+  free(s->p); free(s->vx); free(s->vz); free(s->psi_x); free(s->psi_z); free(s->xi_x); free(s->xi_z);
+}
+static void SFX(vd2d_state_copy)(const oracle_geom *g, SFX(vd2d_state) *dst, const SFX(vd2d_state) *src) {
+  const int64_t nx = g->n[0], nz = g->n[1];
+  const int h = g->halo;
+  memcpy(dst->p, src->p, (size_t)nx * nz * sizeof(real));
+  memcpy(dst->vx, src->vx, (size_t)(nx - 1) * nz * sizeof(real));
+  memcpy(dst->vz, src->vz, (size_t)nx * (nz - 1) * sizeof(real));
+  memcpy(dst->psi_x, src->psi_x, ((size_t)(2 * h) * nz + 1) * sizeof(real));
+  memcpy(dst->psi_z, src->psi_z, ((size_t)nx * (2 * h) + 1) * sizeof(real));
+  memcpy(dst->xi_x, src->xi_x, (size_t)(2 * h + 2) * nz * sizeof(real));
+  memcpy(dst->xi_z, src->xi_z, (size_t)nx * (2 * h + 2) * sizeof(real));
+}
+
+/* swforward_1shot! for VD (acou_forward.jl:83-125) */
+int SFX(oracle_vd2d_forward)(const oracle_geom *g, const real *m0, const real *m1x, const real *m1z,
+                             const SFX(cpml) *cx, const SFX(cpml) *cz, int nt, int nsrc, const int64_t *src,
+                             const real *srctf, int nrec, const int64_t *rec, real *traces) {
+  SFX(vd2d_state) s;
+  SFX(vd2d_state_alloc)(g, &s);
+  for (int it = 1; it <= nt; it++)
+    SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, it, nrec, rec, traces);
+  SFX(vd2d_state_free)(&s);
+  return 0;
+}
+
+/* swgradient_1shot! for VD (acou_gradient.jl:222-300) up to the raw
+ * grad_m0 and grad_m1_stag fields, with the LinearCheckpointer of
+ * acou_models.jl:409-423 (pcur width 1, buffered pcur; pcur, vcur, psi, xi
+ * checkpointed every cf steps). */
+int SFX(oracle_vd2d_gradient)(const oracle_geom *g, const real *m0, const real *m1x, const real *m1z,
+                              const SFX(cpml) *cx, const SFX(cpml) *cz, int nt, int nsrc, const int64_t *src,
+                              const real *srctf, int nrec, const int64_t *rec, const real *adjsrc, int cf,
+                              real *g_m0, real *g_m1x, real *g_m1z, real *traces) {
+  if (cf < 1 || cf >= nt) return -1;
+  const int64_t nx = g->n[0], nz = g->n[1];
+  const size_t N = (size_t)nx * nz;
+  SFX(vd2d_state) s, a;
+  SFX(vd2d_state_alloc)(g, &s);
+  SFX(vd2d_state_alloc)(g, &a);
+  const int last = (nt / cf) * cf;
+  int curr = last;
+  const int nk = (nt + 1) / cf + 1;
+  SFX(vd2d_state) *ck = (SFX(vd2d_state) *)calloc(nk, sizeof(SFX(vd2d_state)));
+  for (int k = 0; k < nk; k++) SFX(vd2d_state_alloc)(g, &ck[k]);   /* ck[0] = zero state */
+  real **buf = (real **)calloc(cf + 1, sizeof(real *));
+  for (int k = 0; k <= cf; k++) buf[k] = (real *)calloc(N, sizeof(real));
+  real *own_tr = traces ? traces : (real *)calloc((size_t)nt * nrec, sizeof(real));
+  for (int it = 1; it <= nt; it++) {
+    SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, it, nrec, rec, own_tr);
+    if (it % cf == 0) SFX(vd2d_state_copy)(g, &ck[it / cf], &s);
+    if (it >= last) memcpy(buf[it - last], s.p, N * sizeof(real));
+  }
+  real *res = (real *)malloc((size_t)nt * nrec * sizeof(real));
+  for (int r = 0; r < nrec; r++) {
+    real f = A2(m0, nx, rec[r], rec[r + nrec]);
+    for (int it = 0; it < nt; it++) res[it + (size_t)nt * r] = adjsrc[it + (size_t)nt * r] * f;
+  }
+  memset(g_m0, 0, N * sizeof(real));
+  memset(g_m1x, 0, (size_t)(nx - 1) * nz * sizeof(real));
+  memset(g_m1z, 0, (size_t)nx * (nz - 1) * sizeof(real));
+  const real _dt = (real)(1.0 / g->dt);
+  const real _dx = (real)(1.0 / g->h[0]), _dz = (real)(1.0 / g->h[1]);
+#define VD_SAVED(t) (((t) % cf == 0) ? ck[(t) / cf].p : buf[(t) - curr])
+  for (int it = nt; it >= 1; it--) {
+    SFX(vd2d_adj_step)(g, m0, m1x, m1z, &a, cx, cz, nrec, rec, res, nt, it);
+    int t = it - 1;
+    int saved = (curr <= t && t <= curr + cf) || (t % cf == 0);
+    if (!saved) {
+      int old = curr;
+      curr -= cf;
+      memcpy(buf[0], ck[curr / cf].p, N * sizeof(real));
+      memcpy(buf[cf], ck[old / cf].p, N * sizeof(real));
+      SFX(vd2d_state_copy)(g, &s, &ck[curr / cf]);
+      for (int rit = curr + 1; rit <= curr + cf - 1; rit++) {
+        SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, rit, 0, NULL, NULL);
+        memcpy(buf[rit - curr], s.p, N * sizeof(real));
+      }
+    }
+    const real *pc = VD_SAVED(it);
+    const real *po = VD_SAVED(it - 1);
+    /* correlate_gradient_m0! (correlate_gradient_xPU.jl:12-21) */
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < N; q++) g_m0[q] = g_m0[q] - (a.p[q] * (pc[q] - po[q]) * _dt);
+    /* correlate_gradient_m1! (acoustic2D_VD_xPU.jl:180-198) */
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 1; j <= nz; j++)
+      for (int64_t i = 1; i <= nx - 1; i++)
+        A2(g_m1x, nx - 1, i, j) = A2(g_m1x, nx - 1, i, j) + A2(a.vx, nx - 1, i, j) * (SFX(d4x)(pc, nx, nx, i, j) * _dx);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 1; j <= nz - 1; j++)
+      for (int64_t i = 1; i <= nx; i++)
+        A2(g_m1z, nx, i, j) = A2(g_m1z, nx, i, j) + A2(a.vz, nx, i, j) * (SFX(d4z)(pc, nx, nz, i, j) * _dz);
+  }
+#undef VD_SAVED
+  free(res);
+  if (!traces) free(own_tr);
+  for (int k = 0; k <= cf; k++) free(buf[k]);
+  free(buf);
+  for (int k = 0; k < nk; k++) SFX(vd2d_state_free)(&ck[k]);
+  free(ck);
+  SFX(vd2d_state_free)(&s);
+  SFX(vd2d_state_free)(&a);
+  return 0;
+}

@@ -1,0 +1,75 @@
+// Device helpers shared by the engines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "swx_internal.hpp"
+
+namespace swx {
+
+// Index of the unique point group at linear cell `cell` (grid row `row`) of
+// shot s, or -1.  rowptr/cells are a PointTables row index (sorted cells).
+__device__ __forceinline__ int find_point(const int *rowptr, const int64_t *cells, int64_t nrows, int s,
+                                          int64_t row, int64_t cell) {
+  const int *rp = rowptr + (int64_t)s * (nrows + 1) + row;
+  int lo = rp[0], hi = rp[1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    int64_t v = cells[mid];
+    if (v == cell) return mid;
+    if (v < cell) lo = mid + 1;
+    else hi = mid;
+  }
+  return -1;
+}
+
+// Packed C-PML coefficients of one axis: a (2h+2) | a_h (2h) | b (2h+2) | b_h (2h)
+template <typename T>
+struct AxisCoef {
+  const T *a, *ah, *b, *bh;
+};
+template <typename T>
+__device__ __forceinline__ AxisCoef<T> axis_coef(const T *base, int h, int axis) {
+  const T *p = base + (int64_t)axis * (8 * h + 4);
+  return AxisCoef<T>{p, p + (2 * h + 2), p + (4 * h + 2), p + (6 * h + 4)};
+}
+
+// Sum over shots (in shot order) times `scale`, or a scaled copy per shot.
+template <typename T>
+__global__ void reduce_shots(T *out, const T *grad, int64_t N, int ns, T scale, int per_shot) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    if (per_shot) {
+      for (int s = 0; s < ns; s++) out[(int64_t)s * N + q] = grad[(int64_t)s * N + q] * scale;
+    } else {
+      T acc = 0;
+      for (int s = 0; s < ns; s++) acc += grad[(int64_t)s * N + q] * scale;
+      out[q] = acc;
+    }
+  }
+}
+
+// Packs the per-shot C-PML coefficient vectors of the shot descriptors.
+template <typename T>
+std::vector<T> pack_cpml(int nshots, const swx_shot_desc *shots, int ndim, int h) {
+  const int stride = ndim * (8 * h + 4);
+  std::vector<T> hc((size_t)nshots * stride);
+  const int la = 2 * h + 2, lh = 2 * h;
+  for (int s = 0; s < nshots; s++) {
+    T *dst = hc.data() + (size_t)s * stride;
+    for (int ax = 0; ax < ndim; ax++) {
+      const int len[4] = {la, lh, la, lh};
+      T *p = dst + ax * (8 * h + 4);
+      for (int k = 0; k < 4; k++) {
+        const T *src = static_cast<const T *>(shots[s].cpml[ax][k]);
+        if (len[k] && !src) throw Error(SWX_EINVAL, "missing C-PML coefficient array");
+        if (len[k]) std::memcpy(p, src, sizeof(T) * len[k]);
+        p += len[k];
+      }
+    }
+  }
+  return hc;
+}
+
+}  // namespace swx

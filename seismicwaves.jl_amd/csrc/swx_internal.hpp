@@ -93,6 +93,12 @@ struct PointTables {
   DevBuf d_urec_list;
   DevBuf d_rec_off;   // int32 [ns+1] receivers per shot (recording)
   DevBuf d_rec_cell;  // int64 [*]
+  // Row index of the unique cells (sorted by linear index): for shot s the
+  // unique source cells of grid row r (all indices but the first) are
+  // [srow[s*(nrows+1)+r], srow[s*(nrows+1)+r+1]); same for receivers.
+  int64_t nrows = 0;
+  DevBuf d_srow;      // int32 [ns*(nrows+1)]
+  DevBuf d_rrow;      // int32 [ns*(nrows+1)]
   DevBuf d_srctf;     // T [total_tf]
   DevBuf d_traces;    // T [total_tr]
   DevBuf d_res;       // T [total_tr] (adjoint sources)
@@ -105,6 +111,7 @@ struct PointArgsView {
   const int *usrc_off; const int64_t *usrc_cell; const int *usrc_ptr; const int *usrc_list;
   const int *urec_off; const int64_t *urec_cell; const int *urec_ptr; const int *urec_list;
   const int *rec_off; const int64_t *rec_cell;
+  const int *srow; const int *rrow; int64_t nrows;
 };
 
 inline PointArgsView view(const PointTables &t) {
@@ -112,7 +119,7 @@ inline PointArgsView view(const PointTables &t) {
                        t.d_usrc_off.as<int>(), t.d_usrc_cell.as<int64_t>(), t.d_usrc_ptr.as<int>(),
                        t.d_usrc_list.as<int>(), t.d_urec_off.as<int>(), t.d_urec_cell.as<int64_t>(),
                        t.d_urec_ptr.as<int>(), t.d_urec_list.as<int>(), t.d_rec_off.as<int>(),
-                       t.d_rec_cell.as<int64_t>()};
+                       t.d_rec_cell.as<int64_t>(), t.d_srow.as<int>(), t.d_rrow.as<int>(), t.nrows};
 }
 
 // Builds PointTables from the ABI shot descriptors.  `grid` are the point

@@ -98,6 +98,10 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
   std::vector<int> usrc_off(1, 0), usrc_ptr(1, 0), usrc_list, urec_off(1, 0), urec_ptr(1, 0), urec_list,
       rec_off(1, 0);
   std::vector<int64_t> usrc_cell, urec_cell, rec_cell;
+  std::vector<int> srow, rrow;
+  int64_t rows = 1;
+  for (int d = 1; d < ndim; d++) rows *= grid[d];
+  t.nrows = rows;
   int64_t otf = 0, otr = 0;
   auto linear = [&](const int64_t *idx, int n, int k, const char *what, int s) {
     int64_t lin = 0, stride = 1;
@@ -124,33 +128,38 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
     t.off_tr[s] = otr;
     otf += (int64_t)sh.nt * sh.nsrc;
     otr += (int64_t)sh.nt * sh.nrec;
-    // group sources by cell, keeping index order inside a group
+    // group sources by cell (cells in ascending linear order, source
+    // indices ascending inside a group) and index the groups by grid row
+    const int64_t nrow = rows;
     std::map<int64_t, std::vector<int>> g;
-    std::vector<int64_t> order;
-    for (int k = 0; k < sh.nsrc; k++) {
-      int64_t c = linear(sh.src_idx, sh.nsrc, k, "Source", s);
-      if (!g.count(c)) order.push_back(c);
-      g[c].push_back(k);
-    }
-    for (int64_t c : order) {
-      usrc_cell.push_back(c);
-      for (int k : g[c]) usrc_list.push_back(k);
+    for (int k = 0; k < sh.nsrc; k++) g[linear(sh.src_idx, sh.nsrc, k, "Source", s)].push_back(k);
+    for (auto &kv : g) {
+      usrc_cell.push_back(kv.first);
+      for (int k : kv.second) usrc_list.push_back(k);
       usrc_ptr.push_back((int)usrc_list.size());
     }
-    usrc_off.push_back((int)usrc_cell.size());
     std::map<int64_t, std::vector<int>> gr;
-    std::vector<int64_t> rorder;
     for (int k = 0; k < sh.nrec; k++) {
       int64_t c = linear(sh.rec_idx, sh.nrec, k, "Receiver", s);
       rec_cell.push_back(c);
-      if (!gr.count(c)) rorder.push_back(c);
       gr[c].push_back(k);
     }
-    for (int64_t c : rorder) {
-      urec_cell.push_back(c);
-      for (int k : gr[c]) urec_list.push_back(k);
+    for (auto &kv : gr) {
+      urec_cell.push_back(kv.first);
+      for (int k : kv.second) urec_list.push_back(k);
       urec_ptr.push_back((int)urec_list.size());
     }
+    auto rowptr = [&](std::vector<int> &dst, const std::vector<int64_t> &cells, int c0) {
+      // dst[r] = first unique cell of this shot whose row is >= r
+      int c = c0;
+      for (int64_t r = 0; r <= nrow; r++) {
+        while (c < (int)cells.size() && cells[c] / grid[0] < r) c++;
+        dst.push_back(c);
+      }
+    };
+    rowptr(srow, usrc_cell, usrc_off.back());
+    rowptr(rrow, urec_cell, urec_off.back());
+    usrc_off.push_back((int)usrc_cell.size());
     urec_off.push_back((int)urec_cell.size());
     rec_off.push_back((int)rec_cell.size());
   }
@@ -174,6 +183,8 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
   up(t.d_urec_list, urec_list.data(), sizeof(int) * urec_list.size());
   up(t.d_rec_off, rec_off.data(), sizeof(int) * rec_off.size());
   up(t.d_rec_cell, rec_cell.data(), sizeof(int64_t) * rec_cell.size());
+  up(t.d_srow, srow.data(), sizeof(int) * srow.size());
+  up(t.d_rrow, rrow.data(), sizeof(int) * rrow.size());
   // source time functions: concatenated per shot
   std::vector<char> tf((size_t)otf * esize);
   for (int s = 0; s < nshots; s++)
