@@ -2,14 +2,21 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one full gradient of the workload (forward with device-resident
-checkpoints, host adjoint source, adjoint with replay and correlation),
-all inputs uploaded to HBM before the timed region.  The printed JSON line
-carries the throughput in GCell-updates/s (2 * cells * sum of shot nt per
-step: one forward plus one adjoint update per cell and time step; replay is
-not counted as work), the roofline fraction of the dominant stencil kernel
-timed with HIP events on its own stream, and a CPU baseline (the oracle's
-OpenMP restatement of the reference's threads backend on a bounded sample).
+A step is one full gradient of the workload's batch of shots: forward with
+device-resident checkpoints, host adjoint source, adjoint with replay and
+gradient correlation; all inputs are uploaded to HBM before the timed
+region.  The JSON line carries
+
+  value        GCell-updates/s = 2 * cells * sum(nt of every shot) / time
+               (one forward and one adjoint update per cell and time step;
+               replay is not counted as work),
+  roofline     the dominant kernel's algorithmic bytes per launch over its
+               average launch time (HIP events around every launch on the
+               plan's stream) against MI355X's 8 TB/s,
+  one_shot     the same kernels run one shot at a time (the plain 1-shot
+               path), on a bounded sample of the shots,
+  cpu_baseline the oracle's OpenMP restatement of the reference's threads
+               backend on a bounded sample, same precision.
 """
 from __future__ import annotations
 
@@ -34,7 +41,47 @@ HBM_PEAK_GBS = 8000.0
 # ---------------------------------------------------------------------------
 # Workloads
 # ---------------------------------------------------------------------------
-def workload_cd2d_grad_1000(dtype=np.float32):
+def workload_vd2d_batched():
+    """Batched 2D acoustic variable-density gradient w.r.t. vp and rho:
+    2000 x 800, dh = 5 m, dt = 0.5 ms, 12 Hz Ricker, 64 shots at seeded x
+    along z = 10 m, nt per shot uniform in 3000..5000, 500 receivers per
+    shot at z = 10 m, L2 misfit, checkpoint every 50 steps.  vp: 8 seeded
+    layers (1500-4000 m/s) plus 200 elliptical inclusions of +-15%; rho:
+    Gardner with +-5% noise.  The starting model is the true one smoothed
+    with sigma = 100 m."""
+    import seismicwaves as sw
+    from seismicwaves import synthetic as syn
+    nx, nz = 2000, 800
+    h = (5.0, 5.0)
+    dt, f0, halo, nshots = 5e-4, 12.0, 20, 64
+    rng = np.random.default_rng(2)
+    vtrue = syn.add_ellipses(syn.layered(nx, nz, h, rng), h, rng, n=200)
+    rtrue = syn.gardner(vtrue, rng)
+    v0 = syn.gaussian_smooth(vtrue, 100.0 / h[0])
+    r0 = syn.gaussian_smooth(rtrue, 100.0 / h[0])
+    nts = rng.integers(3000, 5001, nshots)
+    xs = rng.uniform(halo * h[0], (nx - 1 - halo) * h[0], nshots)
+    ntmax = int(nts.max())
+    tf = syn.ricker_tf(ntmax, dt, f0, 1.2 / f0)[:, None].astype(np.float32)
+    rec = syn.surface_receivers(500, nx, h, halo, 10.0)
+    shots = [sw.ScalarShot(sw.ScalarSources(np.array([[x, 10.0]]), tf[:nt], f0),
+                           sw.ScalarReceivers(rec, int(nt), dtype=np.float32)) for x, nt in zip(xs, nts)]
+    params = sw.InputParametersAcoustic(ntmax, dt, (nx, nz), h, sw.CPMLBoundaryConditionParameters(halo=halo))
+    f32 = np.float32
+    return dict(name="acoustic_vd_2d_gradient_2000x800_64shots_nt3000-5000", params=params, physics="vd2d",
+                mp_true=sw.VpRhoAcousticVDMaterialProperties(vtrue.astype(f32), rtrue.astype(f32)),
+                mp0=sw.VpRhoAcousticVDMaterialProperties(v0.astype(f32), r0.astype(f32)),
+                shots=shots, check_freq=50,
+                # overlapping -15% inclusions push the slowest cells below
+                # 10 points per wavelength: warn instead of failing, as
+                # RunParameters(erroronPPW=false) does in the reference
+                run={"erroronPPW": False},
+                config={"nx": nx, "nz": nz, "dh_m": 5.0, "dt_s": dt, "f0_hz": f0, "shots": nshots,
+                        "nt_min": int(nts.min()), "nt_max": ntmax, "sum_nt": int(nts.sum()), "receivers": 500,
+                        "check_freq": 50, "halo": halo})
+
+
+def workload_cd2d_grad_1000():
     """2D acoustic constant-density gradient: 1000 x 1000, dh = 5 m,
     dt = 0.5 ms, nt = 4000, seeded 8-layer model, start model smoothed with
     sigma = 100 m, one 12 Hz Ricker shot at (2500 m, 10 m), 400 receivers at
@@ -47,50 +94,116 @@ def workload_cd2d_grad_1000(dtype=np.float32):
     rng = np.random.default_rng(1)
     vtrue = syn.layered(nx, nz, h, rng)
     v0 = syn.gaussian_smooth(vtrue, 100.0 / h[0])
-    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0)[:, None]
+    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0)[:, None].astype(np.float32)
     rec = syn.surface_receivers(400, nx, h, halo, 10.0)
     params = sw.InputParametersAcoustic(nt, dt, (nx, nz), h, sw.CPMLBoundaryConditionParameters(halo=halo))
-    shots = [sw.ScalarShot(sw.ScalarSources(np.array([[2500.0, 10.0]]), tf.astype(dtype), f0),
-                           sw.ScalarReceivers(rec, nt, dtype=dtype))]
-    return dict(name="acoustic_cd_2d_gradient_1000x1000_nt4000", params=params,
-                mp_true=sw.VpAcousticCDMaterialProperties(vtrue.astype(dtype)),
-                mp0=sw.VpAcousticCDMaterialProperties(v0.astype(dtype)), shots=shots, check_freq=50,
-                physics="cd2d", config={"nx": nx, "nz": nz, "dh_m": 5.0, "dt_s": dt, "nt": nt, "shots": 1,
-                                        "receivers": 400, "check_freq": 50, "halo": halo})
+    shots = [sw.ScalarShot(sw.ScalarSources(np.array([[2500.0, 10.0]]), tf, f0),
+                           sw.ScalarReceivers(rec, nt, dtype=np.float32))]
+    return dict(name="acoustic_cd_2d_gradient_1000x1000_nt4000", params=params, physics="cd2d",
+                mp_true=sw.VpAcousticCDMaterialProperties(vtrue.astype(np.float32)),
+                mp0=sw.VpAcousticCDMaterialProperties(v0.astype(np.float32)), shots=shots, check_freq=50, run={},
+                config={"nx": nx, "nz": nz, "dh_m": 5.0, "dt_s": dt, "nt": nt, "shots": 1, "receivers": 400,
+                        "check_freq": 50, "halo": halo})
 
 
-WORKLOADS = {"cd2d_grad_1000": workload_cd2d_grad_1000}
+WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_cd2d_grad_1000}
 
 
 # ---------------------------------------------------------------------------
-# Algorithmic bytes of the dominant (stencil) kernel
+# Algorithmic bytes per launch of the step kernels (each array counted once
+# per launch, perfect halo reuse).  e = bytes per value.
 # ---------------------------------------------------------------------------
-def stencil_alg_bytes_cd2d(nx, nz, halo, e, nts, S):
-    """Bytes the 2D CD stencil launches of one gradient must move, each
-    array counted once per launch: forward r p^n, r p^{n-1}, r fact,
-    w p^{n+1} (4e per cell); + w D when the replay slot is stored (5e);
-    adjoint with fused correlation adds r D and a grad read-modify-write
-    (7e).  C-PML strips: psi read+write and xi read+write per strip entry
-    (4e per strip cell and axis)."""
+def _strip_cells(nx, nz, halo):
+    # C-PML memory variables touched per launch: psi (2h per axis) and xi
+    # (2h+2 per axis), each read and written once.
+    return (2 * halo * nz + 2 * halo * nx) + ((2 * halo + 2) * nz + (2 * halo + 2) * nx)
+
+
+def alg_bytes(physics, nx, nz, halo, e, nts, S):
+    """Returns {kernel family: (total bytes of one gradient, launches)} for
+    family 0 (forward/replay step kernel) and 1 (adjoint step kernel)."""
     N = nx * nz
-    strip = (2 * halo * nz + 2 * halo * nx) * 4 * e
-    total = 0
+    strip = _strip_cells(nx, nz, halo) * 2 * e
+    fwd_b = fwd_n = adj_b = adj_n = 0
     for nt in nts:
         kl = (nt - 1) // S
-        last = nt - kl * S                     # steps of the last segment (stored in the forward)
-        fwd = (nt - last) * 4 * e * N + last * 5 * e * N
-        rep = (nt - last) * 5 * e * N          # replay of every other segment
-        nseg = kl + 1
-        adj = (nt - nseg) * 7 * e * N + nseg * 4 * e * N
-        total += fwd + rep + adj + (nt + (nt - last) + nt) * strip
-    return total
+        last = nt - kl * S              # steps of the last segment: stored during the forward
+        rep = nt - last                 # steps replayed from checkpoints
+        if physics == "vd2d":
+            # forward: r p, vx, vz, m0, m1x, m1z; w p, vx, vz (9e); + w p-history (10e)
+            fwd_b += (nt - last) * 9 * e * N + last * 10 * e * N + nt * strip
+            fwd_b += rep * 10 * e * N + rep * strip
+            fwd_n += nt + rep
+            # adjoint: 9e + r p^it, p^it-1 (2e) + grad_m0/m1x/m1z read-modify-write (6e)
+            adj_b += nt * 17 * e * N + nt * strip
+            adj_n += nt
+        else:
+            # CD: r p^n, p^n-1, fact; w p^n+1 (4e); + w D (5e).  Adjoint with
+            # the fused correlation: + r D, grad read-modify-write (7e).
+            fwd_b += (nt - last) * 4 * e * N + last * 5 * e * N + nt * strip
+            fwd_b += rep * 5 * e * N + rep * strip
+            fwd_n += nt + rep
+            nseg = kl + 1
+            adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
+            adj_n += nt
+    return {0: (fwd_b, fwd_n), 1: (adj_b, adj_n)}
+
+
+# ---------------------------------------------------------------------------
+# Gradient driver on an already-populated plan
+# ---------------------------------------------------------------------------
+class Runner:
+    def __init__(self, wl, shots, max_batch=None):
+        import seismicwaves as sw
+        p = wl["params"]
+        self.wl = wl
+        self.ws = sw.build_wavesim(p, wl["mp0"], runparams=sw.RunParameters(max_batch=max_batch, **wl["run"]),
+                                   gradparams=sw.GradParameters(check_freq=wl["check_freq"]), gradient=True)
+        self.ws.set_matprop(wl["mp0"])
+        self.descs = [self.ws.shot_desc(sh) for sh in shots]
+        self.misfits = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in shots]
+        self.plan = self.ws._ensure_plan(len(self.descs))
+        self.plan.set_shots(self.descs)
+        self.ntr = sum(d["nt"] * d["rec_idx"].shape[0] for d in self.descs)
+        self.traces = np.zeros(self.ntr, dtype=np.float32)
+        size = sum(int(np.prod(s)) for s in self.ws.grad_fields_shapes())
+        self.grad = np.zeros(size, dtype=np.float32)
+
+    def step(self):
+        self.plan.forward(self.traces, True)
+        off, adj = 0, []
+        for d, m in zip(self.descs, self.misfits):
+            nt, nr = d["nt"], d["rec_idx"].shape[0]
+            syn_ = self.traces[off:off + nt * nr].reshape((nt, nr), order="F")
+            adj.append(np.asfortranarray(-(syn_.astype(np.float64) - m.observed), dtype=np.float32).ravel(order="F"))
+            off += nt * nr
+        self.plan.adjoint(np.concatenate(adj), self.grad, False)
+
+    def timed_kernels(self):
+        """One gradient with HIP events around every step-kernel launch."""
+        self.plan.set_kernel_timing(True)
+        ms, n = [0.0, 0.0], [0, 0]
+        self.plan.forward(self.traces, True)
+        i = self.plan.info()
+        for k in range(2):
+            ms[k] += i.ms_kernel[k]
+            n[k] += i.n_kernel[k]
+        self.plan.adjoint(np.zeros(self.ntr, dtype=np.float32), self.grad, False)
+        i = self.plan.info()
+        for k in range(2):
+            ms[k] += i.ms_kernel[k]
+            n[k] += i.n_kernel[k]
+        self.plan.set_kernel_timing(False)
+        return ms, n
+
+    def close(self):
+        self.plan.close()
 
 
 # ---------------------------------------------------------------------------
 # CPU baseline: oracle OpenMP restatement of the reference's threads backend
 # ---------------------------------------------------------------------------
-def cpu_baseline(wl, nt_sample=400):
-    import ctypes
+def cpu_baseline(wl, nshots_sample=2, nt_sample=400):
     import oracle
     oracle.build()
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
@@ -99,121 +212,131 @@ def cpu_baseline(wl, nt_sample=400):
     nx, nz = p.gridsize
     h = p.gridspacing
     dt = float(p.dt)
-    vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
-    sh = wl["shots"][0]
-    f0 = float(sh.srcs.domfreq)
+    halo, rcoef, ft = p.boundcond.halo, p.boundcond.rcoef, p.boundcond.freeboundtop
     T = np.float32
-    cp = oracle.init_bdc(vp, dt, h, p.boundcond.halo, p.boundcond.rcoef, p.boundcond.freeboundtop, f0, T)
-    si = oracle.find_nearest_grid_points(sh.srcs.positions, h)
-    ri = oracle.find_nearest_grid_points(sh.recs.positions, h)
-    stf = oracle.scale_srctf_cd(sh.srcs.tf[:nt_sample], h, dt, vp, si, T)
-    fact = ((T(dt) ** 2) * vp.astype(T) ** 2).astype(T)
-    adj = np.random.default_rng(0).standard_normal((nt_sample, ri.shape[0])).astype(T) * T(1e-3)
-    t0 = time.perf_counter()
-    oracle.cd2d_gradient(fact, h, dt, p.boundcond.halo, cp, nt_sample, si, stf, ri, adj, wl["check_freq"], T)
-    el = time.perf_counter() - t0
-    return {"value": 2.0 * nx * nz * nt_sample / el / 1e9, "unit": "GCell-updates/s", "cores": cores,
-            "kind": "port",
-            "sample": f"FP32 gradient of the same model, 1 shot, nt={nt_sample} (of {p.ntimesteps}), "
-                      f"check_freq={wl['check_freq']}, OpenMP threads={cores}; restatement of the reference's "
-                      f"threads backend, not the Julia binary; {el:.2f} s"}
+    cf = wl["check_freq"]
+    vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
+    shots = wl["shots"][:nshots_sample]
+    rng = np.random.default_rng(0)
+    el = 0.0
+    for sh in shots:
+        f0 = float(sh.srcs.domfreq)
+        cp = oracle.init_bdc(vp, dt, h, halo, rcoef, ft, f0, T)
+        si = oracle.find_nearest_grid_points(sh.srcs.positions, h)
+        ri = oracle.find_nearest_grid_points(sh.recs.positions, h)
+        adj = (rng.standard_normal((nt_sample, ri.shape[0])) * 1e-3).astype(T)
+        if wl["physics"] == "vd2d":
+            rho = np.asarray(wl["mp0"].rho, dtype=np.float64)
+            m0, m1x, m1z = oracle.vd_model_fields(vp, rho, dt, "arithmetic", T)
+            stf = oracle.scale_srctf_vd(sh.srcs.tf[:nt_sample], h, dt, vp, rho, si, T)
+            t0 = time.perf_counter()
+            oracle.vd2d_gradient(m0, m1x, m1z, h, dt, halo, cp, nt_sample, si, stf, ri, adj, cf, T)
+        else:
+            stf = oracle.scale_srctf_cd(sh.srcs.tf[:nt_sample], h, dt, vp, si, T)
+            fact = ((T(dt) ** 2) * vp.astype(T) ** 2).astype(T)
+            t0 = time.perf_counter()
+            oracle.cd2d_gradient(fact, h, dt, halo, cp, nt_sample, si, stf, ri, adj, cf, T)
+        el += time.perf_counter() - t0
+    work = 2.0 * nx * nz * nt_sample * len(shots)
+    return {"value": round(work / el / 1e9, 4), "unit": "GCell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP32 gradient of the same model: {len(shots)} shot(s) x nt={nt_sample} "
+                      f"(check_freq={cf}), sequential shots, OpenMP threads={cores}; C restatement of the "
+                      f"reference's threads backend (not the Julia binary); {el:.2f} s"}
 
 
 # ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cd2d_grad_1000", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="vd2d_batched", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-one-shot", action="store_true", help="skip the plain 1-shot path")
+    ap.add_argument("--one-shot-sample", type=int, default=2, help="shots timed on the 1-shot path")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU shot sharding is a design note only "
-                                   "(see the sharding section of the design document)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build targets one GPU: shots would be sharded across GPUs with one "
+                                   "gradient all-reduce, which is described but not run here",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import seismicwaves as sw
-    from seismicwaves import _native
-
     wl = WORKLOADS[args.workload]()
     p = wl["params"]
-    dtype = np.float32
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    log("workload", wl["name"])
     # observed data from the true model (outside the timed region)
-    sw.swforward(p, wl["mp_true"], wl["shots"], runparams=sw.RunParameters())
-    misfits = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in wl["shots"]]
-    ws = sw.build_wavesim(p, wl["mp0"], runparams=sw.RunParameters(),
-                          gradparams=sw.GradParameters(check_freq=wl["check_freq"]), gradient=True)
-    ws.set_matprop(wl["mp0"])
-    descs = [ws.shot_desc(sh) for sh in wl["shots"]]
-    plan = ws._ensure_plan(len(descs))
-    plan.set_shots(descs)
-    nts = [d["nt"] for d in descs]
-    ntr = sum(d["nt"] * d["rec_idx"].shape[0] for d in descs)
-    traces = np.zeros(ntr, dtype=dtype)
-    grad = np.zeros(int(np.prod(p.gridsize)), dtype=dtype)
-
-    def one_step():
-        plan.forward(traces, True)
-        off, adj = 0, []
-        for d, m in zip(descs, misfits):
-            nt, nr = d["nt"], d["rec_idx"].shape[0]
-            syn_ = traces[off:off + nt * nr].reshape((nt, nr), order="F")
-            r = syn_.astype(np.float64) - m.observed
-            adj.append(np.asfortranarray(-r, dtype=dtype).ravel(order="F"))
-            off += nt * nr
-        plan.adjoint(np.concatenate(adj), grad, False)
-        inf = plan.info()
-        return inf.ms_forward + inf.ms_adjoint
-
+    sw.swforward(p, wl["mp_true"], wl["shots"], runparams=sw.RunParameters(**wl["run"]))
+    log("observed data ready")
+    run = Runner(wl, wl["shots"])
+    nts = [d["nt"] for d in run.descs]
     for _ in range(args.warmup):
-        one_step()
-    # clean timing (no per-kernel events)
+        run.step()
+    log("warmup done")
     t0 = time.perf_counter()
-    dev_ms = 0.0
     for _ in range(args.steps):
-        dev_ms += one_step()
+        run.step()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
-    # dominant-kernel timing: HIP events around every stencil launch on the
-    # plan's stream, over the same number of steps
-    plan.set_kernel_timing(True)
-    st_ms, st_n = 0.0, 0
-    for _ in range(args.steps):
-        plan.forward(traces, True)
-        i1 = plan.info()
-        st_ms += i1.ms_stencil
-        st_n += i1.n_stencil
-        plan.adjoint(np.zeros(ntr, dtype=dtype), grad, False)
-        i2 = plan.info()
-        st_ms += i2.ms_stencil
-        st_n += i2.n_stencil
-    plan.set_kernel_timing(False)
-    info = plan.info()
-    cells = int(np.prod(p.gridsize))
-    work = 2.0 * cells * sum(nts)
-    value = work / (ms_step / 1e3) / 1e9
+    log(f"timed {args.steps} steps: {ms_step:.1f} ms/step")
+    kms, kn = run.timed_kernels()
+    info = run.plan.info()
+    S = info.segment
+    run.close()
     nx, nz = p.gridsize
-    alg = stencil_alg_bytes_cd2d(nx, nz, p.boundcond.halo, 4, nts, info.segment)
-    launches_per_step = st_n / args.steps
-    avg_launch_ms = st_ms / st_n
-    alg_per_launch = alg / launches_per_step
-    achieved = alg_per_launch / (avg_launch_ms / 1e3) / 1e9
+    cells = nx * nz
+    value = 2.0 * cells * sum(nts) / (ms_step / 1e3) / 1e9
+    ab = alg_bytes(wl["physics"], nx, nz, p.boundcond.halo, 4, nts, S)
+    fam = {}
+    for k, name in ((0, "forward/replay step"), (1, "adjoint step + correlation")):
+        b, n_alg = ab[k]
+        avg_ms = kms[k] / max(kn[k], 1)
+        per_launch = b / max(n_alg, 1)
+        fam[k] = {"name": name, "launches": kn[k], "avg_launch_us": round(avg_ms * 1e3, 2),
+                  "alg_bytes_per_launch": int(per_launch),
+                  "achieved_GBs": round(per_launch / (avg_ms / 1e3) / 1e9, 1), "total_ms": round(kms[k], 1)}
+    dom = max(fam, key=lambda k: fam[k]["total_ms"])
+    kern = {"vd2d": ("vd2d_fwd_kernel<float>", "vd2d_adj_kernel<float>"),
+            "cd2d": ("cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)")}[wl["physics"]]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "GCell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded layered model, Ricker sources)",
-        "config": {"workload": wl["name"], **wl["config"], "segment_used": info.segment},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "cd2d_step_kernel<float>", "avg_launch_us": round(avg_launch_ms * 1e3, 3),
-                     "alg_bytes_per_launch": int(alg_per_launch), "launches_per_step": launches_per_step},
-        "device_ms_per_step": round(dev_ms / args.steps, 3),
+        "data": "synthetic (seeded layered model, Ricker sources, observed data from a forward run)",
+        "config": {"workload": wl["name"], **wl["config"], "segment_used": S},
+        "roofline": {"bound": "hbm", "achieved": fam[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(fam[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": kern[dom], "avg_launch_us": fam[dom]["avg_launch_us"],
+                     "alg_bytes_per_launch": fam[dom]["alg_bytes_per_launch"]},
+        "kernels": {kern[k]: fam[k] for k in fam},
+        "device_ms_per_step": round(info.ms_forward + info.ms_adjoint, 1),
     }
+    if not args.no_one_shot and len(wl["shots"]) > 1:
+        # the same kernels on a plan holding one shot, shots run in sequence
+        sample = wl["shots"][:args.one_shot_sample]
+        one = Runner(wl, sample[:1])
+        one.step()    # warm-up
+        descs = [one.ws.shot_desc(sh) for sh in sample]
+        t0 = time.perf_counter()
+        for d, sh in zip(descs, sample):
+            one.descs = [d]
+            one.misfits = [sw.L2Misfit(sh.recs.seismograms.copy())]
+            one.plan.set_shots([d])
+            one.ntr = d["nt"] * d["rec_idx"].shape[0]
+            one.traces = np.zeros(one.ntr, dtype=np.float32)
+            one.step()
+        el = time.perf_counter() - t0
+        one.close()
+        v1 = 2.0 * cells * sum(d["nt"] for d in descs) / el / 1e9
+        out["one_shot"] = {"value": round(v1, 3), "unit": "GCell-updates/s",
+                           "sample": f"{len(sample)} shots run one at a time on a 1-shot plan",
+                           "batched_over_one_shot": round(value / v1, 3)}
+        log("one-shot path done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 2)
+        log("cpu baseline done")
     print(json.dumps(out), flush=True)
 
 

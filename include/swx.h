@@ -109,9 +109,12 @@ typedef struct {
   int32_t nshots;            /* shots currently set                         */
   double ms_forward;         /* device time of the last swx_forward         */
   double ms_adjoint;         /* device time of the last swx_adjoint         */
-  double ms_stencil;         /* summed device time of the stencil launches
-                                of the last call with kernel timing on      */
-  int64_t n_stencil;         /* number of those launches                    */
+  /* With kernel timing on: summed device time and number of launches of
+   * the forward-step kernel family [0] (forward and replay steps) and of
+   * the adjoint-step kernel family [1], over the last swx_forward or
+   * swx_adjoint call. */
+  double ms_kernel[2];
+  int64_t n_kernel[2];
 } swx_plan_info;
 
 /* Version of this ABI (SWX_ABI_VERSION). No reference counterpart. */

@@ -417,7 +417,7 @@ class Cd2dEngine final : public Engine {
     dim3 grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
     c_.ktimer.begin(c_.stream);
     cd2d_step_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
-    c_.ktimer.end(c_.stream);
+    c_.ktimer.end(c_.stream, a.corr ? 1 : 0);
     SWX_HIP(hipGetLastError());
   }
 

@@ -504,7 +504,7 @@ class Vd2dEngine final : public Engine {
     a.traces = traces;
     c_.ktimer.begin(c_.stream);
     vd2d_fwd_kernel<T><<<grid(), dim3(TX, BY), 0, c_.stream>>>(a);
-    c_.ktimer.end(c_.stream);
+    c_.ktimer.end(c_.stream, 0);
     SWX_HIP(hipGetLastError());
   }
 
@@ -513,7 +513,7 @@ class Vd2dEngine final : public Engine {
     a.cbase = cbase;
     c_.ktimer.begin(c_.stream);
     vd2d_adj_kernel<T><<<grid(), dim3(TX, BY), 0, c_.stream>>>(a);
-    c_.ktimer.end(c_.stream);
+    c_.ktimer.end(c_.stream, 1);
     SWX_HIP(hipGetLastError());
   }
 

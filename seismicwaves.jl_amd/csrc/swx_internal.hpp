@@ -133,14 +133,16 @@ class KernelTimer {
  public:
   ~KernelTimer();
   void begin(hipStream_t s);
-  void end(hipStream_t s);
-  // Sums elapsed times of recorded pairs (stream must be synchronized).
-  double collect_ms(int64_t *count);
-  void reset() { used_ = 0; }
+  // kind 0: forward-step kernel family, 1: adjoint-step kernel family
+  void end(hipStream_t s, int kind);
+  // Sums elapsed times of recorded pairs per kind (stream synchronized).
+  void collect(double ms[2], int64_t n[2]);
+  void reset() { used_ = 0; kinds_.clear(); }
   bool enabled = false;
 
  private:
   std::vector<hipEvent_t> ev_;
+  std::vector<int> kinds_;
   size_t used_ = 0;
 };
 
@@ -170,8 +172,9 @@ struct PlanCore {
   PointTables pts;
   KernelTimer ktimer;
   bool forward_done = false;
-  double ms_forward = 0, ms_adjoint = 0, ms_stencil = 0;
-  int64_t n_stencil = 0;
+  double ms_forward = 0, ms_adjoint = 0;
+  double ms_kernel[2] = {0, 0};
+  int64_t n_kernel[2] = {0, 0};
   size_t budget = 0;  // bytes
 };
 

@@ -49,22 +49,24 @@ void KernelTimer::begin(hipStream_t s) {
   SWX_HIP(hipEventRecord(ev_[used_], s));
 }
 
-void KernelTimer::end(hipStream_t s) {
+void KernelTimer::end(hipStream_t s, int kind) {
   if (!enabled) return;
   SWX_HIP(hipEventRecord(ev_[used_ + 1], s));
+  kinds_.push_back(kind);
   used_ += 2;
 }
 
-double KernelTimer::collect_ms(int64_t *count) {
-  double tot = 0;
+void KernelTimer::collect(double ms[2], int64_t n[2]) {
+  ms[0] = ms[1] = 0;
+  n[0] = n[1] = 0;
   for (size_t k = 0; k + 1 < used_; k += 2) {
-    float ms = 0;
-    SWX_HIP(hipEventElapsedTime(&ms, ev_[k], ev_[k + 1]));
-    tot += ms;
+    float t = 0;
+    SWX_HIP(hipEventElapsedTime(&t, ev_[k], ev_[k + 1]));
+    const int kind = kinds_[k / 2] ? 1 : 0;
+    ms[kind] += t;
+    n[kind] += 1;
   }
-  if (count) *count = (int64_t)(used_ / 2);
-  used_ = 0;
-  return tot;
+  reset();
 }
 
 int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail) {
@@ -305,7 +307,7 @@ int swx_forward(swx_plan *plan, void *traces, int save_state) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c.ms_forward = ms;
-    if (c.ktimer.enabled) c.ms_stencil = c.ktimer.collect_ms(&c.n_stencil);
+    if (c.ktimer.enabled) c.ktimer.collect(c.ms_kernel, c.n_kernel);
     if (traces)
       SWX_HIP(hipMemcpy(traces, c.pts.d_traces.get(), (size_t)c.pts.total_tr * c.esize, hipMemcpyDeviceToHost));
     c.forward_done = save_state != 0;
@@ -334,7 +336,7 @@ int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c.ms_adjoint = ms;
-    if (c.ktimer.enabled) c.ms_stencil = c.ktimer.collect_ms(&c.n_stencil);
+    if (c.ktimer.enabled) c.ktimer.collect(c.ms_kernel, c.n_kernel);
     plan->engine->copy_grad(grad, per_shot != 0);
   });
 }
@@ -350,8 +352,10 @@ int swx_plan_get_info(swx_plan *plan, swx_plan_info *info) {
     info->nshots = plan->core.pts.nshots;
     info->ms_forward = plan->core.ms_forward;
     info->ms_adjoint = plan->core.ms_adjoint;
-    info->ms_stencil = plan->core.ms_stencil;
-    info->n_stencil = plan->core.n_stencil;
+    for (int k = 0; k < 2; k++) {
+      info->ms_kernel[k] = plan->core.ms_kernel[k];
+      info->n_kernel[k] = plan->core.n_kernel[k];
+    }
   });
 }
 

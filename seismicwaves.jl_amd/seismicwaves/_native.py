@@ -48,7 +48,7 @@ class PlanInfo(ctypes.Structure):
         ("bytes_state", ctypes.c_int64), ("bytes_checkpoints", ctypes.c_int64),
         ("bytes_replay", ctypes.c_int64), ("segment", ctypes.c_int32), ("nshots", ctypes.c_int32),
         ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
-        ("ms_stencil", ctypes.c_double), ("n_stencil", ctypes.c_int64),
+        ("ms_kernel", ctypes.c_double * 2), ("n_kernel", ctypes.c_int64 * 2),
     ]
 
 
