@@ -20,6 +20,7 @@
 //   grad   [ns][nx*nz]     per-shot raw gradient (reduced in shot order)
 #include <algorithm>
 
+#include "swx_device.hpp"
 #include "swx_internal.hpp"
 
 namespace swx {
@@ -49,19 +50,13 @@ struct Cd2dStepArgs {
   T *dbuf; int64_t dslot; int dmode; int S; int cbase;
   // correlation of the current (adjoint) field with D slot it-cbase
   T *grad; int corr; int corr_lim;
+  // sources/receivers: forward injects srctf and records traces (nullable);
+  // the adjoint (res != nullptr) injects the residuals prescaled by fact
+  PointArgsView pt;
+  const T *srctf;
+  T *traces;
+  const T *res;
 };
-
-// Packed coefficient block of one shot: for axis x then z,
-//   a (2h+2) | a_h (2h) | b (2h+2) | b_h (2h)
-template <typename T>
-struct AxisCoef {
-  const T *a, *ah, *b, *bh;
-};
-template <typename T>
-__device__ __forceinline__ AxisCoef<T> axis_coef(const T *base, int h, int axis) {
-  const T *p = base + (int64_t)axis * (8 * h + 4);
-  return AxisCoef<T>{p, p + (2 * h + 2), p + (4 * h + 2), p + (6 * h + 4)};
-}
 
 template <typename T>
 __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
@@ -135,7 +130,39 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       lz = d2z + dpsi + xi;
     }
     pnv = T(2) * pc - po_[c] + a.fact[c] * (lx + lz);
-    pn_[c] = pnv;
+  }
+  // Points of this cell: inject_sources! / record_receivers! (forward,
+  // acoustic2D_xPU.jl:48-62) or the prescaled residual injection of the
+  // adjoint (:64-76, :161-162), in the reference's serial order per cell.
+  // Edge cells are not updated by the stencil: their value is the stale
+  // time level plus any injection, as in the reference's buffer rotation.
+  bool wrote = interior;
+  if (a.res == nullptr) {
+    const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c);
+    if (u >= 0) {
+      if (!interior) pnv = pn_[c];
+      const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
+      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pnv += tf[(int64_t)nts * a.pt.usrc_list[k]];
+      wrote = true;
+    }
+  } else {
+    const int u = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
+    if (u >= 0) {
+      if (!interior) pnv = pn_[c];
+      const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
+      const T f = a.fact[c];
+      for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++) pnv += rs[(int64_t)nts * a.pt.urec_list[k]] * f;
+      wrote = true;
+    }
+  }
+  if (wrote) pn_[c] = pnv;
+  if (a.traces) {
+    const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
+    if (r >= 0) {
+      const T v = wrote ? pnv : pn_[c];
+      T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
+      for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = v;
+    }
   }
   if (a.dmode) {
     int slot = -1;
@@ -146,7 +173,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       slot = a.it - a.cbase - 1;
     }
     if (slot >= 0) {
-      const T pn = interior ? pnv : pn_[c];
+      const T pn = wrote ? pnv : pn_[c];
       a.dbuf[(int64_t)slot * a.dslot + so + c] = (po_[c] - T(2) * pc) + pn;
     }
   }
@@ -154,66 +181,6 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
     // correlate_gradient! of the previous adjoint step (adjcur = pcur here)
     const int slot = a.it - a.cbase;
     a.grad[so + c] += pc * a.dbuf[(int64_t)slot * a.dslot + so + c];
-  }
-}
-
-template <typename T>
-struct Cd2dPointArgs {
-  PointArgsView pt;
-  int64_t N;
-  const T *srctf;
-  T *traces;        // forward: record (nullable)
-  const T *res;     // adjoint: residuals (nullable)
-  const T *fact;    // prescale of the residuals at the receivers
-  T *pnew; const T *pcur; const T *pold;
-  int it, act_min;
-  T *dbuf; int64_t dslot; int dmode; int S; int cbase;
-};
-
-// inject_sources! + record_receivers! (acoustic2D_xPU.jl:48-62) for the
-// forward/replay step, or the residual injection of adjoint_onestep_CPML!
-// with prescale_residuals! folded in (acoustic2D_xPU.jl:64-76, :161-162).
-template <typename T>
-__global__ __launch_bounds__(256) void cd2d_point_kernel(Cd2dPointArgs<T> a) {
-  const int s = blockIdx.x;
-  const int nts = a.pt.nt[s];
-  if (a.it > nts || nts <= a.act_min) return;
-  const int64_t so = (int64_t)s * a.N;
-  T *pn = a.pnew + so;
-  const int tid = threadIdx.x;
-  if (a.res == nullptr) {
-    int slot = -1;
-    if (a.dmode == 1) {
-      const int seg0 = ((nts - 1) / a.S) * a.S;
-      if (a.it > seg0) slot = a.it - seg0 - 1;
-    } else if (a.dmode == 2) {
-      slot = a.it - a.cbase - 1;
-    }
-    const int u0 = a.pt.usrc_off[s], u1 = a.pt.usrc_off[s + 1];
-    const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
-    for (int u = u0 + tid; u < u1; u += blockDim.x) {
-      const int64_t cell = a.pt.usrc_cell[u];
-      T v = pn[cell];
-      for (int q = a.pt.usrc_ptr[u]; q < a.pt.usrc_ptr[u + 1]; q++) v += tf[(int64_t)nts * a.pt.usrc_list[q]];
-      pn[cell] = v;
-      if (slot >= 0) a.dbuf[(int64_t)slot * a.dslot + so + cell] = (a.pold[so + cell] - T(2) * a.pcur[so + cell]) + v;
-    }
-    if (a.traces) {
-      __syncthreads();
-      const int r0 = a.pt.rec_off[s], r1 = a.pt.rec_off[s + 1];
-      T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
-      for (int r = r0 + tid; r < r1; r += blockDim.x) tr[(int64_t)nts * (r - r0)] = pn[a.pt.rec_cell[r]];
-    }
-  } else {
-    const int u0 = a.pt.urec_off[s], u1 = a.pt.urec_off[s + 1];
-    const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
-    for (int u = u0 + tid; u < u1; u += blockDim.x) {
-      const int64_t cell = a.pt.urec_cell[u];
-      const T f = a.fact[cell];
-      T v = pn[cell];
-      for (int q = a.pt.urec_ptr[u]; q < a.pt.urec_ptr[u + 1]; q++) v += rs[(int64_t)nts * a.pt.urec_list[q]] * f;
-      pn[cell] = v;
-    }
   }
 }
 
@@ -226,21 +193,6 @@ __global__ void cd2d_correlate_kernel(T *grad, const T *adj, const T *d, int64_t
   const int64_t so = (int64_t)s * N;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x)
     grad[so + q] += adj[so + q] * d[so + q];
-}
-
-// Sum of the per-shot gradients in shot order times 1/dt^2, or a scaled copy
-// per shot.
-template <typename T>
-__global__ void reduce_shots_kernel(T *out, const T *grad, int64_t N, int ns, T scale, int per_shot) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
-    if (per_shot) {
-      for (int s = 0; s < ns; s++) out[(int64_t)s * N + q] = grad[(int64_t)s * N + q] * scale;
-    } else {
-      T acc = 0;
-      for (int s = 0; s < ns; s++) acc += grad[(int64_t)s * N + q] * scale;
-      out[q] = acc;
-    }
-  }
 }
 
 template <typename T>
@@ -325,8 +277,7 @@ class Cd2dEngine final : public Engine {
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
     for (int it = 1; it <= ntmax_; it++) {
-      step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0);
-      point_fwd(it, 0, tr, save ? 1 : 0, 0);
+      step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
       if (save && it % S_ == 0) {
         const int k = it / S_;
         if (k >= 1 && k <= K_ - 2) save_ckpt(k, it);
@@ -352,14 +303,12 @@ class Cd2dEngine final : public Engine {
       if (need) {
         restore_ckpt(k, cb);
         for (int it = cb + 1; it <= cb + S_; it++) {
-          step_fwd(it, cb + S_, 2, cb);
-          point_fwd(it, cb + S_, nullptr, 2, cb);
+          step_fwd(it, cb + S_, 2, cb, nullptr);
         }
       }
       const int top = std::min(cb + S_, ntmax_);
       for (int it = top; it >= cb + 1; it--) {
         step_adj(it, cb);
-        point_adj(it);
       }
       // correlation of the first step of the segment
       dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 1024), nsh_);
@@ -369,7 +318,7 @@ class Cd2dEngine final : public Engine {
     }
     const T scale = T(1.0 / (c_.d.dt * c_.d.dt));
     dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 4096));
-    reduce_shots_kernel<T><<<g, 256, 0, c_.stream>>>(gout_.as<T>(), grad_.as<T>(), N_, nsh_, scale, per_shot ? 1 : 0);
+    reduce_shots<T><<<g, 256, 0, c_.stream>>>(gout_.as<T>(), grad_.as<T>(), N_, nsh_, scale, per_shot ? 1 : 0);
     SWX_HIP(hipGetLastError());
     (void)e;
   }
@@ -409,6 +358,8 @@ class Cd2dEngine final : public Engine {
     a.nt_shot = c_.pts.d_nt.as<int>();
     a.it = it;
     a.dbuf = dbuf_.as<T>(); a.dslot = (int64_t)ns_ * N_; a.S = S_ > 0 ? S_ : 1;
+    a.pt = view(c_.pts);
+    a.srctf = c_.pts.d_srctf.as<T>();
     return a;
   }
 
@@ -421,7 +372,7 @@ class Cd2dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
 
-  void step_fwd(int it, int act_min, int dmode, int cbase) {
+  void step_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
     Cd2dStepArgs<T> a = base_args(it);
     a.pold = P(it - 2); a.pcur = P(it - 1); a.pnew = P(it);
     a.psix_o = psix_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spx_;
@@ -433,6 +384,7 @@ class Cd2dEngine final : public Engine {
     a.dmode = c_.d.gradient ? dmode : 0;
     a.cbase = cbase;
     a.corr = 0;
+    a.traces = traces;
     launch(a);
   }
 
@@ -450,35 +402,8 @@ class Cd2dEngine final : public Engine {
     a.grad = grad_.as<T>();
     a.corr = 1;
     a.corr_lim = cbase + S_;
-    launch(a);
-  }
-
-  void point_fwd(int it, int act_min, T *traces, int dmode, int cbase) {
-    Cd2dPointArgs<T> a{};
-    a.pt = view(c_.pts);
-    a.N = N_;
-    a.srctf = c_.pts.d_srctf.as<T>();
-    a.traces = traces;
-    a.res = nullptr;
-    a.fact = fact_.as<T>();
-    a.pnew = P(it); a.pcur = P(it - 1); a.pold = P(it - 2);
-    a.it = it; a.act_min = act_min;
-    a.dbuf = dbuf_.as<T>(); a.dslot = (int64_t)ns_ * N_;
-    a.dmode = c_.d.gradient ? dmode : 0; a.S = S_ > 0 ? S_ : 1; a.cbase = cbase;
-    cd2d_point_kernel<T><<<nsh_, 256, 0, c_.stream>>>(a);
-    SWX_HIP(hipGetLastError());
-  }
-
-  void point_adj(int it) {
-    Cd2dPointArgs<T> a{};
-    a.pt = view(c_.pts);
-    a.N = N_;
     a.res = c_.pts.d_res.as<T>();
-    a.fact = fact_.as<T>();
-    a.pnew = A(it); a.pcur = A(it + 1); a.pold = A(it + 2);
-    a.it = it; a.act_min = 0;
-    cd2d_point_kernel<T><<<nsh_, 256, 0, c_.stream>>>(a);
-    SWX_HIP(hipGetLastError());
+    launch(a);
   }
 
   // checkpoint layout (all shots): p^{c-1} | p^c | psi_x | psi_z | xi_x | xi_z

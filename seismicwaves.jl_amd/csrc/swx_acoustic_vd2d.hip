@@ -12,19 +12,29 @@
 // Fusion.  The forward step is p <- p - m0 div~ v, inject, v <- v - m1 grad~ p.
 // The velocity update needs the *new* pressure one cell left / two right of
 // each velocity point, so a block recomputes the new pressure on its tile
-// plus that halo in LDS (redundant work on ~20% of the cells, no extra HBM
-// traffic), then updates the velocities of its tile.  Because neighbouring
-// blocks read the old state around their tiles, the state is double
-// buffered by step parity (read parity it-1, write parity it), including
-// the C-PML memory variables.  The adjoint step (v first, then p) is fused
-// the same way around the old adjoint pressure, and the correlation of
-// grad_m0 / grad_m1 with the replayed pressure is done in the same launch.
+// plus that cross-shaped halo in LDS (redundant work on a few % of the
+// cells, no extra HBM traffic), then updates the velocities of its tile.
+// Because neighbouring blocks read the old state around their tiles, the
+// state is double buffered by step parity (read parity it-1, write parity
+// it), including the C-PML memory variables.  The adjoint step (v first,
+// then p) is fused the same way around the old adjoint pressure, and the
+// correlation of grad_m0 / grad_m1 with the replayed pressure is done in
+// the same launch.
+//
+// Memory-level parallelism: every global load a thread needs that does not
+// depend on LDS (old pressure, material factors, halo cells) is issued
+// before the first barrier, so a block has all its HBM requests in flight
+// at once.  Tiles are handed to blocks XCD-aware: the hardware deals blocks
+// round-robin over the 8 XCDs, and the block -> tile map gives each XCD a
+// contiguous run of tiles so the halo rows shared by neighbouring tiles are
+// found in that XCD's L2.
 //
 // Layout: every field is an nx*nz slab per shot with x fastest.  vx has
 // nx-1 valid columns and vz nx*(nz-1) valid rows; the extra column/row is
 // kept at zero, which is exactly the zero extension of the reference's
 // truncated stencils.
 #include <algorithm>
+#include <cstdlib>
 
 #include "swx_device.hpp"
 #include "swx_internal.hpp"
@@ -32,14 +42,19 @@
 namespace swx {
 namespace {
 
-constexpr int TX = 64;   // tile width  (one wavefront)
-constexpr int TZ = 16;   // tile height (4 rows of threads x 4)
+constexpr int TX = 64;   // tile width (one wavefront)
 constexpr int BY = 4;    // thread rows per block
 constexpr int NT = TX * BY;
+
+// tile heights per precision (LDS: forward ~29 KB f32 / 30 KB f64)
+template <typename T> struct Tiles;
+template <> struct Tiles<float> { static constexpr int fwd_max = 32; };
+template <> struct Tiles<double> { static constexpr int fwd_max = 32; };
 
 template <typename T>
 struct Vd2dArgs {
   int nx, nz, h;
+  int nbx, nbz;                  // tiles per shot along x and z
   int64_t N;
   T idx, idz;
   const T *m0, *m1x, *m1z;
@@ -79,14 +94,23 @@ __device__ __forceinline__ int store_base(int smode, int it, int nts, int S, int
   return -1;
 }
 
+// Block -> tile map giving each XCD a contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= per * 8) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
 // Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.
-template <typename T>
+template <typename T, int TZ>
 __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
-  const int s = blockIdx.z;
+  constexpr int R = TZ / BY;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
   const int nx = a.nx, nz = a.nz, h = a.h;
-  const int I0 = blockIdx.x * TX, J0 = blockIdx.y * TZ;
+  const int I0 = bx * TX, J0 = bz * TZ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
@@ -96,6 +120,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   __shared__ T s_vz[TZ + 6][TX + 3];   // vz^{it-1}: x in [I0-1, I0+TX+2), z in [J0-3, J0+TZ+3)
   __shared__ T s_p[TZ + 3][TX + 3];    // p^{it}:    x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2)
 
+  // ---- phase 0: every load that does not depend on LDS -------------------
   for (int q = tid; q < (TZ + 3) * (TX + 6); q += NT) {
     const int lz = q / (TX + 6), lx = q % (TX + 6);
     const int gi = I0 - 3 + lx, gj = J0 - 1 + lz;
@@ -110,70 +135,122 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
     if (gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
     s_vz[lz][lx] = v;
   }
-  __syncthreads();
-
-  const int sb = store_base(a.smode, a.it, nts, a.S, a.cbase);
-  // new pressure on the tile and the cross-shaped halo the velocities need
-  for (int q = tid; q < (TZ + 3) * (TX + 3); q += NT) {
-    const int lz = q / (TX + 3), lx = q % (TX + 3);
-    const bool xarm = lz >= 1 && lz <= TZ, zarm = lx >= 1 && lx <= TX;
-    T val = 0;
-    const int gi = I0 - 1 + lx, gj = J0 - 1 + lz;
-    if ((xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) {
-      const int64_t c = (int64_t)gj * nx + gi;
-      const bool owned = xarm && zarm;
-      const T pold = a.p_o[so + c];
-      T pv = pold;
-      if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= nz - 2) {
-        // update_p_CPML! (acoustic2D_VD_xPU.jl:17-37)
-        T dvx = d4(s_vx[lz][lx], s_vx[lz][lx + 1], s_vx[lz][lx + 2], s_vx[lz][lx + 3]) * a.idx;
-        if (gi <= h || gi >= nx - h - 1) {
-          const int k = gi <= h ? gi : gi - nx + 2 * h + 2;
-          const int64_t qx = (int64_t)s * a.sxx + (int64_t)gj * (2 * h + 2) + k;
-          const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
-          if (owned) a.xix_n[qx] = xi;
-          dvx = dvx + xi;
-        }
-        T dvz = d4(s_vz[lz][lx], s_vz[lz + 1][lx], s_vz[lz + 2][lx], s_vz[lz + 3][lx]) * a.idz;
-        if (gj <= h || gj >= nz - h - 1) {
-          const int k = gj <= h ? gj : gj - nz + 2 * h + 2;
-          const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi;
-          const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
-          if (owned) a.xiz_n[qz] = xi;
-          dvz = dvz + xi;
-        }
-        pv = pv - a.m0[c] * (dvx + dvz);
-      }
-      // inject_sources! (acoustic2D_VD_xPU.jl:1-7), serial order per cell
-      const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, gj, c);
-      if (u >= 0) {
-        const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
-        for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pv += tf[(int64_t)nts * a.pt.usrc_list[k]];
-      }
-      val = pv;
-      if (owned) {
-        a.p_n[so + c] = pv;
-        if (sb >= 0) {
-          a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
-          if (a.it == sb + 1) a.pbuf[so + c] = pold;
-        }
-        if (a.traces) {
-          // record_receivers! (acoustic2D_VD_xPU.jl:9-15)
-          const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, gj, c);
-          if (r >= 0) {
-            T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
-            for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = pv;
-          }
-        }
-      }
+  const int gi = I0 + tx;
+  T rp[R], rm0[R], rm1x[R], rm1z[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int gj = J0 + ty + BY * r;
+    const bool ok = gi < nx && gj < nz;
+    const int64_t c = (int64_t)gj * nx + gi;
+    rp[r] = ok ? a.p_o[so + c] : T(0);
+    rm0[r] = ok ? a.m0[c] : T(0);
+    rm1x[r] = ok ? a.m1x[c] : T(0);
+    rm1z[r] = ok ? a.m1z[c] : T(0);
+  }
+  // halo of the new-pressure region: 3 columns x TZ rows, 3 rows x TX columns
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
+  // does any source lie in the rows whose new pressure this block computes,
+  // or any receiver in its tile rows?  (most blocks: no point lookups)
+  __shared__ int s_pts[2];
+  if (tid == 0) {
+    const int *sr = a.pt.srow + (int64_t)s * (a.pt.nrows + 1);
+    const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+    s_pts[0] = sr[max(J0 - 1, 0)] < sr[min(J0 + TZ + 2, nz)];
+    s_pts[1] = a.traces != nullptr && rr[J0] < rr[min(J0 + TZ, nz)];
+  }
+  T hp[HR], hm0[HR];
+  int hlx[HR], hlz[HR];
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    int lx = -1, lz = -1;
+    if (q < HX) {
+      const int col = q / TZ;
+      lz = 1 + q % TZ;
+      lx = col == 0 ? 0 : TX + col;
+    } else if (q < H) {
+      const int row = (q - HX) / TX;
+      lx = 1 + (q - HX) % TX;
+      lz = row == 0 ? 0 : TZ + row;
     }
-    s_p[lz][lx] = val;
+    T vp = 0, vm = 0;
+    const int gi2 = I0 - 1 + lx, gj2 = J0 - 1 + lz;
+    if (lx >= 0 && gi2 >= 0 && gi2 <= nx - 1 && gj2 >= 0 && gj2 <= nz - 1) {
+      const int64_t c2 = (int64_t)gj2 * nx + gi2;
+      vp = a.p_o[so + c2];
+      vm = a.m0[c2];
+    }
+    hlx[k] = lx;
+    hlz[k] = lz;
+    hp[k] = vp;
+    hm0[k] = vm;
   }
   __syncthreads();
 
-  // update_vx_CPML! / update_vy_CPML! (acoustic2D_VD_xPU.jl:39-75)
-  const int gi = I0 + tx;
-  for (int r = 0; r < TZ / BY; r++) {
+  // ---- phase 1: new pressure (update_p_CPML! + inject_sources!) ----------
+  const int sb = store_base(a.smode, a.it, nts, a.S, a.cbase);
+  const bool any_src = s_pts[0] != 0, any_rec = s_pts[1] != 0;
+  auto new_p = [&](int lx, int lz, T pold, T m0v, bool owned) -> T {
+    const int gi2 = I0 - 1 + lx, gj2 = J0 - 1 + lz;
+    if (gi2 < 0 || gi2 >= nx || gj2 < 0 || gj2 >= nz) return T(0);
+    const int64_t c = (int64_t)gj2 * nx + gi2;
+    T pv = pold;
+    if (gi2 >= 1 && gi2 <= nx - 2 && gj2 >= 1 && gj2 <= nz - 2) {
+      // update_p_CPML! (acoustic2D_VD_xPU.jl:17-37)
+      T dvx = d4(s_vx[lz][lx], s_vx[lz][lx + 1], s_vx[lz][lx + 2], s_vx[lz][lx + 3]) * a.idx;
+      if (gi2 <= h || gi2 >= nx - h - 1) {
+        const int k = gi2 <= h ? gi2 : gi2 - nx + 2 * h + 2;
+        const int64_t qx = (int64_t)s * a.sxx + (int64_t)gj2 * (2 * h + 2) + k;
+        const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+        if (owned) a.xix_n[qx] = xi;
+        dvx = dvx + xi;
+      }
+      T dvz = d4(s_vz[lz][lx], s_vz[lz + 1][lx], s_vz[lz + 2][lx], s_vz[lz + 3][lx]) * a.idz;
+      if (gj2 <= h || gj2 >= nz - h - 1) {
+        const int k = gj2 <= h ? gj2 : gj2 - nz + 2 * h + 2;
+        const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi2;
+        const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+        if (owned) a.xiz_n[qz] = xi;
+        dvz = dvz + xi;
+      }
+      pv = pv - m0v * (dvx + dvz);
+    }
+    // inject_sources! (acoustic2D_VD_xPU.jl:1-7), serial order per cell
+    const int u = any_src ? find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, gj2, c) : -1;
+    if (u >= 0) {
+      const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
+      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pv += tf[(int64_t)nts * a.pt.usrc_list[k]];
+    }
+    if (owned) {
+      a.p_n[so + c] = pv;
+      if (sb >= 0) {
+        a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
+        if (a.it == sb + 1) a.pbuf[so + c] = pold;
+      }
+      if (any_rec) {
+        // record_receivers! (acoustic2D_VD_xPU.jl:9-15)
+        const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, gj2, c);
+        if (r >= 0) {
+          T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
+          for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = pv;
+        }
+      }
+    }
+    return pv;
+  };
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r + 1;
+    s_p[lz][tx + 1] = new_p(tx + 1, lz, rp[r], rm0[r], true);
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++)
+    if (hlx[k] >= 0) s_p[hlz[k]][hlx[k]] = new_p(hlx[k], hlz[k], hp[k], hm0[k], false);
+  __syncthreads();
+
+  // ---- phase 2: update_vx_CPML! / update_vy_CPML! (:39-75) ---------------
+#pragma unroll
+  for (int r = 0; r < R; r++) {
     const int lz = ty + BY * r;
     const int gj = J0 + lz;
     if (gi >= nx || gj >= nz) continue;
@@ -187,7 +264,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
         a.psx_n[q] = ps;
         d = d + ps;
       }
-      a.vx_n[so + c] = s_vx[lz + 1][tx + 3] - a.m1x[c] * d;
+      a.vx_n[so + c] = s_vx[lz + 1][tx + 3] - rm1x[r] * d;
     }
     if (gj <= nz - 2) {
       T d = d4(s_p[lz][tx + 1], s_p[lz + 1][tx + 1], s_p[lz + 2][tx + 1], s_p[lz + 3][tx + 1]) * a.idz;
@@ -198,29 +275,36 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
         a.psz_n[q] = ps;
         d = d + ps;
       }
-      a.vz_n[so + c] = s_vz[lz + 3][tx + 1] - a.m1z[c] * d;
+      a.vz_n[so + c] = s_vz[lz + 3][tx + 1] - rm1z[r] * d;
     }
   }
 }
 
 // Adjoint step it: (p~, v~)^{it+1} -> (p~, v~)^{it}, then the correlation
 // with p^{it}, p^{it-1} of the replayed segment.
-template <typename T>
+template <typename T, int TZ>
 __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
-  const int s = blockIdx.z;
+  constexpr int R = TZ / BY;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
   const int nts = a.nt_shot[s];
   if (a.it > nts) return;
   const int nx = a.nx, nz = a.nz, h = a.h;
-  const int I0 = blockIdx.x * TX, J0 = blockIdx.y * TZ;
+  const int I0 = bx * TX, J0 = bz * TZ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  const int slot = a.it - a.cbase;
+  const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
+  const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
 
   __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
+  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
   __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
   __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
 
+  // ---- phase 0 -------------------------------------------------------------
   for (int q = tid; q < (TZ + 6) * (TX + 6); q += NT) {
     const int lz = q / (TX + 6), lx = q % (TX + 6);
     const bool xarm = lz >= 3 && lz < TZ + 3, zarm = lx >= 3 && lx < TX + 3;
@@ -229,57 +313,125 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
     if ((xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) v = a.p_o[so + (int64_t)gj * nx + gi];
     s_p[lz][lx] = v;
   }
-  __syncthreads();
-
-  // velocities first (adjoint_onestep_CPML!, acoustic2D_VD_xPU.jl:163-171)
-  for (int q = tid; q < TZ * (TX + 3); q += NT) {
+  for (int q = tid; q < (TZ + 3) * (TX + 3); q += NT) {
     const int lz = q / (TX + 3), lx = q % (TX + 3);
-    const int gi = I0 - 2 + lx, gj = J0 + lz;
+    const bool xarm = lz >= 1 && lz <= TZ, zarm = lx >= 1 && lx <= TX;
+    const int gi = I0 - 1 + lx, gj = J0 - 1 + lz;
     T v = 0;
-    if (gi >= 0 && gi <= nx - 2 && gj <= nz - 1) {
-      const int64_t c = (int64_t)gj * nx + gi;
-      const bool owned = lx >= 2 && lx < TX + 2;
-      T d = d4(s_p[lz + 3][lx], s_p[lz + 3][lx + 1], s_p[lz + 3][lx + 2], s_p[lz + 3][lx + 3]) * a.idx;
-      if (gi <= h - 1 || gi >= nx - h - 1) {
-        const int k = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
-        const int64_t qq = (int64_t)s * a.spx + (int64_t)gj * (2 * h) + k;
-        const T ps = cx.bh[k] * a.psx_o[qq] + cx.ah[k] * d;
-        if (owned) a.psx_n[qq] = ps;
-        d = d + ps;
-      }
-      v = a.vx_o[so + c] - a.m1x[c] * d;
-      if (owned) a.vx_n[so + c] = v;
-    }
-    s_vx[lz][lx] = v;
+    if ((xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) v = pc_[(int64_t)gj * nx + gi];
+    s_pc[lz][lx] = v;
   }
-  for (int q = tid; q < (TZ + 3) * TX; q += NT) {
-    const int lz = q / TX, lx = q % TX;
-    const int gi = I0 + lx, gj = J0 - 2 + lz;
-    T v = 0;
-    if (gi <= nx - 1 && gj >= 0 && gj <= nz - 2) {
-      const int64_t c = (int64_t)gj * nx + gi;
-      const bool owned = lz >= 2 && lz < TZ + 2;
-      T d = d4(s_p[lz][lx + 3], s_p[lz + 1][lx + 3], s_p[lz + 2][lx + 3], s_p[lz + 3][lx + 3]) * a.idz;
-      if (gj <= h - 1 || gj >= nz - h - 1) {
-        const int k = gj <= h - 1 ? gj : gj + 1 - nz + 2 * h;
-        const int64_t qq = (int64_t)s * a.spz + (int64_t)k * nx + gi;
-        const T ps = cz.bh[k] * a.psz_o[qq] + cz.ah[k] * d;
-        if (owned) a.psz_n[qq] = ps;
-        d = d + ps;
+  const int gi = I0 + tx;
+  T rvx[R], rvz[R], rm1x[R], rm1z[R], rm0[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int gj = J0 + ty + BY * r;
+    const bool ok = gi < nx && gj < nz;
+    const int64_t c = (int64_t)gj * nx + gi;
+    rvx[r] = ok ? a.vx_o[so + c] : T(0);
+    rvz[r] = ok ? a.vz_o[so + c] : T(0);
+    rm1x[r] = ok ? a.m1x[c] : T(0);
+    rm1z[r] = ok ? a.m1z[c] : T(0);
+    rm0[r] = ok ? a.m0[c] : T(0);
+    rpo[r] = ok ? po_[c] : T(0);
+    rg0[r] = ok ? a.g0[so + c] : T(0);
+    rg1x[r] = ok ? a.g1x[so + c] : T(0);
+    rg1z[r] = ok ? a.g1z[so + c] : T(0);
+  }
+  // velocity halo: v~x columns I0-2, I0-1, I0+TX on tile rows; v~z rows
+  // J0-2, J0-1, J0+TZ on tile columns
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
+  __shared__ int s_rec;
+  if (tid == 0) {
+    const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+    s_rec = rr[J0] < rr[min(J0 + TZ, nz)];
+  }
+  T hv[HR], hm[HR];
+  int hq[HR];
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    T v = 0, m = 0;
+    if (q < HX) {
+      const int col = q / TZ, lz = q % TZ;
+      const int lx = col < 2 ? col : TX + 2;
+      const int gi2 = I0 - 2 + lx, gj2 = J0 + lz;
+      if (gi2 >= 0 && gi2 <= nx - 2 && gj2 <= nz - 1) {
+        const int64_t c2 = (int64_t)gj2 * nx + gi2;
+        v = a.vx_o[so + c2];
+        m = a.m1x[c2];
       }
-      v = a.vz_o[so + c] - a.m1z[c] * d;
-      if (owned) a.vz_n[so + c] = v;
+    } else if (q < H) {
+      const int row = (q - HX) / TX, lx = (q - HX) % TX;
+      const int lz = row < 2 ? row : TZ + 2;
+      const int gi2 = I0 + lx, gj2 = J0 - 2 + lz;
+      if (gi2 <= nx - 1 && gj2 >= 0 && gj2 <= nz - 2) {
+        const int64_t c2 = (int64_t)gj2 * nx + gi2;
+        v = a.vz_o[so + c2];
+        m = a.m1z[c2];
+      }
     }
-    s_vz[lz][lx] = v;
+    hq[k] = q;
+    hv[k] = v;
+    hm[k] = m;
   }
   __syncthreads();
 
-  // pressure (update_p_CPML!), residual injection and correlation
-  const int gi = I0 + tx;
-  const int slot = a.it - a.cbase;
-  const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;
-  const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;
-  for (int r = 0; r < TZ / BY; r++) {
+  // ---- phase 1: velocities first (adjoint_onestep_CPML!, :163-171) --------
+  auto new_vx = [&](int lx, int lz, T vold, T m1, bool owned) -> T {
+    const int gi2 = I0 - 2 + lx, gj2 = J0 + lz;
+    if (gi2 < 0 || gi2 > nx - 2 || gj2 > nz - 1) return T(0);
+    T d = d4(s_p[lz + 3][lx], s_p[lz + 3][lx + 1], s_p[lz + 3][lx + 2], s_p[lz + 3][lx + 3]) * a.idx;
+    if (gi2 <= h - 1 || gi2 >= nx - h - 1) {
+      const int k = gi2 <= h - 1 ? gi2 : gi2 + 1 - nx + 2 * h;
+      const int64_t qq = (int64_t)s * a.spx + (int64_t)gj2 * (2 * h) + k;
+      const T ps = cx.bh[k] * a.psx_o[qq] + cx.ah[k] * d;
+      if (owned) a.psx_n[qq] = ps;
+      d = d + ps;
+    }
+    const T v = vold - m1 * d;
+    if (owned) a.vx_n[so + (int64_t)gj2 * nx + gi2] = v;
+    return v;
+  };
+  auto new_vz = [&](int lx, int lz, T vold, T m1, bool owned) -> T {
+    const int gi2 = I0 + lx, gj2 = J0 - 2 + lz;
+    if (gi2 > nx - 1 || gj2 < 0 || gj2 > nz - 2) return T(0);
+    T d = d4(s_p[lz][lx + 3], s_p[lz + 1][lx + 3], s_p[lz + 2][lx + 3], s_p[lz + 3][lx + 3]) * a.idz;
+    if (gj2 <= h - 1 || gj2 >= nz - h - 1) {
+      const int k = gj2 <= h - 1 ? gj2 : gj2 + 1 - nz + 2 * h;
+      const int64_t qq = (int64_t)s * a.spz + (int64_t)k * nx + gi2;
+      const T ps = cz.bh[k] * a.psz_o[qq] + cz.ah[k] * d;
+      if (owned) a.psz_n[qq] = ps;
+      d = d + ps;
+    }
+    const T v = vold - m1 * d;
+    if (owned) a.vz_n[so + (int64_t)gj2 * nx + gi2] = v;
+    return v;
+  };
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r;
+    s_vx[lz][tx + 2] = new_vx(tx + 2, lz, rvx[r], rm1x[r], true);
+    s_vz[lz + 2][tx] = new_vz(tx, lz + 2, rvz[r], rm1z[r], true);
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = hq[k];
+    if (q < HX) {
+      const int col = q / TZ, lz = q % TZ;
+      const int lx = col < 2 ? col : TX + 2;
+      s_vx[lz][lx] = new_vx(lx, lz, hv[k], hm[k], false);
+    } else if (q < H) {
+      const int row = (q - HX) / TX, lx = (q - HX) % TX;
+      const int lz = row < 2 ? row : TZ + 2;
+      s_vz[lz][lx] = new_vz(lx, lz, hv[k], hm[k], false);
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: pressure (update_p_CPML!), residual injection, correlation
+#pragma unroll
+  for (int r = 0; r < R; r++) {
     const int lz = ty + BY * r;
     const int gj = J0 + lz;
     if (gi >= nx || gj >= nz) continue;
@@ -302,31 +454,27 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
         a.xiz_n[qz] = xi;
         dvz = dvz + xi;
       }
-      pv = pv - a.m0[c] * (dvx + dvz);
+      pv = pv - rm0[r] * (dvx + dvz);
     }
     // residual injection with prescale_residuals! by fact_m0 (:77-89)
-    const int u = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, gj, c);
+    const int u = s_rec ? find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, gj, c) : -1;
     if (u >= 0) {
       const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
-      const T f = a.m0[c];
-      for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++) pv += rs[(int64_t)nts * a.pt.urec_list[k]] * f;
+      for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++)
+        pv += rs[(int64_t)nts * a.pt.urec_list[k]] * rm0[r];
     }
     a.p_n[so + c] = pv;
     // correlate_gradient_m0! (correlate_gradient_xPU.jl:12-21)
-    const T pcv = pc_[c];
-    a.g0[so + c] = a.g0[so + c] - (pv * (pcv - po_[c]) * a.idt);
+    const T pcv = s_pc[lz + 1][tx + 1];
+    a.g0[so + c] = rg0[r] - (pv * (pcv - rpo[r]) * a.idt);
     // correlate_gradient_m1! (acoustic2D_VD_xPU.jl:180-198), 4th order, no C-PML
     if (gi <= nx - 2) {
-      const T pm = gi >= 1 ? pc_[c - 1] : T(0);
-      const T pp2 = gi + 2 <= nx - 1 ? pc_[c + 2] : T(0);
-      const T dpx = d4(pm, pcv, pc_[c + 1], pp2) * a.idx;
-      a.g1x[so + c] = a.g1x[so + c] + s_vx[lz][tx + 2] * dpx;
+      const T dpx = d4(s_pc[lz + 1][tx], pcv, s_pc[lz + 1][tx + 2], s_pc[lz + 1][tx + 3]) * a.idx;
+      a.g1x[so + c] = rg1x[r] + s_vx[lz][tx + 2] * dpx;
     }
     if (gj <= nz - 2) {
-      const T pm = gj >= 1 ? pc_[c - nx] : T(0);
-      const T pp2 = gj + 2 <= nz - 1 ? pc_[c + 2 * (int64_t)nx] : T(0);
-      const T dpz = d4(pm, pcv, pc_[c + nx], pp2) * a.idz;
-      a.g1z[so + c] = a.g1z[so + c] + s_vz[lz + 2][tx] * dpz;
+      const T dpz = d4(s_pc[lz][tx + 1], pcv, s_pc[lz + 2][tx + 1], s_pc[lz + 3][tx + 1]) * a.idz;
+      a.g1z[so + c] = rg1z[r] + s_vz[lz + 2][tx] * dpz;
     }
   }
 }
@@ -400,6 +548,10 @@ class Vd2dEngine final : public Engine {
   void forward(bool save) override {
     SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
     zero(st_);
+    if (!tz_fwd_) {
+      const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", {8, 16, Tiles<T>::fwd_max}, [&](int tz) { launch_fwd(a1, tz); });
+    }
     for (int it = 1; it <= ntmax_; it++) {
       fwd_step(it, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
       if (save && it % S_ == 0) {
@@ -410,9 +562,14 @@ class Vd2dEngine final : public Engine {
   }
 
   void adjoint(bool per_shot) override {
+    const int Kmax = (int)ceil_div(ntmax_, S_);
+    if (!tz_adj_) {
+      // the top step of the last segment reads only forward-stored history
+      const Vd2dArgs<T> a1 = adj_args(ntmax_, (Kmax - 1) * S_);
+      tz_adj_ = tune("SWX_VD2D_TILE_ADJ", {8, 16}, [&](int tz) { launch_adj(a1, tz); });
+    }
     zero(ad_);
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
-    const int Kmax = (int)ceil_div(ntmax_, S_);
     for (int k = Kmax - 1; k >= 0; k--) {
       const int cb = k * S_;
       bool need = false;
@@ -494,28 +651,87 @@ class Vd2dEngine final : public Engine {
     return a;
   }
 
-  dim3 grid() const { return dim3((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_); }
+  // 1-D grid of tiles x shots; the kernels map blocks to tiles XCD-aware
+  unsigned grid(Vd2dArgs<T> &a, int tz) const {
+    a.nbx = (int)ceil_div(nx_, TX);
+    a.nbz = (int)ceil_div(nz_, tz);
+    return (unsigned)((int64_t)a.nbx * a.nbz * nsh_);
+  }
 
-  void fwd_step(int it, int act_min, int smode, int cbase, T *traces) {
+  // Tile-height variants; the autotuner picks one per kernel and plan.
+  void launch_fwd(Vd2dArgs<T> a, int tz) {
+    const unsigned g = grid(a, tz);
+    if (tz == 8) vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    else if (tz == 16) vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    else vd2d_fwd_kernel<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    SWX_HIP(hipGetLastError());
+  }
+  void launch_adj(Vd2dArgs<T> a, int tz) {
+    const unsigned g = grid(a, tz);
+    if (tz == 8) vd2d_adj_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    else vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    SWX_HIP(hipGetLastError());
+  }
+
+  Vd2dArgs<T> fwd_args(int it, int act_min, int smode, int cbase, T *traces) const {
     Vd2dArgs<T> a = args(st_, it, (it - 1) & 1, it & 1);
     a.act_min = act_min;
     a.smode = smode;
     a.cbase = cbase;
     a.traces = traces;
+    return a;
+  }
+  Vd2dArgs<T> adj_args(int it, int cbase) const {
+    Vd2dArgs<T> a = args(ad_, it, (it + 1) & 1, it & 1);
+    a.cbase = cbase;
+    return a;
+  }
+
+  void fwd_step(int it, int act_min, int smode, int cbase, T *traces) {
+    const Vd2dArgs<T> a = fwd_args(it, act_min, smode, cbase, traces);
     c_.ktimer.begin(c_.stream);
-    vd2d_fwd_kernel<T><<<grid(), dim3(TX, BY), 0, c_.stream>>>(a);
+    launch_fwd(a, tz_fwd_);
     c_.ktimer.end(c_.stream, 0);
-    SWX_HIP(hipGetLastError());
   }
 
   void adj_step(int it, int cbase) {
-    Vd2dArgs<T> a = args(ad_, it, (it + 1) & 1, it & 1);
-    a.cbase = cbase;
+    const Vd2dArgs<T> a = adj_args(it, cbase);
     c_.ktimer.begin(c_.stream);
-    vd2d_adj_kernel<T><<<grid(), dim3(TX, BY), 0, c_.stream>>>(a);
+    launch_adj(a, tz_adj_);
     c_.ktimer.end(c_.stream, 1);
-    SWX_HIP(hipGetLastError());
   }
+
+  // Times each candidate tile height on a real launch (a step depends only
+  // on the previous parity, so repeating it is harmless; the adjoint tuning
+  // runs before the adjoint state and gradients are cleared) and returns the
+  // fastest.  SWX_VD2D_TILE_FWD / _ADJ pin a value instead.
+  template <typename F>
+  int tune(const char *env, const std::vector<int> &cands, F launch) {
+    if (const char *v = std::getenv(env)) {
+      const int t = std::atoi(v);
+      for (int c : cands)
+        if (c == t) return t;
+    }
+    hipEvent_t e0, e1;
+    SWX_HIP(hipEventCreate(&e0));
+    SWX_HIP(hipEventCreate(&e1));
+    int best = cands.front();
+    float bestms = 1e30f;
+    for (int c : cands) {
+      launch(c);
+      SWX_HIP(hipEventRecord(e0, c_.stream));
+      for (int r = 0; r < 3; r++) launch(c);
+      SWX_HIP(hipEventRecord(e1, c_.stream));
+      SWX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < bestms) { bestms = ms; best = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+  }
+  int tz_fwd_ = 0, tz_adj_ = 0;
 
   // checkpoint k (all shots): p, vx, vz, psi_x, psi_z, xi_x, xi_z at parity cb&1
   void ckpt_copy(int k, int cb, bool save) {
