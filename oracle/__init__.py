@@ -489,3 +489,93 @@ def acoustic_vd_gradient(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, 
         misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
         seis.append(tr)
     return {"vp": gvp, "rho": grho}, misfit, seis
+
+
+# ---------------------------------------------------------------------------
+# 3D acoustic constant density
+# ---------------------------------------------------------------------------
+class _Cpml3(ctypes.Structure):
+    _fields_ = [("ax", _Cpml * 3)]
+
+
+def _cpml3(cpml, dtype):
+    keep = []
+    st = (_Cpml * 3)()
+    for d in range(3):
+        s, arrs = _cpml_struct(cpml[d], dtype)
+        st[d] = s
+        keep += arrs
+    return st, keep
+
+
+def cd3d_forward(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, dtype=np.float64):
+    """swforward_1shot! for 3D CD (acoustic3D_xPU.jl) on prepared inputs."""
+    fact = _f_order(fact, dtype)
+    g = _geom(fact.shape, spacing, dt, halo)
+    c, keep = _cpml3(cpml, dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    rc = _fn("oracle_cd3d_forward", dtype)(
+        ctypes.byref(g), _ptr(fact), c, ctypes.c_int(nt), ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf),
+        ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(traces))
+    assert rc == 0
+    return np.ascontiguousarray(traces)
+
+
+def cd3d_gradient(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, adjsrc, check_freq, dtype=np.float64):
+    fact = _f_order(fact, dtype)
+    g = _geom(fact.shape, spacing, dt, halo)
+    c, keep = _cpml3(cpml, dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    grad = np.zeros(fact.shape, dtype=dtype, order="F")
+    rc = _fn("oracle_cd3d_gradient", dtype)(
+        ctypes.byref(g), _ptr(fact), c, ctypes.c_int(nt), ctypes.c_int(src.shape[0]), _ptr(src), _ptr(stf),
+        ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(adj), ctypes.c_int(check_freq), _ptr(grad), _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return np.array(grad), np.ascontiguousarray(traces)
+
+
+def acoustic_cd3d_forward(vp, spacing, dt, nt, halo, rcoef, freetop, shots, dtype=np.float64):
+    """run_swforward! for CD 3D."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2
+    out = []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_cd(sh["srctf"][:nt_s], spacing, dt, vp, si, T)
+        out.append(cd3d_forward(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, T))
+    return out
+
+
+def acoustic_cd3d_gradient(vp, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1, dtype=np.float64):
+    """run_swgradient! for CD 3D; returns ({"vp"}, misfit, seismograms)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2
+    tot = np.zeros(vp.shape, dtype=T)
+    misfit, seis = 0.0, []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+        si = find_nearest_grid_points(sh["src_pos"], spacing)
+        ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+        stf = scale_srctf_cd(sh["srctf"][:nt_s], spacing, dt, vp, si, T)
+        syn = cd3d_forward(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
+        g, tr = cd3d_gradient(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, adj.astype(T),
+                              min(check_freq, nt_s - 1), T)
+        tot += (T(2.0) / vp ** 3) * g
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return {"vp": tot}, misfit, seis

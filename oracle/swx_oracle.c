@@ -21,13 +21,16 @@ typedef struct {
 #define SFX(x) x##_f64
 typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
+#include "oracle_cd3d.h"
 #undef real
 #undef SFX
+#undef A3
 
 #define real float
 #define SFX(x) x##_f32
 typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
+#include "oracle_cd3d.h"
 #undef real
 #undef SFX
 

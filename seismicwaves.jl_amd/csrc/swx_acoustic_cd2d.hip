@@ -101,7 +101,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   const bool interior = (i >= 1 && i <= nx - 2 && j >= 1 && j <= nz - 2);
   const T *po_ = a.pold + so;
   T *pn_ = a.pnew + so;
-  T pnv = 0;
+  T pnv = 0, flap = 0, inj = 0;
   if (interior) {
     // update_p_CPML! (acoustic2D_xPU.jl:27-46)
     const T pl = pc_[c - 1], pr = pc_[c + 1], pd = pc_[c - nx], pu = pc_[c + nx];
@@ -129,7 +129,8 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       a.xiz[qz] = xi;
       lz = d2z + dpsi + xi;
     }
-    pnv = T(2) * pc - po_[c] + a.fact[c] * (lx + lz);
+    flap = a.fact[c] * (lx + lz);
+    pnv = T(2) * pc - po_[c] + flap;
   }
   // Points of this cell: inject_sources! / record_receivers! (forward,
   // acoustic2D_xPU.jl:48-62) or the prescaled residual injection of the
@@ -142,7 +143,11 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
     if (u >= 0) {
       if (!interior) pnv = pn_[c];
       const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
-      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pnv += tf[(int64_t)nts * a.pt.usrc_list[k]];
+      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) {
+        const T v = tf[(int64_t)nts * a.pt.usrc_list[k]];
+        pnv += v;
+        inj += v;
+      }
       wrote = true;
     }
   } else {
@@ -174,7 +179,9 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
     }
     if (slot >= 0) {
       const T pn = wrote ? pnv : pn_[c];
-      a.dbuf[(int64_t)slot * a.dslot + so + c] = (po_[c] - T(2) * pc) + pn;
+      // p^{it} - 2 p^{it-1} + p^{it-2}: in the interior exactly the stencil
+      // term plus the injection, formed without cancellation
+      a.dbuf[(int64_t)slot * a.dslot + so + c] = interior ? flap + inj : (po_[c] - T(2) * pc) + pn;
     }
   }
   if (a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim) {

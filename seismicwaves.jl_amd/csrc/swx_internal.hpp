@@ -180,6 +180,7 @@ struct PlanCore {
 
 std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core);
 std::unique_ptr<Engine> make_acoustic_vd2d(PlanCore &core);
+std::unique_ptr<Engine> make_acoustic_cd3d(PlanCore &core);
 
 // Helpers
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -242,6 +242,8 @@ int swx_plan_create(const swx_plan_desc *desc, swx_plan **out) {
     p->core.budget = d.mem_budget > 0 ? (size_t)d.mem_budget : (size_t)(0.9 * (double)fr);
     if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 2) {
       p->engine = make_acoustic_cd2d(p->core);
+    } else if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 3) {
+      p->engine = make_acoustic_cd3d(p->core);
     } else if (d.physics == SWX_ACOUSTIC_VD && d.ndim == 2) {
       p->engine = make_acoustic_vd2d(p->core);
     } else {

@@ -1,0 +1,114 @@
+"""3D acoustic constant density on the GPU, through the C ABI, against the
+FP64 oracle: seismograms and gradients as relative L2 error, <= 1e-10 for
+FP64 and <= 1e-4 for FP32."""
+import numpy as np
+import pytest
+
+import seismicwaves as sw
+from seismicwaves import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-10, np.float32: 1e-4}
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def _problem(dtype, nts=(230, 260), n=(70, 52, 45), freetop=True, seed=9, halo=8):
+    h, dt, f0 = (10.0, 10.0, 10.0), 1e-3, 8.0
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    vtrue = 2000 + 400 * rng.random(n)
+    vtrue = syn.gaussian_smooth(vtrue, 2.0)
+    v0 = np.full(n, float(vtrue.mean()))
+    v0[1, 1, 1] = float(vtrue.max())
+    ntmax = max(nts)
+    tf = syn.ricker_tf(ntmax, dt, f0, 1.2 / f0)[:, None]
+    shots, osh = [], []
+    for k, nt in enumerate(nts):
+        src = np.array([[(20 + 15 * k) * h[0], 25 * h[1], 12 * h[2]]])
+        rec = np.array([[x * h[0], y * h[1], 30.0] for x in range(12, 60, 6) for y in (15, 26, 37)])
+        shots.append(sw.ScalarShot(sw.ScalarSources(src, tf[:nt].astype(dtype), f0),
+                                   sw.ScalarReceivers(rec, nt, dtype=dtype)))
+        osh.append(dict(src_pos=src, srctf=tf[:nt], f0=f0, rec_pos=rec, nt=nt))
+    params = sw.InputParametersAcoustic(ntmax, dt, n, h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=freetop))
+    return dict(params=params, vtrue=vtrue, v0=v0, shots=shots, osh=osh, h=h, dt=dt, halo=halo, freetop=freetop,
+                nts=nts)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_forward_batched(oracle_mod, dtype):
+    print(f"[cd3d forward {np.dtype(dtype).name}]", flush=True)
+    p = _problem(dtype)
+    sw.swforward(p["params"], sw.VpAcousticCDMaterialProperties(p["vtrue"].astype(dtype)), p["shots"],
+                 runparams=sw.RunParameters())
+    ref = oracle_mod.acoustic_cd3d_forward(p["vtrue"], p["h"], p["dt"], max(p["nts"]), p["halo"], 1e-4,
+                                           p["freetop"], p["osh"])
+    for sh, r in zip(p["shots"], ref):
+        err = rel_l2(sh.recs.seismograms, r)
+        print(f"rel-L2 seismograms = {err:.3e}")
+        assert np.max(np.abs(r)) > 0
+        assert err <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_gradient_batched(oracle_mod, dtype):
+    print(f"[cd3d gradient {np.dtype(dtype).name}]", flush=True)
+    p = _problem(dtype)
+    obs = oracle_mod.acoustic_cd3d_forward(p["vtrue"], p["h"], p["dt"], max(p["nts"]), p["halo"], 1e-4,
+                                           p["freetop"], p["osh"])
+    mis = [sw.L2Misfit(o.astype(dtype)) for o in obs]
+    grad, misfit = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"].astype(dtype)), p["shots"],
+                                 mis, runparams=sw.RunParameters(),
+                                 gradparams=sw.GradParameters(check_freq=40, compute_misfit=True))
+    osh = [dict(o, observed=ob) for o, ob in zip(p["osh"], obs)]
+    gref, mref, _ = oracle_mod.acoustic_cd3d_gradient(p["v0"], p["h"], p["dt"], max(p["nts"]), p["halo"], 1e-4,
+                                                      p["freetop"], osh, check_freq=40)
+    err = rel_l2(grad["vp"], gref["vp"])
+    print(f"rel-L2 gradient = {err:.3e}")
+    assert np.max(np.abs(gref["vp"])) > 0
+    assert err <= TOL[dtype]
+    assert abs(misfit - mref) <= TOL[dtype] * abs(mref)
+
+
+def test_zchunk_and_checkpoints_agree_bitwise(oracle_mod, monkeypatch):
+    """The z-chunking of the marching kernel and the checkpoint interval do
+    not change results."""
+    p = _problem(np.float32, nts=(200,))
+    rng = np.random.default_rng(0)
+    mis = [sw.L2Misfit(rng.standard_normal(sh.recs.seismograms.shape).astype(np.float32) * 1e-3) for sh in p["shots"]]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"].astype(np.float32))
+    outs = []
+    for chunk, cf in (("32", 1), ("7", 50), ("45", 13)):
+        monkeypatch.setenv("SWX_CD3D_ZCHUNK", chunk)
+        outs.append(sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                                  gradparams=sw.GradParameters(check_freq=cf))["vp"])
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_point_edge_cases(oracle_mod):
+    """Colocated sources and receivers on the outermost planes."""
+    n, h, dt, nt, f0, halo = (40, 36, 30), (10.0, 10.0, 10.0), 1e-3, 180, 8.0, 6
+    rng = np.random.default_rng(3)
+    vtrue = 2000 + 300 * rng.random(n)
+    v0 = np.full(n, 2100.0)
+    tf = np.stack([syn.ricker_tf(nt, dt, f0, 1.2 / f0), 0.5 * syn.ricker_tf(nt, dt, f0, 1.4 / f0)], 1)
+    src = np.array([[200.0, 180.0, 150.0], [200.0, 180.0, 150.0]])
+    rec = np.array([[100.0, 100.0, 0.0], [100.0, 100.0, 0.0], [0.0, 200.0, 100.0], [390.0, 0.0, 290.0],
+                    [250.0, 150.0, 120.0]])
+    osh = dict(src_pos=src, srctf=tf, f0=f0, rec_pos=rec)
+    obs = oracle_mod.acoustic_cd3d_forward(vtrue, h, dt, nt, halo, 1e-4, True, [osh])[0]
+    osh["observed"] = obs
+    gref, _, sref = oracle_mod.acoustic_cd3d_gradient(v0, h, dt, nt, halo, 1e-4, True, [osh], check_freq=1)
+    params = sw.InputParametersAcoustic(nt, dt, n, h, sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4))
+    shots = [sw.ScalarShot(sw.ScalarSources(src, tf, f0), sw.ScalarReceivers(rec, nt))]
+    g = sw.swgradient(params, sw.VpAcousticCDMaterialProperties(v0), shots, [sw.L2Misfit(obs)],
+                      runparams=sw.RunParameters(), gradparams=sw.GradParameters(check_freq=1))
+    assert rel_l2(shots[0].recs.seismograms, sref[0]) <= 1e-10
+    assert rel_l2(g["vp"], gref["vp"]) <= 1e-10
