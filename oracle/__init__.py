@@ -182,23 +182,24 @@ def l2_misfit(syn, obs, invcov=None, windows=()):
         mask = np.zeros(res.shape[0])
         for a, b in windows:
             mask[a - 1:b] = 1.0
-        res = mask[:, None] * res
+        res = mask.reshape((-1,) + (1,) * (res.ndim - 1)) * res
     if invcov is None:
         return 0.5 * float(np.sum(res * res))
-    return 0.5 * float(np.sum(res * (np.asarray(invcov) @ res)))
+    # vector receivers: sum over components (L2Misfit.jl:44-62)
+    return 0.5 * float(np.sum(res * np.tensordot(np.asarray(invcov), res, axes=(1, 0))))
 
 
 def l2_dchi_du(syn, obs, invcov=None, windows=()):
-    """∂χ_∂u, src/inversion/misfits/L2Misfit.jl:64-77."""
+    """∂χ_∂u, src/inversion/misfits/L2Misfit.jl:64-95 (per component for 3D data)."""
     res = np.asarray(syn, dtype=np.float64) - np.asarray(obs, dtype=np.float64)
     if windows:
         mask = np.zeros(res.shape[0])
         for a, b in windows:
             mask[a - 1:b] = 1.0
-        res = mask[:, None] * res
+        res = mask.reshape((-1,) + (1,) * (res.ndim - 1)) * res
     if invcov is None:
         return res
-    return np.asarray(invcov) @ res
+    return np.tensordot(np.asarray(invcov), res, axes=(1, 0))
 
 
 def mute_around_points(grad, points, spacing, radius):
@@ -579,3 +580,261 @@ def acoustic_cd3d_gradient(vp, spacing, dt, nt, halo, rcoef, freetop, shots, che
         misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
         seis.append(tr)
     return {"vp": tot}, misfit, seis
+
+
+# ---------------------------------------------------------------------------
+# 2D isotropic elastic
+# ---------------------------------------------------------------------------
+def kaiser(x, r, beta):
+    """Kaiser window, src/utils/utils.jl:71 (besseli(0, .) = modified Bessel I0)."""
+    if -r <= x <= r:
+        return float(np.i0(beta * np.sqrt(1.0 - (x / r) ** 2)) / np.i0(beta))
+    return 0.0
+
+
+def modd(x, x0, r, beta, dx):
+    """Band-limited delta, utils.jl:84-87: kaiser(x-x0, r dx, beta) sinc((x-x0)/dx)."""
+    return kaiser(x - x0, r * dx, beta) * float(np.sinc((x - x0) / dx))
+
+
+def coeffsinc1d(x0, dx, nx, r, beta, xstart, mirror, xbl, xbr):
+    """coeffsinc1D, utils.jl:105-154, literally: taps i0-r-1..i0+r+1 around
+    the nearest grid point, |coef| <= 1e-15 dropped, taps beyond [xbl, xbr]
+    folded back (sign flipped when `mirror`), coincident taps summed.
+    Returns (1-based indices, coefficients) sorted by index."""
+    xs = xstart + dx * np.arange(nx)
+
+    def findnearest(x):
+        return int(np.argmin(np.abs(x - xs))) + 1
+
+    i0 = findnearest(x0)
+    idxs, coeffs = [], []
+    for idx in range(i0 - r - 1, i0 + r + 2):
+        xcurr = (idx - 1) * dx + xstart
+        coe = modd(xcurr, x0, r, beta, dx)
+        if not abs(coe) <= 1e-15:
+            idxs.append(idx)
+            coeffs.append(coe)
+    acc = {}
+    for idx, coe in zip(idxs, coeffs):
+        xcurr = (idx - 1) * dx + xstart
+        if xcurr < xbl:
+            idx = findnearest(xbl + (xbl - xcurr))
+            coe = -coe if mirror else coe
+        elif xcurr > xbr:
+            idx = findnearest(xbr - (xcurr - xbr))
+            coe = -coe if mirror else coe
+        acc[idx] = acc.get(idx, 0.0) + coe
+    keys = sorted(acc)
+    return np.array(keys, dtype=np.int64), np.array([acc[k] for k in keys])
+
+
+def spread_positions(positions, spacing, gridsize, shift, mirror, r=4, beta=6.31):
+    """spread_positions with freesurfposition=:ongridbound, utils.jl:168-214.
+    Returns one (ij (ntaps, 2) 1-based, coef (ntaps,)) pair per position,
+    tensor product with the first dimension fastest."""
+    positions = np.atleast_2d(np.asarray(positions, dtype=np.float64))
+    N = positions.shape[1]
+    extent = [(gridsize[d] - 1) * spacing[d] for d in range(N)]
+    out = []
+    for p in positions:
+        per = [coeffsinc1d(p[d], spacing[d], gridsize[d], r, beta, shift[d], mirror, 0.0, extent[d])
+               for d in range(N)]
+        ii, jj = np.meshgrid(per[0][0], per[1][0], indexing="ij")
+        ci, cj = np.meshgrid(per[0][1], per[1][1], indexing="ij")
+        ij = np.stack([ii.ravel(order="F"), jj.ravel(order="F")], 1)
+        out.append((ij, (ci * cj).ravel(order="F")))
+    return out
+
+
+def el_possrcrec(kind, src_pos, rec_pos, spacing, gridsize, sincinterp=True):
+    """possrcrec_scaletf point part, src/models/elastic/ela_models.jl:6-90.
+    kind "mt": sources on sxx/szz (shift 0, mirror) and sxz (shift h/2, mirror);
+    kind "ef": sources on ux (shift dx/2) and uz (shift dz/2).  Receivers on
+    ux and uz.  Returns (src_fam0, src_fam1, rec_ux, rec_uz)."""
+    dx, dz = spacing
+    if not sincinterp:
+        si = find_nearest_grid_points(src_pos, spacing)
+        ri = find_nearest_grid_points(rec_pos, spacing)
+        one = lambda idx: [(idx[k:k + 1].copy(), np.ones(1)) for k in range(idx.shape[0])]
+        return one(si), one(si), one(ri), one(ri)
+    if kind == "mt":
+        s0 = spread_positions(src_pos, spacing, gridsize, (0.0, 0.0), True)
+        s1 = spread_positions(src_pos, spacing, gridsize, (dx / 2, dz / 2), True)
+    else:
+        s0 = spread_positions(src_pos, spacing, gridsize, (dx / 2, 0.0), False)
+        s1 = spread_positions(src_pos, spacing, gridsize, (0.0, dz / 2), False)
+    r0 = spread_positions(rec_pos, spacing, gridsize, (dx / 2, 0.0), False)
+    r1 = spread_positions(rec_pos, spacing, gridsize, (0.0, dz / 2), False)
+    return s0, s1, r0, r1
+
+
+def interp_dims(m, dims, method="arithmetic"):
+    """interp with a list of dims, interpolations.jl:4-5 and :34-42: linear
+    B-spline at the half points (mean of the 2^len(dims) corners)."""
+    def avg(a):
+        for d in dims:
+            ax = d - 1
+            n = a.shape[ax]
+            a = (np.take(a, range(0, n - 1), axis=ax) + np.take(a, range(1, n), axis=ax)) / 2
+        return a
+    if method == "arithmetic":
+        return avg(np.asarray(m, dtype=np.float64))
+    return 1.0 / avg(1.0 / np.asarray(m, dtype=np.float64))
+
+
+def back_interp_dims(m, g, dims, method="arithmetic"):
+    """back_interp with a list of dims, interpolations.jl:15-28 and :45-47."""
+    m = np.asarray(m, dtype=np.float64)
+    res = np.zeros(m.shape)
+    k = len(dims)
+    if method == "arithmetic":
+        w = np.full(g.shape, 1.0 / 2 ** k)
+    else:
+        itp = interp_dims(m, dims, method)
+    for perm in range(2 ** k):
+        sl = [slice(None)] * m.ndim
+        for b, d in enumerate(dims):
+            off = (perm >> (k - 1 - b)) & 1
+            sl[d - 1] = slice(off, m.shape[d - 1] - 1 + off)
+        if method != "arithmetic":
+            w = itp ** 2 / m[tuple(sl)] ** 2 / 2 ** k
+        res[tuple(sl)] += g * w
+    return res
+
+
+def el_model_fields(lam, mu, rho, interp_rho="arithmetic", interp_mu="arithmetic", dtype=np.float64):
+    """precomp_elaprop!, ela_models.jl:156-173: lam, mu, rho at (i+1/2, j),
+    rho at (i, j+1/2), mu at (i+1/2, j+1/2)."""
+    T = dtype
+    return (np.asarray(lam, dtype=T), np.asarray(mu, dtype=T),
+            interp_dims(rho, [1], interp_rho).astype(T), interp_dims(rho, [2], interp_rho).astype(T),
+            interp_dims(mu, [1, 2], interp_mu).astype(T))
+
+
+class _Taps(ctypes.Structure):
+    _fields_ = [("npts", ctypes.c_int), ("ptr", ctypes.c_void_p), ("ij", ctypes.c_void_p),
+                ("ntaps", ctypes.c_int64), ("coef", ctypes.c_void_p)]
+
+
+def _taps(fam, dtype):
+    ptr = np.zeros(len(fam) + 1, dtype=np.int32)
+    for k, (ij, c) in enumerate(fam):
+        ptr[k + 1] = ptr[k] + len(c)
+    ij = np.concatenate([f[0] for f in fam], 0).astype(np.int64)
+    ij = np.asfortranarray(ij)
+    coef = np.ascontiguousarray(np.concatenate([f[1] for f in fam]), dtype=dtype)
+    st = _Taps(len(fam), _ptr(ptr), _ptr(ij), ij.shape[0], _ptr(coef))
+    return st, (ptr, ij, coef)
+
+
+def _el_args(fields, spacing, dt, halo, freetop, cpml, dtype):
+    lam, mu, rih, rjh, mhh = [_f_order(f, dtype) for f in fields]
+    nx, nz = lam.shape
+    g = _geom((nx, nz), spacing, dt, halo, freetop)
+    cx, kx = _cpml_struct(cpml[0], dtype)
+    cz, kz = _cpml_struct(cpml[1], dtype)
+    keep = (lam, mu, rih, rjh, mhh, kx, kz, cx, cz)
+    return g, keep
+
+
+def el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, moments, srctf, dtype=np.float64):
+    """Forward loop of swforward_1shot! (ela_forward.jl:4-159) on prepared
+    inputs.  fields = el_model_fields(...); fams = el_possrcrec(...);
+    srctf already divided by dx dz (MT (nt, nsrc), EF (nt, 2, nsrc)).
+    Returns traces (nt, 2, nrec)."""
+    g, keep = _el_args(fields, spacing, dt, halo, freetop, cpml, dtype)
+    lam, mu, rih, rjh, mhh, _, _, cx, cz = keep
+    t = [_taps(f, dtype) for f in fams]
+    M = np.ascontiguousarray(moments if moments is not None else np.zeros((1, 3)), dtype=dtype)
+    stf = _f_order(srctf, dtype)
+    nrec = len(fams[2])
+    traces = np.zeros((nt, 2, nrec), dtype=dtype, order="F")
+    rc = _fn("oracle_el2d_forward", dtype)(
+        ctypes.byref(g), _ptr(lam), _ptr(mu), _ptr(rih), _ptr(rjh), _ptr(mhh), ctypes.byref(cx), ctypes.byref(cz),
+        ctypes.c_int(nt), ctypes.c_int(0 if kind == "mt" else 1), ctypes.byref(t[0][0]), ctypes.byref(t[1][0]),
+        _ptr(M), _ptr(stf), ctypes.byref(t[2][0]), ctypes.byref(t[3][0]), _ptr(traces))
+    assert rc == 0
+    return np.array(traces)
+
+
+def el2d_gradient(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, moments, srctf, adjsrc, check_freq,
+                  dtype=np.float64):
+    """swgradient_1shot! (ela_gradient.jl:4-362) up to the raw grad fields.
+    Returns ((g_rho_ih, g_rho_jh, g_lam, g_mu, g_mu_ihjh), traces)."""
+    g, keep = _el_args(fields, spacing, dt, halo, freetop, cpml, dtype)
+    lam, mu, rih, rjh, mhh, _, _, cx, cz = keep
+    nx, nz = lam.shape
+    t = [_taps(f, dtype) for f in fams]
+    M = np.ascontiguousarray(moments if moments is not None else np.zeros((1, 3)), dtype=dtype)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    nrec = len(fams[2])
+    traces = np.zeros((nt, 2, nrec), dtype=dtype, order="F")
+    gr = [np.zeros(s, dtype=dtype, order="F") for s in ((nx - 1, nz), (nx, nz - 1), (nx, nz), (nx, nz),
+                                                         (nx - 1, nz - 1))]
+    rc = _fn("oracle_el2d_gradient", dtype)(
+        ctypes.byref(g), _ptr(lam), _ptr(mu), _ptr(rih), _ptr(rjh), _ptr(mhh), ctypes.byref(cx), ctypes.byref(cz),
+        ctypes.c_int(nt), ctypes.c_int(0 if kind == "mt" else 1), ctypes.byref(t[0][0]), ctypes.byref(t[1][0]),
+        _ptr(M), _ptr(stf), ctypes.byref(t[2][0]), ctypes.byref(t[3][0]), _ptr(adj), ctypes.c_int(check_freq),
+        *[_ptr(x) for x in gr], _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return tuple(np.array(x) for x in gr), np.array(traces)
+
+
+def _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh, sincinterp, dtype):
+    T = dtype
+    fields = el_model_fields(lam, mu, rho, dtype=T)
+    vp = np.sqrt((np.asarray(lam, np.float64) + 2 * np.asarray(mu, np.float64)) / np.asarray(rho, np.float64))
+    cpml = init_bdc(vp.astype(T), dt, spacing, halo, rcoef, freetop, sh["f0"], T)      # ela_init_bc.jl:7-40
+    kind = "mt" if "moments" in sh else "ef"
+    n = np.asarray(lam).shape
+    fams = el_possrcrec(kind, sh["src_pos"], sh["rec_pos"], spacing, n, sincinterp)
+    stf = np.asarray(sh["srctf"], dtype=T) / T(np.prod(np.asarray(spacing, dtype=T)))  # ela_models.jl:46,89
+    mom = None if kind == "ef" else np.asarray(sh["moments"], dtype=T)
+    return fields, cpml, kind, fams, stf, mom
+
+
+def elastic_forward(lam, mu, rho, spacing, dt, nt, halo, rcoef, freetop, shots, sincinterp=True, dtype=np.float64):
+    """run_swforward! for ElasticIsoCPMLWaveSimulation 2D.  Shot dicts:
+    src_pos (nsrc, 2), srctf ((nt, nsrc) with `moments` (nsrc, 3) = Mxx, Mzz,
+    Mxz for moment-tensor sources, or (nt, 2, nsrc) for external forces),
+    f0, rec_pos (nrec, 2).  Returns seismograms (nt, 2, nrec) per shot."""
+    out = []
+    for sh in shots:
+        fields, cpml, kind, fams, stf, mom = _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh,
+                                                         sincinterp, dtype)
+        out.append(el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, mom, stf, dtype))
+    return out
+
+
+def elastic_gradient(lam, mu, rho, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1, sincinterp=True,
+                     mute_radius_src=0, mute_radius_rec=0, dtype=np.float64):
+    """run_swgradient! for elastic 2D with L2 misfits (ela_gradient.jl:4-186,
+    :188-362): raw correlations, chain rule through the staggered
+    interpolations (:161-165), muting (:167-178).  Returns (gradient dict
+    {"rho", "lambda", "mu"}, total misfit, seismograms per shot)."""
+    T = dtype
+    n = np.asarray(lam).shape
+    tot = {k: np.zeros(n, dtype=T) for k in ("rho", "lambda", "mu")}
+    misfit = 0.0
+    seis = []
+    for sh in shots:
+        fields, cpml, kind, fams, stf, mom = _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh,
+                                                         sincinterp, T)
+        syn = el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, mom, stf, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
+        (grih, grjh, glam, gmu, gmhh), tr = el2d_gradient(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams,
+                                                          mom, stf, adj.astype(T), check_freq, T)
+        g_rho = (back_interp_dims(rho, grih, [1]) + back_interp_dims(rho, grjh, [2])).astype(T)
+        g_mu = (gmu + back_interp_dims(mu, gmhh, [1, 2])).astype(T)
+        g_lam = glam.astype(T)
+        g_rho, g_lam, g_mu = [mute_around_points(mute_around_points(g, sh["src_pos"], spacing, mute_radius_src),
+                                                 sh["rec_pos"], spacing, mute_radius_rec) for g in (g_rho, g_lam, g_mu)]
+        tot["rho"] += g_rho
+        tot["lambda"] += g_lam
+        tot["mu"] += g_mu
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return tot, misfit, seis

@@ -22,6 +22,7 @@ typedef struct {
 typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
 #include "oracle_cd3d.h"
+#include "oracle_el2d.h"
 #undef real
 #undef SFX
 #undef A3
@@ -31,6 +32,7 @@ typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
 #include "oracle_cd3d.h"
+#include "oracle_el2d.h"
 #undef real
 #undef SFX
 
