@@ -101,6 +101,45 @@ typedef struct {
   const void *cpml[3][4];   /* [axis][a, a_h, b, b_h], T                     */
 } swx_shot_desc;
 
+/* Source kinds of an elastic shot: MomentTensorSources and
+ * ExternalForceSources (src/shots/sources.jl:54-91). */
+typedef enum {
+  SWX_SRC_MOMENT_TENSOR = 1,
+  SWX_SRC_EXTERNAL_FORCE = 2
+} swx_source_kind;
+
+/* Points spread over grid taps, the srccoeij_* / srccoeval_* (and reccoeij_*
+ * / reccoeval_*) lists of possrcrec_scaletf (ela_models.jl:6-90) built by
+ * spread_positions (src/utils/utils.jl:168-214): point p owns taps
+ * ptr[p] .. ptr[p+1]-1; tap k is the 1-based cell (ij[2k], ij[2k+1]) of the
+ * family's staggered grid with weight coef[k] (T). */
+typedef struct {
+  const int32_t *ptr;       /* npts + 1                                      */
+  const int64_t *ij;        /* 2 x ntaps, (i, j) pairs                       */
+  const void *coef;         /* ntaps, T                                      */
+} swx_taps;
+
+/* One elastic shot (MomentTensorShot / ExternalForceShot, shot.jl:23-44)
+ * with VectorReceivers (receivers.jl:64-100).
+ *   moment tensor:  src[0] taps on the sxx/szz grid (nx x nz), src[1] on
+ *                   the sxz grid (nx-1 x nz-1); moments = nsrc x (Mxx, Mzz,
+ *                   Mxz) (T); srctf nt x nsrc.
+ *   external force: src[0] taps on the ux grid (nx-1 x nz), src[1] on the uz
+ *                   grid (nx x nz-1); srctf nt x 2 x nsrc.
+ *   srctf is already divided by dx*dz (possrcrec_scaletf).  rec[0] / rec[1]
+ *   are the receivers' ux / uz taps.  Seismograms are nt x 2 x nrec.       */
+typedef struct {
+  int32_t nt;
+  int32_t source_kind;      /* swx_source_kind                               */
+  int32_t nsrc;
+  swx_taps src[2];
+  const void *moments;      /* moment tensor sources only                    */
+  const void *srctf;
+  int32_t nrec;
+  swx_taps rec[2];
+  const void *cpml[3][4];   /* as in swx_shot_desc                           */
+} swx_elastic_shot_desc;
+
 typedef struct {
   int64_t bytes_state;       /* fields + C-PML memory variables             */
   int64_t bytes_checkpoints; /* checkpoint storage                          */
@@ -129,17 +168,28 @@ int swx_plan_create(const swx_plan_desc *desc, swx_plan **plan);
 
 int swx_plan_destroy(swx_plan *plan);
 
-/* Replaces precompute_fact! uploads (acou_models.jl:60, :292-300): the
- * material factor fields consumed by the kernels, host arrays of T.
+/* Replaces precompute_fact! uploads (acou_models.jl:60, :292-300) and
+ * precomp_elaprop! (ela_models.jl:156-173): the material fields consumed by
+ * the kernels, host arrays of T.
  *   SWX_ACOUSTIC_CD: fields[0] = fact = dt^2 vp^2            (n)
  *   SWX_ACOUSTIC_VD: fields[0] = fact_m0 = vp^2 rho dt       (n)
  *                    fields[1] = fact_m1_stag[1] (x)         (nx-1, nz)
- *                    fields[2] = fact_m1_stag[2] (z)         (nx, nz-1) */
+ *                    fields[2] = fact_m1_stag[2] (z)         (nx, nz-1)
+ *   SWX_ELASTIC_ISO: fields[0] = lambda, fields[1] = mu      (n)
+ *                    fields[2] = rho_ihalf                   (nx-1, nz)
+ *                    fields[3] = rho_jhalf                   (nx, nz-1)
+ *                    fields[4] = mu_ihalf_jhalf              (nx-1, nz-1) */
 int swx_set_model(swx_plan *plan, int nfields, const void *const *fields);
 
 /* Replaces the per-shot uploads of swforward_1shot!/swgradient_1shot!
- * (acou_forward.jl:38-41) for a batch of shots.  Validates positions. */
+ * (acou_forward.jl:38-41) for a batch of shots.  Validates positions.
+ * Acoustic plans only. */
 int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots);
+
+/* Elastic counterpart of swx_set_shots: the per-shot uploads of the elastic
+ * swforward_1shot!/swgradient_1shot! (ela_forward.jl:21-43,
+ * ela_gradient.jl:24-46).  All shots of a batch share one source kind. */
+int swx_set_shots_elastic(swx_plan *plan, int nshots, const swx_elastic_shot_desc *shots);
 
 /* Forward time loop of every shot (acou_forward.jl:49-58 per shot).
  * traces: host T array, the shots' seismograms (nt_s x nrec_s, column-major)
@@ -148,14 +198,19 @@ int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots);
 int swx_forward(swx_plan *plan, void *traces, int save_state);
 
 /* Adjoint time loop with replay from the device checkpoints and gradient
- * correlation (acou_gradient.jl:51-84 for CD, :145-175 for VD).
- * adjsrc: host T array, -dchi/du per shot (nt_s x nrec_s) concatenated; the
- *   prescale by the material factor at the receivers is done here.
+ * correlation (acou_gradient.jl:51-84 for CD, :145-175 for VD,
+ * ela_gradient.jl:95-150 for elastic).
+ * adjsrc: host T array, -dchi/du per shot (nt_s x nrec_s, elastic
+ *   nt_s x 2 x nrec_s) concatenated; the acoustic prescale by the material
+ *   factor at the receivers is done here.
  * grad: host T output.  Raw gradient fields as the reference's grid fields
  *   hold them after the adjoint loop, already summed over shots unless
  *   per_shot != 0 (then nshots consecutive copies):
  *   SWX_ACOUSTIC_CD: grad_vp (n)
- *   SWX_ACOUSTIC_VD: grad_m0 (n), grad_m1_stag x (nx-1,nz), z (nx,nz-1) */
+ *   SWX_ACOUSTIC_VD: grad_m0 (n), grad_m1_stag x (nx-1,nz), z (nx,nz-1)
+ *   SWX_ELASTIC_ISO: grad_rho_ihalf (nx-1,nz), grad_rho_jhalf (nx,nz-1),
+ *                    grad_lambda (n), grad_mu (n),
+ *                    grad_mu_ihalf_jhalf (nx-1,nz-1) */
 int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot);
 
 /* Sizes, segment length and device timings of the last calls. */

@@ -728,6 +728,17 @@ def _taps(fam, dtype):
     return st, (ptr, ij, coef)
 
 
+def _check_taps(kind, fams, nx, nz):
+    """Every tap lies on its family's staggered grid (the C loops do not check)."""
+    grids = ([(nx, nz), (nx - 1, nz - 1)] if kind == "mt" else [(nx - 1, nz), (nx, nz - 1)]) + \
+        [(nx - 1, nz), (nx, nz - 1)]
+    for fam, (gi, gj) in zip(fams, grids):
+        for ij, _ in fam:
+            if ij.size and not (ij[:, 0].min() >= 1 and ij[:, 0].max() <= gi and ij[:, 1].min() >= 1
+                                and ij[:, 1].max() <= gj):
+                raise ValueError("point taps outside the staggered grid")
+
+
 def _el_args(fields, spacing, dt, halo, freetop, cpml, dtype):
     lam, mu, rih, rjh, mhh = [_f_order(f, dtype) for f in fields]
     nx, nz = lam.shape
@@ -745,6 +756,7 @@ def el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, momen
     Returns traces (nt, 2, nrec)."""
     g, keep = _el_args(fields, spacing, dt, halo, freetop, cpml, dtype)
     lam, mu, rih, rjh, mhh, _, _, cx, cz = keep
+    _check_taps(kind, fams, *lam.shape)
     t = [_taps(f, dtype) for f in fams]
     M = np.ascontiguousarray(moments if moments is not None else np.zeros((1, 3)), dtype=dtype)
     stf = _f_order(srctf, dtype)
@@ -765,6 +777,7 @@ def el2d_gradient(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, mome
     g, keep = _el_args(fields, spacing, dt, halo, freetop, cpml, dtype)
     lam, mu, rih, rjh, mhh, _, _, cx, cz = keep
     nx, nz = lam.shape
+    _check_taps(kind, fams, nx, nz)
     t = [_taps(f, dtype) for f in fams]
     M = np.ascontiguousarray(moments if moments is not None else np.zeros((1, 3)), dtype=dtype)
     stf = _f_order(srctf, dtype)
@@ -791,7 +804,8 @@ def _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh, sincinterp,
     kind = "mt" if "moments" in sh else "ef"
     n = np.asarray(lam).shape
     fams = el_possrcrec(kind, sh["src_pos"], sh["rec_pos"], spacing, n, sincinterp)
-    stf = np.asarray(sh["srctf"], dtype=T) / T(np.prod(np.asarray(spacing, dtype=T)))  # ela_models.jl:46,89
+    nt_s = sh.get("nt", None)
+    stf = np.asarray(sh["srctf"], dtype=T)[:nt_s] / T(np.prod(np.asarray(spacing, dtype=T)))  # ela_models.jl:46,89
     mom = None if kind == "ef" else np.asarray(sh["moments"], dtype=T)
     return fields, cpml, kind, fams, stf, mom
 
@@ -800,12 +814,14 @@ def elastic_forward(lam, mu, rho, spacing, dt, nt, halo, rcoef, freetop, shots, 
     """run_swforward! for ElasticIsoCPMLWaveSimulation 2D.  Shot dicts:
     src_pos (nsrc, 2), srctf ((nt, nsrc) with `moments` (nsrc, 3) = Mxx, Mzz,
     Mxz for moment-tensor sources, or (nt, 2, nsrc) for external forces),
-    f0, rec_pos (nrec, 2).  Returns seismograms (nt, 2, nrec) per shot."""
+    f0, rec_pos (nrec, 2), optionally their own nt.  Returns seismograms
+    (nt, 2, nrec) per shot."""
     out = []
     for sh in shots:
         fields, cpml, kind, fams, stf, mom = _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh,
                                                          sincinterp, dtype)
-        out.append(el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, mom, stf, dtype))
+        out.append(el2d_forward(fields, spacing, dt, halo, freetop, cpml, sh.get("nt", nt), kind, fams, mom, stf,
+                                dtype))
     return out
 
 
@@ -823,10 +839,11 @@ def elastic_gradient(lam, mu, rho, spacing, dt, nt, halo, rcoef, freetop, shots,
     for sh in shots:
         fields, cpml, kind, fams, stf, mom = _el_prepare(lam, mu, rho, spacing, dt, halo, rcoef, freetop, sh,
                                                          sincinterp, T)
-        syn = el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams, mom, stf, T)
+        nt_s = sh.get("nt", nt)
+        syn = el2d_forward(fields, spacing, dt, halo, freetop, cpml, nt_s, kind, fams, mom, stf, T)
         adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
-        (grih, grjh, glam, gmu, gmhh), tr = el2d_gradient(fields, spacing, dt, halo, freetop, cpml, nt, kind, fams,
-                                                          mom, stf, adj.astype(T), check_freq, T)
+        (grih, grjh, glam, gmu, gmhh), tr = el2d_gradient(fields, spacing, dt, halo, freetop, cpml, nt_s, kind,
+                                                          fams, mom, stf, adj.astype(T), check_freq, T)
         g_rho = (back_interp_dims(rho, grih, [1]) + back_interp_dims(rho, grjh, [2])).astype(T)
         g_mu = (gmu + back_interp_dims(mu, gmhh, [1, 2])).astype(T)
         g_lam = glam.astype(T)

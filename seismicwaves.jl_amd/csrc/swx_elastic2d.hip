@@ -1,0 +1,901 @@
+// 2D isotropic elastic engine (displacement-stress, 4th-order staggered
+// space, 2nd-order time, C-PML, optional free surface on the top row) for a
+// batch of shots: two launches per time step (stresses from ucur, then
+// displacements), a receiver gather when traces are recorded, and the
+// gradient correlation fused into the adjoint stress launch.
+//
+// Reference semantics (paths relative to the reference root):
+//   step       src/models/elastic/backends/shared/elastic2D_iso_xPU.jl:1-448
+//   derivs     src/models/elastic/backends/shared/freesurface_derivatives_4th_mirror.jl:1-233
+//   C-PML      src/utils/fdgenerated.jl:178-199 (index maps of d~x4th/d~y4th)
+//   driver     src/models/elastic/ela_gradient.jl:4-362
+//   correlate  src/models/elastic/backends/shared/correlate_gradient_xPU.jl:1-82
+//
+// Layout: every field is an nx*nz slab per shot with x fastest (the Julia
+// column-major layout); ux has nx-1 valid columns, uz nz-1 valid rows and
+// sxz both, the spare column/row is never read as data.  The displacement
+// update reads uold and ucur only at its own cell, so unew overwrites uold in
+// place: the time level u^k lives in slot k mod 2.  The C-PML memory
+// variables keep the reference's own shapes (8 arrays per shot).
+//
+// Adjoint.  The correlation of adjoint step c needs spatial derivatives of
+// the new adjoint field a^c, which the next adjoint stress launch computes
+// anyway (it reads a^c as its ucur), so that launch correlates step c with
+// the replayed forward history u^{c-2}, u^{c-1}, u^c; the first step of each
+// replay segment is correlated by a correlate-only launch.
+#include <algorithm>
+#include <map>
+
+#include "swx_device.hpp"
+#include "swx_internal.hpp"
+
+namespace swx {
+namespace {
+
+constexpr int TX = 64;  // one wavefront along x
+constexpr int TZ = 4;
+
+// Point tables of one tap family for a batch (injection view: taps grouped
+// by cell, in point order inside a cell; gather view: taps per point).
+template <typename T>
+struct TapView {
+  const int64_t *cell;  // unique cells, ascending per shot
+  const int *eptr;      // entries of unique cell u: [eptr[u], eptr[u+1])
+  const int *epoint;    // point index (within its shot) of each entry
+  const T *ecoef;
+  const int *rows;      // row index of the unique cells (find_point)
+  const int *poff;      // first point of each shot (gather view)
+  const int *pptr;      // taps of global point p: [pptr[p], pptr[p+1])
+  const int64_t *pcell;
+  const T *pcoef;
+};
+
+template <typename T>
+struct ElArgs {
+  int nx, nz, h, freetop;
+  int64_t N, nrows;
+  T idx, idz, idt2;
+  const T *lam, *mu, *muhh, *rdx, *rdz;  // rdx = dt^2/rho_ihalf, rdz = dt^2/rho_jhalf
+  const T *cpml; int cstride;
+  const int *nt_shot;
+  int it, act_min;
+  const T *ucur;          // [ns][2][N]: ux | uz at level it-1
+  T *unew;                // level it (= slot of it-2, updated in place)
+  T *sig;                 // [ns][3][N]: sxx | szz | sxz
+  T *psi; int64_t psi_stride;
+  int64_t psioff[8];
+  // points: kind 0 none, 1 moment tensor (stress), 2 force or residual (displacement)
+  int kind;
+  TapView<T> f0, f1;
+  const T *tf; const int64_t *off_tf; int ncomp;   // per shot (nt, ncomp, npts)
+  const T *mom; const int *off_mom;                // moment tensors (3 per source)
+  // forward history for the gradient: slot k holds u^{base+k-1}
+  T *hist; int64_t hslot; int hmode; int S; int cbase;
+  // correlation (stress launch on the adjoint field)
+  T *grad; int64_t gstride; int update; int corr_step; int corr_lim;
+};
+
+// 4th-order staggered difference, freesurface_derivatives_4th_mirror.jl:2-7
+template <typename T>
+__device__ __forceinline__ T d4(T f1, T f2, T f3, T f4, T inv) {
+  return (T(1.0 / 24.0) * f1 - T(27.0 / 24.0) * f2 + T(27.0 / 24.0) * f3 - T(1.0 / 24.0) * f4) * inv;
+}
+
+// Field accessor with the reference's 1-based indices.
+template <typename T>
+struct F2 {
+  const T *p; int nx;
+  __device__ __forceinline__ T operator()(int i, int j) const { return p[(i - 1) + (int64_t)(j - 1) * nx]; }
+};
+
+// Derivative wrappers with the boundary rules of
+// freesurface_derivatives_4th_mirror.jl:72-233.
+template <typename T>
+__device__ __forceinline__ T d_sxx_dx(F2<T> s, int i, int j, int nx, T id) {
+  if (i == 1) return d4<T>(0, s(i, j), s(i + 1, j), s(i + 2, j), id);
+  if (i == nx - 1) return d4<T>(s(i - 1, j), s(i, j), s(i + 1, j), 0, id);
+  return d4<T>(s(i - 1, j), s(i, j), s(i + 1, j), s(i + 2, j), id);
+}
+template <typename T>
+__device__ __forceinline__ T d_szz_dz(F2<T> s, int i, int j, int nz, bool ft, T id) {
+  if (j == 1) return d4<T>(ft ? -s(i, j + 1) : T(0), s(i, j), s(i, j + 1), s(i, j + 2), id);
+  if (j == nz - 1) return d4<T>(s(i, j - 1), s(i, j), s(i, j + 1), 0, id);
+  return d4<T>(s(i, j - 1), s(i, j), s(i, j + 1), s(i, j + 2), id);
+}
+template <typename T>
+__device__ __forceinline__ T d_sxz_dx(F2<T> s, int i, int j, int nx, T id) {
+  if (i == 1) return d4<T>(0, 0, s(i, j), s(i + 1, j), id);
+  if (i == 2) return d4<T>(0, s(i - 1, j), s(i, j), s(i + 1, j), id);
+  if (i == nx - 1) return d4<T>(s(i - 2, j), s(i - 1, j), s(i, j), 0, id);
+  if (i == nx) return d4<T>(s(i - 2, j), s(i - 1, j), 0, 0, id);
+  return d4<T>(s(i - 2, j), s(i - 1, j), s(i, j), s(i + 1, j), id);
+}
+template <typename T>
+__device__ __forceinline__ T d_sxz_dz(F2<T> s, int i, int j, int nz, bool ft, T id) {
+  if (j == 1) {
+    if (ft) return d4<T>(-s(i, j + 1), -s(i, j), s(i, j), s(i, j + 1), id);
+    return d4<T>(0, 0, s(i, j), s(i, j + 1), id);
+  }
+  if (j == 2) return d4<T>(ft ? -s(i, j - 1) : T(0), s(i, j - 1), s(i, j), s(i, j + 1), id);
+  if (j == nz - 1) return d4<T>(s(i, j - 2), s(i, j - 1), s(i, j), 0, id);
+  if (j == nz) return d4<T>(s(i, j - 2), s(i, j - 1), 0, 0, id);
+  return d4<T>(s(i, j - 2), s(i, j - 1), s(i, j), s(i, j + 1), id);
+}
+template <typename T>
+__device__ __forceinline__ T d_ux_dx(F2<T> u, int i, int j, int nx, T id) {
+  if (i == 1) return d4<T>(0, 0, u(i, j), u(i + 1, j), id);
+  if (i == 2) return d4<T>(0, u(i - 1, j), u(i, j), u(i + 1, j), id);
+  if (i == nx - 1) return d4<T>(u(i - 2, j), u(i - 1, j), u(i, j), 0, id);
+  if (i == nx) return d4<T>(u(i - 2, j), u(i - 1, j), 0, 0, id);
+  return d4<T>(u(i - 2, j), u(i - 1, j), u(i, j), u(i + 1, j), id);
+}
+// duz/dz; at the free surface from sigma_zz = 0: -lam/(lam+2mu) dux/dx
+template <typename T>
+__device__ __forceinline__ T d_uz_dz(F2<T> ux, F2<T> uz, T l, T m, int i, int j, int nx, int nz, bool ft, T idx,
+                                     T idz) {
+  if (j == 1) {
+    if (ft) return (-l / (l + T(2) * m)) * d_ux_dx(ux, i, j, nx, idx);
+    return d4<T>(0, 0, uz(i, j), uz(i, j + 1), idz);
+  }
+  if (j == 2) return d4<T>(ft ? uz(i, j - 1) : T(0), uz(i, j - 1), uz(i, j), uz(i, j + 1), idz);
+  if (j == nz - 1) return d4<T>(uz(i, j - 2), uz(i, j - 1), uz(i, j), 0, idz);
+  if (j == nz) return d4<T>(uz(i, j - 2), uz(i, j - 1), 0, 0, idz);
+  return d4<T>(uz(i, j - 2), uz(i, j - 1), uz(i, j), uz(i, j + 1), idz);
+}
+template <typename T>
+__device__ __forceinline__ T d_ux_dz(F2<T> u, int i, int j, int nz, bool ft, T id) {
+  if (j == 1) return d4<T>(ft ? u(i, j + 1) : T(0), u(i, j), u(i, j + 1), u(i, j + 2), id);
+  if (j == nz - 1) return d4<T>(u(i, j - 1), u(i, j), u(i, j + 1), 0, id);
+  return d4<T>(u(i, j - 1), u(i, j), u(i, j + 1), u(i, j + 2), id);
+}
+template <typename T>
+__device__ __forceinline__ T d_uz_dx(F2<T> u, int i, int j, int nx, T id) {
+  if (i == 1) return d4<T>(0, u(i, j), u(i + 1, j), u(i + 2, j), id);
+  if (i == nx - 1) return d4<T>(u(i - 1, j), u(i, j), u(i + 1, j), 0, id);
+  return d4<T>(u(i - 1, j), u(i, j), u(i + 1, j), u(i + 2, j), id);
+}
+
+// C-PML strip index (1-based, 0 = none) of d~4th (fdgenerated.jl:178-195) for
+// base index I along an axis on which the differentiated field has n points.
+__device__ __forceinline__ int strip_half(int I, int n, int h) {  // half = true
+  const int idim = I + 1;
+  if (idim <= h + 1) return idim;
+  if (idim >= n - h) return I - (n - h) + 2 + h;
+  return 0;
+}
+__device__ __forceinline__ int strip_full(int I, int n, int h) {  // half = false
+  if (I <= h) return I;
+  if (I >= n - h) return I - (n - h) + 1 + h;
+  return 0;
+}
+
+// Stress update of one step (update_sxx_szz! :39-60, update_sxz! :62-79),
+// moment-tensor injection (:81-94) and, on the adjoint field, the fused
+// correlation of adjoint step corr_step (correlate_gradient_xPU.jl:1-82).
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  const bool upd = a.update && a.it <= nts && nts > a.act_min;
+  const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
+  if (!upd && !corr) return;
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (i > nx || j > nz) return;
+  const bool ft = a.freetop != 0;
+  const int64_t N = a.N;
+  const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+  const F2<T> ux{a.ucur + (int64_t)s * 2 * N, nx}, uz{a.ucur + (int64_t)s * 2 * N + N, nx};
+  T *sg = a.sig + (int64_t)s * 3 * N;
+  T *ps = a.psi + (int64_t)s * a.psi_stride;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+
+  const bool nrm = i >= 2 && i <= nx - 1 && j >= (ft ? 1 : 2) && j <= nz - 1;
+  const bool shr = i <= nx - 1 && j <= nz - 1;
+  T dux = 0, duz = 0, duzx = 0, duxz = 0;
+  T vxx = 0, vzz = 0, vxz = 0;
+  bool wn = false, ws = false;
+  if (nrm) {
+    const T l = a.lam[c], m = a.mu[c];
+    dux = d_ux_dx(ux, i, j, nx, a.idx);
+    duz = d_uz_dz(ux, uz, l, m, i, j, nx, nz, ft, a.idx, a.idz);
+    if (upd) {
+      T dxc = dux, dzc = duz;
+      int k = strip_half(i - 1, nx - 1, h);
+      if (k) {
+        T &p = ps[a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2)];
+        p = cx.b[k - 1] * p + cx.a[k - 1] * dux;
+        dxc = dux + p;
+      }
+      k = strip_half(j - 1, nz - 1, h);
+      if (k) {
+        T &p = ps[a.psioff[7] + (i - 1) + (int64_t)(k - 1) * nx];
+        p = cz.b[k - 1] * p + cz.a[k - 1] * duz;
+        dzc = duz + p;
+      }
+      vxx = (l + T(2) * m) * dxc + l * dzc;
+      vzz = j == 1 ? T(0) : l * dxc + (l + T(2) * m) * dzc;
+      wn = true;
+    }
+  }
+  if (shr) {
+    duzx = d_uz_dx(uz, i, j, nx, a.idx);
+    duxz = d_ux_dz(ux, i, j, nz, ft, a.idz);
+    if (upd) {
+      T c1 = duzx, c2 = duxz;
+      int k = strip_full(i, nx, h);
+      if (k) {
+        T &p = ps[a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h)];
+        p = cx.bh[k - 1] * p + cx.ah[k - 1] * duzx;
+        c1 = duzx + p;
+      }
+      k = strip_full(j, nz, h);
+      if (k) {
+        T &p = ps[a.psioff[6] + (i - 1) + (int64_t)(k - 1) * (nx - 1)];
+        p = cz.bh[k - 1] * p + cz.ah[k - 1] * duxz;
+        c2 = duxz + p;
+      }
+      vxz = a.muhh[c] * (c1 + c2);
+      ws = true;
+    }
+  }
+  if (upd && a.kind == 1) {
+    // moment-tensor sources in index order; cells the stencil does not
+    // update keep their previous value plus the injection
+    const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+    const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+    int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
+    if (u >= 0) {
+      if (!wn) { vxx = sg[c]; vzz = sg[N + c]; }
+      for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++) {
+        const int p = a.f0.epoint[e];
+        const T f = tf[(int64_t)nts * p];
+        vxx += M[3 * p + 0] * a.f0.ecoef[e] * f;
+        vzz += M[3 * p + 1] * a.f0.ecoef[e] * f;
+      }
+      wn = true;
+    }
+    u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, j - 1, c);
+    if (u >= 0) {
+      if (!ws) vxz = sg[2 * N + c];
+      for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++) {
+        const int p = a.f1.epoint[e];
+        vxz += M[3 * p + 2] * a.f1.ecoef[e] * tf[(int64_t)nts * p];
+      }
+      ws = true;
+    }
+  }
+  if (wn) { sg[c] = vxx; sg[N + c] = vzz; }
+  if (ws) sg[2 * N + c] = vxz;
+
+  if (corr) {
+    // correlate_gradients! of adjoint step corr_step (adjcur = ucur here)
+    const int sl = a.corr_step - a.cbase + 1;   // slot of u^{corr_step}
+    const T *h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
+    const T *h1 = h0 + a.hslot;
+    const T *h2 = h1 + a.hslot;
+    T *g = a.grad + (int64_t)s * a.gstride;
+    if (i <= nx - 1) {
+      const T v = ux(i, j) * (h0[c] - T(2) * h1[c] + h2[c]) * a.idt2;
+      g[c] += (j == 1 && ft) ? v / T(2) : v;
+    }
+    if (j <= nz - 1) g[N + c] += uz(i, j) * (h0[N + c] - T(2) * h1[N + c] + h2[N + c]) * a.idt2;
+    const F2<T> fx{h1, nx}, fz{h1 + N, nx};
+    if (nrm) {
+      const T exx = d_ux_dx(fx, i, j, nx, a.idx);
+      const T ezz = d_uz_dz(fx, fz, a.lam[c], a.mu[c], i, j, nx, nz, ft, a.idx, a.idz);
+      const T dv = exx + ezz, dva = dux + duz;
+      if (j == 1 && ft) {
+        g[2 * N + c] += dv * dva / T(2);
+        g[3 * N + c] += exx * dux + ezz * duz;
+      } else {
+        g[2 * N + c] += dv * dva;
+        g[3 * N + c] += T(2) * (exx * dux + ezz * duz);
+      }
+    }
+    if (shr) {
+      const T exz = (d_uz_dx(fz, i, j, nx, a.idx) + d_ux_dz(fx, i, j, nz, ft, a.idz)) / T(2);
+      const T exza = (duzx + duxz) / T(2);
+      g[4 * N + c] += T(2) * (exz * exza + exz * exza);
+    }
+  }
+}
+
+// Displacement update (update_ux! :1-18, update_uz! :20-37) into the slot of
+// uold, force / adjoint-residual injection (:96-106) and the forward history.
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (i > nx || j > nz) return;
+  const bool ft = a.freetop != 0;
+  const int64_t N = a.N;
+  const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+  const T *uc = a.ucur + (int64_t)s * 2 * N;
+  T *un = a.unew + (int64_t)s * 2 * N;
+  const T *sg = a.sig + (int64_t)s * 3 * N;
+  const F2<T> sxx{sg, nx}, szz{sg + N, nx}, sxz{sg + 2 * N, nx};
+  T *ps = a.psi + (int64_t)s * a.psi_stride;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  const bool hx = i <= nx - 1, hz = j <= nz - 1;
+  T oldx = 0, oldz = 0, curx = 0, curz = 0, nux = 0, nuz = 0;
+  if (hx) {
+    const T d1 = d_sxx_dx(sxx, i, j, nx, a.idx);
+    const T d2 = d_sxz_dz(sxz, i, j, nz, ft, a.idz);
+    T c1 = d1, c2 = d2;
+    int k = strip_full(i, nx, h);
+    if (k) {
+      T &p = ps[a.psioff[0] + (k - 1) + (int64_t)(j - 1) * (2 * h)];
+      p = cx.bh[k - 1] * p + cx.ah[k - 1] * d1;
+      c1 = d1 + p;
+    }
+    k = strip_half(j - 1, nz - 1, h);
+    if (k) {
+      T &p = ps[a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1)];
+      p = cz.b[k - 1] * p + cz.a[k - 1] * d2;
+      c2 = d2 + p;
+    }
+    oldx = un[c];
+    curx = uc[c];
+    nux = T(2) * curx - oldx + a.rdx[c] * (c1 + c2);
+  }
+  if (hz) {
+    const T d1 = d_sxz_dx(sxz, i, j, nx, a.idx);
+    const T d2 = d_szz_dz(szz, i, j, nz, ft, a.idz);
+    T c1 = d1, c2 = d2;
+    int k = strip_half(i - 1, nx - 1, h);
+    if (k) {
+      T &p = ps[a.psioff[1] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2)];
+      p = cx.b[k - 1] * p + cx.a[k - 1] * d1;
+      c1 = d1 + p;
+    }
+    k = strip_full(j, nz, h);
+    if (k) {
+      T &p = ps[a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx];
+      p = cz.bh[k - 1] * p + cz.ah[k - 1] * d2;
+      c2 = d2 + p;
+    }
+    oldz = un[N + c];
+    curz = uc[N + c];
+    nuz = T(2) * curz - oldz + a.rdz[c] * (c1 + c2);
+  }
+  if (a.kind == 2) {
+    // point forces (forward) or adjoint residuals, in point order
+    const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+    if (hx) {
+      const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
+      if (u >= 0)
+        for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++)
+          nux += a.f0.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f0.epoint[e])] * a.rdx[c];
+    }
+    if (hz) {
+      const int u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, j - 1, c);
+      if (u >= 0)
+        for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++)
+          nuz += a.f1.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f1.epoint[e] + 1)] * a.rdz[c];
+    }
+  }
+  if (hx) un[c] = nux;
+  if (hz) un[N + c] = nuz;
+  if (a.hmode) {
+    // forward: the final segment of this shot, from its last checkpoint
+    // floor(nt/S)*S on (LinearCheckpointer, checkpointers.jl); replay: the
+    // segment being replayed (its first two levels come from the checkpoint)
+    const int base = a.hmode == 1 ? (nts / a.S) * a.S : a.cbase;
+    if (a.it > base) {
+      T *hs = a.hist + (int64_t)s * 2 * N + c;
+      const int sl = a.it - base + 1;
+      hs[(int64_t)sl * a.hslot] = nux;
+      hs[(int64_t)sl * a.hslot + N] = nuz;
+      if (a.hmode == 1 && a.it == base + 1) {
+        hs[0] = oldx;
+        hs[N] = oldz;
+        hs[a.hslot] = curx;
+        hs[a.hslot + N] = curz;
+      }
+    }
+  }
+}
+
+// Copies one displacement level (ux | uz of every shot) into a history slot
+// for the shots being replayed (nt_s > act_min).
+template <typename T>
+__global__ void el_hist_copy(T *dst, const T *src, int64_t n2, const int *nt_shot, int act_min) {
+  const int s = blockIdx.y;
+  if (nt_shot[s] <= act_min) return;
+  const int64_t o = (int64_t)s * n2;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n2; q += (int64_t)gridDim.x * blockDim.x)
+    dst[o + q] = src[o + q];
+}
+
+// traces[it, comp, r] = sum over taps of coef * u (record_receivers2D_* and
+// the per-receiver reduction, elastic2D_iso_xPU.jl:108-118, :216-231).
+template <typename T>
+__global__ void el_record_kernel(const T *u, int64_t N, TapView<T> r0, TapView<T> r1, const int *nt_shot,
+                                 const int64_t *off_tr, T *traces, int it, int maxrec) {
+  const int s = blockIdx.y;
+  const int nts = nt_shot[s];
+  if (it > nts) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int comp = q & 1, r = q >> 1;
+  const int p0 = r0.poff[s], np = r0.poff[s + 1] - p0;
+  if (r >= np || r >= maxrec) return;
+  const TapView<T> &t = comp ? r1 : r0;
+  const T *f = u + (int64_t)s * 2 * N + comp * N;
+  T acc = 0;
+  for (int k = t.pptr[p0 + r]; k < t.pptr[p0 + r + 1]; k++) acc += t.pcoef[k] * f[t.pcell[k]];
+  traces[off_tr[s] + (it - 1) + (int64_t)nts * (comp + 2 * r)] = acc;
+}
+
+// dt^2 / rho on a padded slab (the spare column or row holds rho = 0 and
+// stays 0; it is never read as data)
+template <typename T>
+__global__ void el_rdt_kernel(T *out, const T *rho, int64_t n, T dt2) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    out[q] = rho[q] != T(0) ? dt2 / rho[q] : T(0);
+}
+
+// Host-side tap table of one family for a batch of shots.
+struct TapTable {
+  DevBuf cell, eptr, epoint, ecoef, rows, poff, pptr, pcell, pcoef;
+  template <typename T>
+  TapView<T> view() const {
+    return TapView<T>{cell.as<int64_t>(), eptr.as<int>(),   epoint.as<int>(), ecoef.as<T>(), rows.as<int>(),
+                      poff.as<int>(),     pptr.as<int>(),   pcell.as<int64_t>(), pcoef.as<T>()};
+  }
+};
+
+template <typename T>
+void build_taps(TapTable &t, int nshots, const int *npts, const swx_taps *const *fam, int nx, int nz, int ni, int nj,
+                const char *what, hipStream_t stream) {
+  std::vector<int64_t> cell, pcell;
+  std::vector<int> eptr(1, 0), epoint, rows, poff(1, 0), pptr(1, 0);
+  std::vector<T> ecoef, pcoef;
+  for (int s = 0; s < nshots; s++) {
+    const swx_taps &f = *fam[s];
+    SWX_REQUIRE(f.ptr && (f.ptr[npts[s]] == 0 || (f.ij && f.coef)), SWX_EINVAL, "null tap array");
+    std::map<int64_t, std::vector<std::pair<int, T>>> g;
+    const T *coef = static_cast<const T *>(f.coef);
+    for (int p = 0; p < npts[s]; p++) {
+      SWX_REQUIRE(f.ptr[p] <= f.ptr[p + 1], SWX_EINVAL, "tap offsets must be non-decreasing");
+      for (int k = f.ptr[p]; k < f.ptr[p + 1]; k++) {
+        const int64_t i = f.ij[2 * (int64_t)k], j = f.ij[2 * (int64_t)k + 1];
+        if (i < 1 || i > ni || j < 1 || j > nj)
+          throw Error(SWX_EINVAL, std::string(what) + " position of shot " + std::to_string(s + 1) +
+                                      " is not inside the grid!");
+        const int64_t c = (i - 1) + (j - 1) * (int64_t)nx;
+        g[c].push_back({p, coef[k]});
+        pcell.push_back(c);
+        pcoef.push_back(coef[k]);
+      }
+      pptr.push_back((int)pcell.size());
+    }
+    poff.push_back(poff.back() + npts[s]);
+    const int u0 = (int)cell.size();
+    for (auto &kv : g) {
+      cell.push_back(kv.first);
+      for (auto &pe : kv.second) {
+        epoint.push_back(pe.first);
+        ecoef.push_back(pe.second);
+      }
+      eptr.push_back((int)epoint.size());
+    }
+    int u = u0;
+    for (int64_t r = 0; r <= nz; r++) {
+      while (u < (int)cell.size() && cell[u] / nx < r) u++;
+      rows.push_back(u);
+    }
+  }
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+    b.alloc(std::max<size_t>(bytes, 16));
+    if (bytes) SWX_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, stream));
+  };
+  up(t.cell, cell.data(), cell.size() * sizeof(int64_t));
+  up(t.eptr, eptr.data(), eptr.size() * sizeof(int));
+  up(t.epoint, epoint.data(), epoint.size() * sizeof(int));
+  up(t.ecoef, ecoef.data(), ecoef.size() * sizeof(T));
+  up(t.rows, rows.data(), rows.size() * sizeof(int));
+  up(t.poff, poff.data(), poff.size() * sizeof(int));
+  up(t.pptr, pptr.data(), pptr.size() * sizeof(int));
+  up(t.pcell, pcell.data(), pcell.size() * sizeof(int64_t));
+  up(t.pcoef, pcoef.data(), pcoef.size() * sizeof(T));
+  SWX_HIP(hipStreamSynchronize(stream));
+}
+
+template <typename T>
+class El2dEngine final : public Engine {
+ public:
+  explicit El2dEngine(PlanCore &core) : c_(core) {
+    const swx_plan_desc &d = c_.d;
+    nx_ = (int)d.n[0];
+    nz_ = (int)d.n[1];
+    h_ = d.halo;
+    ns_ = d.max_shots;
+    N_ = (int64_t)nx_ * nz_;
+    const int64_t h = h_, nx = nx_, nz = nz_;
+    const int64_t sz[8] = {2 * h * nz,       (2 * h + 2) * (nz - 1), nx * 2 * h,       (nx - 1) * (2 * h + 2),
+                           (2 * h + 2) * nz, 2 * h * (nz - 1),       (nx - 1) * 2 * h, nx * (2 * h + 2)};
+    int64_t off = 0;
+    for (int k = 0; k < 8; k++) {
+      psioff_[k] = off;
+      off += sz[k];
+    }
+    psi_stride_ = std::max<int64_t>(off, 1);
+    cstride_ = 2 * (8 * h_ + 4);
+    const size_t e = sizeof(T);
+    model_.alloc(5 * N_ * e);
+    u_.alloc(2 * ns_ * 2 * N_ * e);
+    sig_.alloc(ns_ * 3 * N_ * e);
+    psi_.alloc(ns_ * psi_stride_ * e);
+    cpml_.alloc((int64_t)ns_ * cstride_ * e);
+    state_bytes_ = model_.bytes() + u_.bytes() + sig_.bytes() + psi_.bytes();
+    if (d.gradient) {
+      au_.alloc(u_.bytes());
+      asig_.alloc(sig_.bytes());
+      apsi_.alloc(psi_.bytes());
+      grad_.alloc(ns_ * 5 * N_ * e);
+      gout_.alloc(ns_ * 5 * N_ * e);
+      state_bytes_ += au_.bytes() + asig_.bytes() + apsi_.bytes() + grad_.bytes() + gout_.bytes();
+      const int64_t slot = ns_ * 2 * N_ * (int64_t)e;
+      ck_bytes_one_ = (int64_t)u_.bytes() + (int64_t)psi_.bytes();
+      const int64_t avail = (int64_t)c_.budget - state_bytes_ - 2 * slot - 2 * ck_bytes_one_;
+      S_ = choose_segment(d.nt_max, d.check_freq, slot, ck_bytes_one_, avail);
+      // check_freq = 1 keeps every step (no replay); otherwise a checkpoint
+      // at every multiple of S, as the reference's LinearCheckpointer
+      store_all_ = d.check_freq <= 1 && S_ == d.nt_max;
+      K_ = (int)ceil_div(d.nt_max, S_);
+      nck_ = store_all_ ? 0 : d.nt_max / S_;
+      hist_.alloc((int64_t)(S_ + 2) * slot);
+      ck_.alloc(std::max<int64_t>(nck_ * ck_bytes_one_, 1));
+    }
+    zero_forward_state(true);
+  }
+
+  void set_model(int nfields, const void *const *fields) override {
+    SWX_REQUIRE(nfields == 5, SWX_EINVAL, "elastic expects five model fields (lambda, mu, rho_ihalf, rho_jhalf, "
+                                          "mu_ihalf_jhalf)");
+    for (int k = 0; k < 5; k++) SWX_REQUIRE(fields[k], SWX_EINVAL, "null model field");
+    const size_t e = sizeof(T);
+    SWX_HIP(hipMemsetAsync(model_.get(), 0, model_.bytes(), c_.stream));
+    SWX_HIP(hipMemcpyAsync(M(0), fields[0], N_ * e, hipMemcpyHostToDevice, c_.stream));
+    SWX_HIP(hipMemcpyAsync(M(1), fields[1], N_ * e, hipMemcpyHostToDevice, c_.stream));
+    DevBuf tmp(2 * N_ * e);
+    T *rx = tmp.as<T>(), *rz = rx + N_;
+    SWX_HIP(hipMemsetAsync(tmp.get(), 0, tmp.bytes(), c_.stream));
+    // rho_ihalf (nx-1, nz) and mu_ihalf_jhalf (nx-1, nz-1) into nx-pitched slabs
+    SWX_HIP(hipMemcpy2DAsync(rx, nx_ * e, fields[2], (nx_ - 1) * e, (nx_ - 1) * e, nz_, hipMemcpyHostToDevice,
+                             c_.stream));
+    SWX_HIP(hipMemcpyAsync(rz, fields[3], (int64_t)nx_ * (nz_ - 1) * e, hipMemcpyHostToDevice, c_.stream));
+    SWX_HIP(hipMemcpy2DAsync(M(2), nx_ * e, fields[4], (nx_ - 1) * e, (nx_ - 1) * e, nz_ - 1, hipMemcpyHostToDevice,
+                             c_.stream));
+    const T dt = T(c_.d.dt);
+    dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 2048));
+    el_rdt_kernel<T><<<g, 256, 0, c_.stream>>>(M(3), rx, N_, dt * dt);
+    el_rdt_kernel<T><<<g, 256, 0, c_.stream>>>(M(4), rz, N_, dt * dt);
+    SWX_HIP(hipGetLastError());
+    SWX_HIP(hipStreamSynchronize(c_.stream));
+    model_set_ = true;
+  }
+
+  void on_shots_set(int, const swx_shot_desc *) override {
+    throw Error(SWX_EINVAL, "elastic plans take swx_set_shots_elastic");
+  }
+
+  void on_elastic_shots(int nshots, const swx_elastic_shot_desc *shots) override {
+    const int esz = sizeof(T);
+    PointTables &pt = c_.pts;
+    pt.nshots = nshots;
+    pt.nt.assign(nshots, 0);
+    pt.nsrc.assign(nshots, 0);
+    pt.nrec.assign(nshots, 0);
+    pt.off_tf.assign(nshots, 0);
+    pt.off_tr.assign(nshots, 0);
+    std::vector<int> nsrc(nshots), nrec(nshots), offm(nshots + 1, 0);
+    int64_t otf = 0, otr = 0;
+    kind_ = shots[0].source_kind;
+    for (int s = 0; s < nshots; s++) {
+      const swx_elastic_shot_desc &sh = shots[s];
+      SWX_REQUIRE(sh.source_kind == SWX_SRC_MOMENT_TENSOR || sh.source_kind == SWX_SRC_EXTERNAL_FORCE, SWX_EINVAL,
+                  "source_kind must be SWX_SRC_MOMENT_TENSOR or SWX_SRC_EXTERNAL_FORCE");
+      SWX_REQUIRE(sh.source_kind == kind_, SWX_EINVAL, "Types of shots are inconsistent.");
+      SWX_REQUIRE(sh.nsrc > 0, SWX_EINVAL, "There must be at least one source!");
+      SWX_REQUIRE(sh.nrec > 0, SWX_EINVAL, "There must be at least one receiver!");
+      SWX_REQUIRE(sh.srctf && (kind_ == SWX_SRC_EXTERNAL_FORCE || sh.moments), SWX_EINVAL,
+                  "null source array");
+      pt.nt[s] = sh.nt;
+      pt.nsrc[s] = nsrc[s] = sh.nsrc;
+      pt.nrec[s] = nrec[s] = sh.nrec;
+      pt.off_tf[s] = otf;
+      pt.off_tr[s] = otr;
+      offm[s + 1] = offm[s] + sh.nsrc;
+      otf += (int64_t)sh.nt * sh.nsrc * (kind_ == SWX_SRC_EXTERNAL_FORCE ? 2 : 1);
+      otr += (int64_t)sh.nt * 2 * sh.nrec;
+    }
+    pt.total_tf = otf;
+    pt.total_tr = otr;
+    std::vector<const swx_taps *> f[4];
+    for (int s = 0; s < nshots; s++) {
+      f[0].push_back(&shots[s].src[0]);
+      f[1].push_back(&shots[s].src[1]);
+      f[2].push_back(&shots[s].rec[0]);
+      f[3].push_back(&shots[s].rec[1]);
+    }
+    const bool mt = kind_ == SWX_SRC_MOMENT_TENSOR;
+    // grids of the families: sxx/szz nx x nz, sxz (nx-1) x (nz-1), ux (nx-1) x nz, uz nx x (nz-1)
+    build_taps<T>(taps_[0], nshots, nsrc.data(), f[0].data(), nx_, nz_, mt ? nx_ : nx_ - 1, mt ? nz_ : nz_, "Source",
+                  c_.stream);
+    build_taps<T>(taps_[1], nshots, nsrc.data(), f[1].data(), nx_, nz_, mt ? nx_ - 1 : nx_, nz_ - 1, "Source",
+                  c_.stream);
+    build_taps<T>(taps_[2], nshots, nrec.data(), f[2].data(), nx_, nz_, nx_ - 1, nz_, "Receiver", c_.stream);
+    build_taps<T>(taps_[3], nshots, nrec.data(), f[3].data(), nx_, nz_, nx_, nz_ - 1, "Receiver", c_.stream);
+    maxrec_ = *std::max_element(nrec.begin(), nrec.end());
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+      b.alloc(std::max<size_t>(bytes, 16));
+      if (bytes) SWX_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, c_.stream));
+    };
+    up(pt.d_nt, pt.nt.data(), sizeof(int) * nshots);
+    up(pt.d_off_tf, pt.off_tf.data(), sizeof(int64_t) * nshots);
+    up(pt.d_off_tr, pt.off_tr.data(), sizeof(int64_t) * nshots);
+    up(offm_, offm.data(), sizeof(int) * offm.size());
+    std::vector<char> tf((size_t)otf * esz), mom((size_t)offm[nshots] * 3 * esz);
+    for (int s = 0; s < nshots; s++) {
+      const int comps = mt ? 1 : 2;
+      std::memcpy(tf.data() + (size_t)pt.off_tf[s] * esz, shots[s].srctf,
+                  (size_t)shots[s].nt * comps * shots[s].nsrc * esz);
+      if (mt) std::memcpy(mom.data() + (size_t)offm[s] * 3 * esz, shots[s].moments, (size_t)shots[s].nsrc * 3 * esz);
+    }
+    up(pt.d_srctf, tf.data(), tf.size());
+    up(mom_, mom.data(), mom.size());
+    pt.d_traces.alloc(std::max<size_t>((size_t)otr * esz, 16));
+    pt.d_res.alloc(std::max<size_t>((size_t)otr * esz, 16));
+    SWX_HIP(hipMemsetAsync(pt.d_traces.get(), 0, (size_t)otr * esz, c_.stream));
+    // C-PML coefficients per shot
+    std::vector<swx_shot_desc> tmp(nshots);
+    for (int s = 0; s < nshots; s++)
+      for (int ax = 0; ax < 3; ax++)
+        for (int k = 0; k < 4; k++) tmp[s].cpml[ax][k] = shots[s].cpml[ax][k];
+    std::vector<T> hc = pack_cpml<T>(nshots, tmp.data(), 2, h_);
+    SWX_HIP(hipMemcpyAsync(cpml_.get(), hc.data(), hc.size() * sizeof(T), hipMemcpyHostToDevice, c_.stream));
+    SWX_HIP(hipStreamSynchronize(c_.stream));
+    nsh_ = nshots;
+    nt_ = pt.nt;
+    ntmax_ = *std::max_element(nt_.begin(), nt_.end());
+  }
+
+  void forward(bool save) override {
+    SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
+    zero_forward_state(true);
+    if (save) SWX_HIP(hipMemsetAsync(hist_.get(), 0, hist_.bytes(), c_.stream));
+    T *tr = c_.pts.d_traces.as<T>();
+    for (int it = 1; it <= ntmax_; it++) {
+      step_fwd(it, 0, save ? 1 : 0, 0, tr);
+      if (save && it % S_ == 0) {
+        const int k = it / S_;
+        if (k >= 1 && k <= nck_) ckpt_copy(k, true);
+      }
+    }
+  }
+
+  void adjoint(bool per_shot) override {
+    SWX_HIP(hipMemsetAsync(au_.get(), 0, au_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(asig_.get(), 0, asig_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(apsi_.get(), 0, apsi_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
+    const int Kmax = (int)ceil_div(ntmax_, S_);
+    const int64_t n2 = 2 * N_;
+    for (int k = Kmax - 1; k >= 0; k--) {
+      const int cb = k * S_;
+      // shots whose checkpoint (k+1)S exists replay segment k: levels cb-1
+      // and cb from checkpoint k, cb+1 .. cb+S-1 recomputed, cb+S-1 and
+      // cb+S from checkpoint k+1 (initrecover!/recover!/getsaved,
+      // checkpointers.jl:80-105)
+      const int act = cb + S_ - 1;
+      bool need = false;
+      for (int s = 0; s < nsh_; s++) need |= !store_all_ && nt_[s] > act;
+      if (need) {
+        if (k == 0) zero_forward_state(false);
+        else ckpt_copy(k, false);
+        T *hs = hist_.as<T>();
+        const int64_t hslot = (int64_t)ns_ * n2;
+        dim3 g((unsigned)std::min<int64_t>(ceil_div(n2, 256), 1024), (unsigned)nsh_);
+        el_hist_copy<T><<<g, 256, 0, c_.stream>>>(hs, U(u_, cb - 1), n2, c_.pts.d_nt.as<int>(), act);
+        el_hist_copy<T><<<g, 256, 0, c_.stream>>>(hs + hslot, U(u_, cb), n2, c_.pts.d_nt.as<int>(), act);
+        SWX_HIP(hipGetLastError());
+        for (int it = cb + 1; it <= cb + S_ - 1; it++) step_fwd(it, act, 2, cb, nullptr);
+        // u^{cb+S-1} and u^{cb+S} are read from checkpoint k+1 (getsaved
+        // prefers checkpointed levels over the replayed buffer)
+        const T *ck1 = reinterpret_cast<const T *>(ck_.as<char>() + (int64_t)k * ck_bytes_one_);
+        for (int t = cb + S_ - 1; t <= cb + S_; t++)
+          el_hist_copy<T><<<g, 256, 0, c_.stream>>>(hs + (int64_t)(t - cb + 1) * hslot, ck1 + (int64_t)(t % 2) * ns_ * n2,
+                                                    n2, c_.pts.d_nt.as<int>(), act);
+        SWX_HIP(hipGetLastError());
+      }
+      const int top = std::min(cb + S_, ntmax_);
+      for (int it = top; it >= cb + 1; it--) step_adj(it, cb);
+      // correlation of the first step of the segment
+      ElArgs<T> a = base_args(cb);
+      a.ucur = U(au_, cb + 1);
+      a.sig = asig_.as<T>();
+      a.psi = apsi_.as<T>();
+      a.update = 0;
+      a.corr_step = cb + 1;
+      a.corr_lim = cb + S_;
+      a.cbase = cb;
+      launch_stress(a, 1);
+    }
+    const int64_t G = 5 * N_;
+    dim3 g((unsigned)std::min<int64_t>(ceil_div(G, 256), 4096));
+    reduce_shots<T><<<g, 256, 0, c_.stream>>>(gout_.as<T>(), grad_.as<T>(), G, nsh_, T(1), per_shot ? 1 : 0);
+    SWX_HIP(hipGetLastError());
+  }
+
+  void copy_grad(void *host, bool per_shot) override {
+    // unpad: g_rho_ihalf (nx-1, nz) | g_rho_jhalf (nx, nz-1) | g_lambda | g_mu | g_mu_ihalf_jhalf (nx-1, nz-1)
+    const size_t e = sizeof(T);
+    char *o = static_cast<char *>(host);
+    const int copies = per_shot ? nsh_ : 1;
+    for (int s = 0; s < copies; s++) {
+      const char *b = gout_.as<char>() + (size_t)s * 5 * N_ * e;
+      SWX_HIP(hipMemcpy2D(o, (nx_ - 1) * e, b, nx_ * e, (nx_ - 1) * e, nz_, hipMemcpyDeviceToHost));
+      o += (size_t)(nx_ - 1) * nz_ * e;
+      SWX_HIP(hipMemcpy(o, b + N_ * e, (size_t)nx_ * (nz_ - 1) * e, hipMemcpyDeviceToHost));
+      o += (size_t)nx_ * (nz_ - 1) * e;
+      SWX_HIP(hipMemcpy(o, b + 2 * N_ * e, (size_t)2 * N_ * e, hipMemcpyDeviceToHost));
+      o += (size_t)2 * N_ * e;
+      SWX_HIP(hipMemcpy2D(o, (nx_ - 1) * e, b + 4 * N_ * e, nx_ * e, (nx_ - 1) * e, nz_ - 1, hipMemcpyDeviceToHost));
+      o += (size_t)(nx_ - 1) * (nz_ - 1) * e;
+    }
+  }
+
+  int64_t bytes_state() const override { return state_bytes_; }
+  int64_t bytes_checkpoints() const override { return ck_.bytes(); }
+  int64_t bytes_replay() const override { return hist_.bytes(); }
+  int segment() const override { return S_; }
+
+ private:
+  T *M(int k) const { return model_.as<T>() + (int64_t)k * N_; }
+  // time level k of a displacement buffer (slot k mod 2)
+  T *U(const DevBuf &b, int k) const { return b.as<T>() + (int64_t)(((k % 2) + 2) % 2) * ns_ * 2 * N_; }
+
+  void zero_forward_state(bool with_stress) {
+    SWX_HIP(hipMemsetAsync(u_.get(), 0, u_.bytes(), c_.stream));
+    SWX_HIP(hipMemsetAsync(psi_.get(), 0, psi_.bytes(), c_.stream));
+    // the stresses are not part of the reference's checkpoints: a replay
+    // from the start keeps them (only never-updated edge cells matter)
+    if (with_stress) SWX_HIP(hipMemsetAsync(sig_.get(), 0, sig_.bytes(), c_.stream));
+  }
+
+  ElArgs<T> base_args(int it) const {
+    ElArgs<T> a{};
+    a.nx = nx_; a.nz = nz_; a.h = h_; a.freetop = c_.d.freetop ? 1 : 0;
+    a.N = N_; a.nrows = nz_;
+    a.idx = T(1.0 / c_.d.h[0]);
+    a.idz = T(1.0 / c_.d.h[1]);
+    a.idt2 = T(1.0 / (c_.d.dt * c_.d.dt));
+    a.lam = M(0); a.mu = M(1); a.muhh = M(2); a.rdx = M(3); a.rdz = M(4);
+    a.cpml = cpml_.as<T>(); a.cstride = cstride_;
+    a.nt_shot = c_.pts.d_nt.as<int>();
+    a.it = it;
+    a.psi_stride = psi_stride_;
+    for (int k = 0; k < 8; k++) a.psioff[k] = psioff_[k];
+    a.off_tf = c_.pts.d_off_tf.as<int64_t>();
+    a.mom = mom_.as<T>(); a.off_mom = offm_.as<int>();
+    a.hist = hist_.as<T>(); a.hslot = (int64_t)ns_ * 2 * N_;
+    a.S = store_all_ ? (1 << 30) : S_;   // store-all: the final segment is the whole run
+    a.grad = grad_.as<T>(); a.gstride = 5 * N_;
+    a.update = 1;
+    return a;
+  }
+
+  void launch_stress(const ElArgs<T> &a, int kind) {
+    dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+    c_.ktimer.begin(c_.stream);
+    el_stress_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    c_.ktimer.end(c_.stream, kind);
+    SWX_HIP(hipGetLastError());
+  }
+  void launch_disp(const ElArgs<T> &a, int kind) {
+    dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+    c_.ktimer.begin(c_.stream);
+    el_disp_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    c_.ktimer.end(c_.stream, kind);
+    SWX_HIP(hipGetLastError());
+  }
+
+  void step_fwd(int it, int act_min, int hmode, int cbase, T *traces) {
+    ElArgs<T> a = base_args(it);
+    a.ucur = U(u_, it - 1);
+    a.unew = U(u_, it);
+    a.sig = sig_.as<T>();
+    a.psi = psi_.as<T>();
+    a.act_min = act_min;
+    a.f0 = taps_[0].view<T>();
+    a.f1 = taps_[1].view<T>();
+    a.tf = c_.pts.d_srctf.as<T>();
+    a.ncomp = 2;
+    a.kind = kind_ == SWX_SRC_MOMENT_TENSOR ? 1 : 0;
+    launch_stress(a, 0);
+    a.kind = kind_ == SWX_SRC_EXTERNAL_FORCE ? 2 : 0;
+    a.hmode = c_.d.gradient ? hmode : 0;
+    a.cbase = cbase;
+    launch_disp(a, 0);
+    if (traces) {
+      const int thr = 2 * maxrec_;
+      dim3 g((unsigned)ceil_div(thr, 128), (unsigned)nsh_);
+      el_record_kernel<T><<<g, 128, 0, c_.stream>>>(U(u_, it), N_, taps_[2].view<T>(), taps_[3].view<T>(),
+                                                     c_.pts.d_nt.as<int>(), c_.pts.d_off_tr.as<int64_t>(), traces,
+                                                     it, maxrec_);
+      SWX_HIP(hipGetLastError());
+    }
+  }
+
+  void step_adj(int it, int cbase) {
+    ElArgs<T> a = base_args(it);
+    a.ucur = U(au_, it + 1);
+    a.unew = U(au_, it);
+    a.sig = asig_.as<T>();
+    a.psi = apsi_.as<T>();
+    a.cbase = cbase;
+    a.corr_step = it + 1;
+    a.corr_lim = cbase + S_;
+    a.kind = 0;
+    launch_stress(a, 1);
+    a.corr_step = 0;
+    a.kind = 2;
+    a.f0 = taps_[2].view<T>();
+    a.f1 = taps_[3].view<T>();
+    a.tf = c_.pts.d_res.as<T>();
+    a.off_tf = c_.pts.d_off_tr.as<int64_t>();
+    a.ncomp = 2;
+    launch_disp(a, 1);
+  }
+
+  // checkpoint k (k = 1 .. floor(nt_max/S)): both displacement levels
+  // (u^{kS-1}, u^{kS}) and the C-PML memory variables of every shot
+  void ckpt_copy(int k, bool save) {
+    char *base = ck_.as<char>() + (int64_t)(k - 1) * ck_bytes_one_;
+    struct Part { void *dev; int64_t bytes; };
+    Part parts[2] = {{u_.get(), (int64_t)u_.bytes()}, {psi_.get(), (int64_t)psi_.bytes()}};
+    int64_t off = 0;
+    for (auto &p : parts) {
+      if (save)
+        SWX_HIP(hipMemcpyAsync(base + off, p.dev, p.bytes, hipMemcpyDeviceToDevice, c_.stream));
+      else
+        SWX_HIP(hipMemcpyAsync(p.dev, base + off, p.bytes, hipMemcpyDeviceToDevice, c_.stream));
+      off += p.bytes;
+    }
+  }
+
+  PlanCore &c_;
+  int nx_, nz_, h_, ns_;
+  int64_t N_, psi_stride_ = 1;
+  int64_t psioff_[8];
+  int cstride_;
+  DevBuf model_, u_, sig_, psi_, cpml_;
+  DevBuf au_, asig_, apsi_, grad_, gout_, hist_, ck_;
+  DevBuf mom_, offm_;
+  TapTable taps_[4];
+  int kind_ = SWX_SRC_MOMENT_TENSOR, maxrec_ = 0;
+  int64_t ck_bytes_one_ = 0, state_bytes_ = 0;
+  int S_ = 1, K_ = 1, nck_ = 0;
+  bool store_all_ = true;
+  int nsh_ = 0, ntmax_ = 0;
+  std::vector<int> nt_;
+  bool model_set_ = false;
+};
+
+}  // namespace
+
+std::unique_ptr<Engine> make_elastic2d(PlanCore &core) {
+  if (core.d.dtype == SWX_F32) return std::make_unique<El2dEngine<float>>(core);
+  return std::make_unique<El2dEngine<double>>(core);
+}
+
+}  // namespace swx

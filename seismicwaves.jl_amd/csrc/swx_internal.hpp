@@ -154,6 +154,11 @@ class Engine {
   // Receives the shot descriptors after the point tables were built (the
   // engine uploads the per-shot C-PML coefficients).
   virtual void on_shots_set(int nshots, const swx_shot_desc *shots) = 0;
+  // Elastic engines build their own tap tables (and the trace buffers of
+  // the PlanCore point tables) from the elastic shot descriptors.
+  virtual void on_elastic_shots(int, const swx_elastic_shot_desc *) {
+    throw Error(SWX_EINVAL, "swx_set_shots_elastic needs an SWX_ELASTIC_ISO plan");
+  }
   virtual void forward(bool save_state) = 0;
   // Runs the adjoint loop on the device (residuals already in pts.d_res).
   virtual void adjoint(bool per_shot) = 0;
@@ -181,6 +186,7 @@ struct PlanCore {
 std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core);
 std::unique_ptr<Engine> make_acoustic_vd2d(PlanCore &core);
 std::unique_ptr<Engine> make_acoustic_cd3d(PlanCore &core);
+std::unique_ptr<Engine> make_elastic2d(PlanCore &core);
 
 // Helpers
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -246,6 +246,8 @@ int swx_plan_create(const swx_plan_desc *desc, swx_plan **out) {
       p->engine = make_acoustic_cd3d(p->core);
     } else if (d.physics == SWX_ACOUSTIC_VD && d.ndim == 2) {
       p->engine = make_acoustic_vd2d(p->core);
+    } else if (d.physics == SWX_ELASTIC_ISO && d.ndim == 2) {
+      p->engine = make_elastic2d(p->core);
     } else {
       throw Error(SWX_EUNSUPPORTED, "physics/dimension combination not supported by this build");
     }
@@ -282,10 +284,25 @@ int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots) {
     for (int s = 0; s < nshots; s++)
       SWX_REQUIRE(shots[s].nt >= 1 && shots[s].nt <= d.nt_max, SWX_EINVAL,
                   "shot nt must be in 1..nt_max");
+    SWX_REQUIRE(d.physics != SWX_ELASTIC_ISO, SWX_EINVAL, "elastic plans take swx_set_shots_elastic");
     SWX_HIP(hipSetDevice(d.device));
     build_point_tables(plan->core.pts, d.ndim, d.n, plan->core.esize, nshots, shots, plan->core.stream);
     plan->core.forward_done = false;
     plan->engine->on_shots_set(nshots, shots);
+  });
+}
+
+int swx_set_shots_elastic(swx_plan *plan, int nshots, const swx_elastic_shot_desc *shots) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && shots, SWX_EINVAL, "null argument");
+    SWX_REQUIRE(nshots >= 1 && nshots <= plan->core.d.max_shots, SWX_EINVAL,
+                "number of shots must be in 1..max_shots");
+    const swx_plan_desc &d = plan->core.d;
+    for (int s = 0; s < nshots; s++)
+      SWX_REQUIRE(shots[s].nt >= 1 && shots[s].nt <= d.nt_max, SWX_EINVAL, "shot nt must be in 1..nt_max");
+    SWX_HIP(hipSetDevice(d.device));
+    plan->core.forward_done = false;
+    plan->engine->on_elastic_shots(nshots, shots);
   });
 }
 

@@ -16,11 +16,12 @@ LIBPATH = os.path.join(_HERE, "libswx.so")
 SWX_OK, SWX_EINVAL, SWX_ENOMEM, SWX_EHIP, SWX_ESTATE, SWX_EUNSUPPORTED = range(6)
 SWX_ACOUSTIC_CD, SWX_ACOUSTIC_VD, SWX_ELASTIC_ISO = 1, 2, 3
 SWX_F32, SWX_F64 = 1, 2
+SWX_SRC_MOMENT_TENSOR, SWX_SRC_EXTERNAL_FORCE = 1, 2
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = (
     "swx_abi_version", "swx_last_error", "swx_plan_create", "swx_plan_destroy",
-    "swx_set_model", "swx_set_shots", "swx_forward", "swx_adjoint",
+    "swx_set_model", "swx_set_shots", "swx_set_shots_elastic", "swx_forward", "swx_adjoint",
     "swx_plan_get_info", "swx_set_kernel_timing",
 )
 
@@ -40,6 +41,18 @@ class ShotDesc(ctypes.Structure):
         ("nt", ctypes.c_int32), ("nsrc", ctypes.c_int32), ("src_idx", ctypes.c_void_p),
         ("srctf", ctypes.c_void_p), ("nrec", ctypes.c_int32), ("rec_idx", ctypes.c_void_p),
         ("cpml", (ctypes.c_void_p * 4) * 3),
+    ]
+
+
+class Taps(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("ij", ctypes.c_void_p), ("coef", ctypes.c_void_p)]
+
+
+class ElasticShotDesc(ctypes.Structure):
+    _fields_ = [
+        ("nt", ctypes.c_int32), ("source_kind", ctypes.c_int32), ("nsrc", ctypes.c_int32),
+        ("src", Taps * 2), ("moments", ctypes.c_void_p), ("srctf", ctypes.c_void_p),
+        ("nrec", ctypes.c_int32), ("rec", Taps * 2), ("cpml", (ctypes.c_void_p * 4) * 3),
     ]
 
 
@@ -75,6 +88,7 @@ def load(path: str | None = None):
     lib.swx_plan_destroy.argtypes = [vp]
     lib.swx_set_model.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
     lib.swx_set_shots.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ShotDesc)]
+    lib.swx_set_shots_elastic.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ElasticShotDesc)]
     lib.swx_forward.argtypes = [vp, vp, ctypes.c_int]
     lib.swx_adjoint.argtypes = [vp, vp, vp, ctypes.c_int]
     lib.swx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
@@ -169,6 +183,49 @@ class Plan:
                     keep.append(arr)
                     dd.cpml[ax][j] = arr.ctypes.data if arr.size else None
         check(load().swx_set_shots(self._h, len(shots), descs))
+        self._keep = keep
+
+    def _cpml(self, dd, cpml, keep):
+        for ax, co in enumerate(cpml):
+            for j, key in enumerate(("a", "a_h", "b", "b_h")):
+                arr = np.ascontiguousarray(co[key], dtype=self.dtype)
+                keep.append(arr)
+                dd.cpml[ax][j] = arr.ctypes.data if arr.size else None
+
+    def _taps(self, t, fam, keep):
+        """fam: list over points of (ij (ntaps, 2) 1-based, coef (ntaps,))."""
+        ptr_ = np.zeros(len(fam) + 1, dtype=np.int32)
+        for k, (ij, c) in enumerate(fam):
+            ptr_[k + 1] = ptr_[k] + len(c)
+        ij = np.ascontiguousarray(np.concatenate([np.asarray(f[0], dtype=np.int64).reshape(-1, 2) for f in fam]))
+        coef = np.ascontiguousarray(np.concatenate([np.asarray(f[1]) for f in fam]), dtype=self.dtype)
+        keep += [ptr_, ij, coef]
+        t.ptr, t.ij, t.coef = ptr_.ctypes.data, ij.ctypes.data, coef.ctypes.data
+
+    def set_shots_elastic(self, shots):
+        """shots: list of dicts with nt, kind (SWX_SRC_*), src (two tap
+        families), moments (nsrc, 3) for moment tensors, srctf ((nt, nsrc) or
+        (nt, 2, nsrc), divided by dx dz), rec (ux and uz tap families), cpml."""
+        keep = []
+        descs = (ElasticShotDesc * len(shots))()
+        for k, sh in enumerate(shots):
+            dd = descs[k]
+            dd.nt = int(sh["nt"])
+            dd.source_kind = int(sh["kind"])
+            dd.nsrc = len(sh["src"][0])
+            for f in range(2):
+                self._taps(dd.src[f], sh["src"][f], keep)
+                self._taps(dd.rec[f], sh["rec"][f], keep)
+            if sh.get("moments") is not None:
+                mom = np.ascontiguousarray(sh["moments"], dtype=self.dtype)
+                keep.append(mom)
+                dd.moments = mom.ctypes.data
+            tf = np.asfortranarray(sh["srctf"], dtype=self.dtype)
+            keep.append(tf)
+            dd.srctf = tf.ctypes.data
+            dd.nrec = len(sh["rec"][0])
+            self._cpml(dd, sh["cpml"], keep)
+        check(load().swx_set_shots_elastic(self._h, len(shots), descs))
         self._keep = keep
 
     def forward(self, traces: np.ndarray | None, save_state: bool):

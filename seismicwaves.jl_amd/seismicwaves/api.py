@@ -20,15 +20,17 @@ import numpy as np
 from . import _native
 from .cpml import init_bdc
 from .misfits import L2Misfit
-from .params import (ARITHMETIC, CPMLBoundaryConditionParameters, GradParameters, InputParametersAcoustic,
-                     RunParameters, VpAcousticCDMaterialProperties, VpRhoAcousticVDMaterialProperties)
-from .shots import ScalarShot
+from .params import (ARITHMETIC, CPMLBoundaryConditionParameters, ElasticIsoMaterialProperties, GradParameters,
+                     InputParametersAcoustic, InputParametersElastic, RunParameters, VpAcousticCDMaterialProperties,
+                     VpRhoAcousticVDMaterialProperties)
+from .points import nearest_taps, spread_positions
+from .shots import ExternalForceShot, MomentTensorShot, ScalarShot
 
 BACKENDS = ("mi355x",)
 
 
 def _dtype_of(matprop):
-    dt = np.asarray(matprop.vp).dtype
+    dt = np.asarray(matprop.lam if isinstance(matprop, ElasticIsoMaterialProperties) else matprop.vp).dtype
     return np.float32 if dt == np.float32 else np.float64
 
 
@@ -41,35 +43,42 @@ def find_nearest_grid_points(positions, spacing):
 # ---------------------------------------------------------------------------
 # Material interpolation for the staggered VD grid (interpolations.jl:1-47)
 # ---------------------------------------------------------------------------
+def _dims(dim):
+    return [dim] if np.isscalar(dim) else list(dim)
+
+
+def _corners(shape, dims):
+    """Slices selecting, for every corner of the averaging cell along
+    `dims`, the lower (0) or upper (1) neighbour of each half point."""
+    for bits in range(2 ** len(dims)):
+        sl = [slice(None)] * len(shape)
+        for b, d in enumerate(dims):
+            off = (bits >> (len(dims) - 1 - b)) & 1
+            sl[d - 1] = slice(off, shape[d - 1] - 1 + off)
+        yield tuple(sl)
+
+
 def interp_stag(m, dim, method):
-    """Value of m at the half points along `dim` (1-based dim as in Julia)."""
-    ax = dim - 1
-    lo = np.take(m, range(0, m.shape[ax] - 1), axis=ax)
-    hi = np.take(m, range(1, m.shape[ax]), axis=ax)
+    """Value of m at the half points along `dim` (an axis or a list of axes,
+    1-based as in Julia): mean of the 2^k cell corners (linear B-spline at
+    the half points), or its harmonic counterpart (interpolations.jl:4-43)."""
+    dims = _dims(dim)
+    m = np.asarray(m)
     if method == ARITHMETIC:
-        return (lo + hi) / 2
-    return 1 / ((1 / lo + 1 / hi) / 2)
+        return sum(m[sl] for sl in _corners(m.shape, dims)) / 2 ** len(dims)
+    return 1 / (sum(1 / m[sl] for sl in _corners(m.shape, dims)) / 2 ** len(dims))
 
 
 def back_interp(method, m, g_stag, dim):
     """Adjoint of interp_stag (interpolations.jl:15-28, ∂f∂m :45-47)."""
-    ax = dim - 1
-    n = m.shape[ax]
+    dims = _dims(dim)
+    m = np.asarray(m)
     res = np.zeros(m.shape, dtype=np.result_type(m, g_stag))
-    lo = np.take(m, range(0, n - 1), axis=ax)
-    hi = np.take(m, range(1, n), axis=ax)
-    if method == ARITHMETIC:
-        dlo = dhi = np.full(g_stag.shape, 0.5, dtype=res.dtype)
-    else:
-        itp = interp_stag(m, dim, method)
-        dlo = itp ** 2 / lo ** 2 / 2
-        dhi = itp ** 2 / hi ** 2 / 2
-    sl_lo = [slice(None)] * m.ndim
-    sl_hi = [slice(None)] * m.ndim
-    sl_lo[ax] = slice(0, n - 1)
-    sl_hi[ax] = slice(1, n)
-    res[tuple(sl_lo)] += g_stag * dlo
-    res[tuple(sl_hi)] += g_stag * dhi
+    k = 2 ** len(dims)
+    itp = None if method == ARITHMETIC else interp_stag(m, dims, method)
+    for sl in _corners(m.shape, dims):
+        w = (1.0 / k) if itp is None else itp ** 2 / m[sl] ** 2 / k
+        res[sl] += g_stag * w
     return res
 
 
@@ -217,7 +226,10 @@ class _AcousticCPMLWaveSimulation:
         ri = find_nearest_grid_points(shot.recs.positions, self.spacing)
         nt_s = self.shot_nt(shot)
         stf = self.scale_srctf(np.asarray(shot.srcs.tf[:nt_s], dtype=T), si)
-        return {"nt": nt_s, "src_idx": si, "rec_idx": ri, "srctf": stf, "cpml": cp}
+        return {"nt": nt_s, "src_idx": si, "rec_idx": ri, "srctf": stf, "cpml": cp, "tshape": (nt_s, ri.shape[0])}
+
+    def upload_shots(self, plan, descs):
+        plan.set_shots(descs)
 
     def batches(self, nshots):
         b = self.runparams.max_batch or nshots
@@ -337,12 +349,141 @@ class AcousticVDStaggeredCPMLWaveSimulation(_AcousticCPMLWaveSimulation):
         return {"vp": tot_vp, "rho": tot_rho}
 
 
+class ElasticIsoCPMLWaveSimulation(_AcousticCPMLWaveSimulation):
+    """ElasticIsoCPMLWaveSimulation 2D, src/models/elastic/ela_models.jl:177-436
+    (sincinterp = true as the reference's default, :208)."""
+    physics = _native.SWX_ELASTIC_ISO
+    cfl_factor = 7.0 / 6.0
+
+    def __init__(self, params, matprop, cpmlparams, *, runparams, gradparams=None, gradient=False,
+                 sincinterp: bool = True):
+        assert len(params.gridsize) == 2, "Only elastic 2D is currently implemented."
+        super().__init__(params, matprop, cpmlparams, runparams=runparams, gradparams=gradparams, gradient=gradient)
+        self.sincinterp = sincinterp
+
+    @staticmethod
+    def _vp(matprop):
+        return np.sqrt((np.asarray(matprop.lam, np.float64) + 2.0 * np.asarray(matprop.mu, np.float64))
+                       / np.asarray(matprop.rho, np.float64))
+
+    def check_matprop(self, matprop):
+        """check_matprop, ela_models.jl:93-114."""
+        lam, mu, rho = (np.asarray(a) for a in (matprop.lam, matprop.mu, matprop.rho))
+        assert np.all(lam >= 0), "Lamè coefficient λ must be positive!"
+        assert np.all(mu >= 0), "Lamè coefficient μ must be positive!"
+        assert np.all(rho > 0), "Density must be positive!"
+        vp = self._vp(matprop)
+        assert vp.ndim == len(self.gridsize), "Material property dimensionality must be the same as the wavesim!"
+        assert lam.shape == mu.shape == rho.shape == self.gridsize, \
+            "Material property number of grid points must be the same as the wavesim!"
+        self.check_courant_condition(vp)
+
+    def check_numerics(self, shot):
+        """check_numerics, ela_models.jl:116-143: minimum S velocity (P if
+        the medium has no shear modulus)."""
+        vs = np.sqrt(np.asarray(self.matprop.mu, np.float64) / np.asarray(self.matprop.rho, np.float64))
+        vel_min = float(np.min(vs))
+        if vel_min == 0:
+            vel_min = float(np.min(self._vp(self.matprop)))
+        fmax = shot.srcs.domfreq * 2.0
+        ppw = vel_min / (fmax * max(self.spacing))
+        minppw = self.runparams.minPPW
+        msg = (f"Not enough points per wavelength (assuming fmax = 2*domfreq)! \n [{round(ppw, 1)} instead of "
+               f">= {minppw}]\n  Grid spacing should be <= {round(vel_min / (minppw * fmax), 2)}")
+        if self.runparams.erroronPPW:
+            assert ppw >= minppw, msg
+        elif ppw < minppw:
+            warnings.warn(msg)
+
+    def model_fields(self):
+        """precomp_elaprop!, ela_models.jl:156-173."""
+        T = self.dtype
+        mp = self.matprop
+        rho = np.asarray(mp.rho, dtype=T)
+        mu = np.asarray(mp.mu, dtype=T)
+        return [np.asarray(mp.lam, dtype=T), mu,
+                interp_stag(rho, 1, mp.interp_method_rho).astype(T),
+                interp_stag(rho, 2, mp.interp_method_rho).astype(T),
+                interp_stag(mu, [1, 2], mp.interp_method_mu).astype(T)]
+
+    def shot_nt(self, shot):
+        nt_s = shot.recs.seismograms.shape[0]
+        assert 1 <= nt_s <= self.nt, "number of seismogram rows must be in 1..ntimesteps"
+        assert shot.srcs.tf.shape[0] >= nt_s, "source time functions shorter than the seismograms"
+        return nt_s
+
+    def shot_desc(self, shot):
+        """init_shot! + init_bdc! (ela_init_bc.jl:7-40) + possrcrec_scaletf
+        (ela_models.jl:6-90)."""
+        self.check_shot(shot)
+        T = self.dtype
+        vel_max = self.cpmlparams.vel_max if self.cpmlparams.vel_max is not None else float(np.max(self._vp(self.matprop)))
+        cp = init_bdc(vel_max, self.dt, self.spacing, self.cpmlparams.halo, self.cpmlparams.rcoef,
+                      self.cpmlparams.freeboundtop, shot.srcs.domfreq, T)
+        nt_s = self.shot_nt(shot)
+        sp, n = self.spacing, self.gridsize
+        mt = isinstance(shot, MomentTensorShot)
+        if self.sincinterp:
+            if mt:
+                src = [spread_positions(shot.srcs.positions, sp, n, (0.0, 0.0), True),
+                       spread_positions(shot.srcs.positions, sp, n, (sp[0] / 2, sp[1] / 2), True)]
+            else:
+                src = [spread_positions(shot.srcs.positions, sp, n, (sp[0] / 2, 0.0), False),
+                       spread_positions(shot.srcs.positions, sp, n, (0.0, sp[1] / 2), False)]
+            rec = [spread_positions(shot.recs.positions, sp, n, (sp[0] / 2, 0.0), False),
+                   spread_positions(shot.recs.positions, sp, n, (0.0, sp[1] / 2), False)]
+        else:
+            si = find_nearest_grid_points(shot.srcs.positions, sp)
+            ri = find_nearest_grid_points(shot.recs.positions, sp)
+            src = [nearest_taps(si), nearest_taps(si)]
+            rec = [nearest_taps(ri), nearest_taps(ri)]
+        stf = np.asarray(shot.srcs.tf[:nt_s], dtype=T) / T(np.prod(np.asarray(sp, dtype=T)))
+        mom = None
+        if mt:
+            mom = np.array([[m.Mxx, m.Mzz, m.Mxz] for m in shot.srcs.momtens], dtype=T)
+        nrec = shot.recs.positions.shape[0]
+        return {"nt": nt_s, "kind": _native.SWX_SRC_MOMENT_TENSOR if mt else _native.SWX_SRC_EXTERNAL_FORCE,
+                "src": src, "rec": rec, "moments": mom, "srctf": stf, "cpml": cp, "tshape": (nt_s, 2, nrec)}
+
+    def upload_shots(self, plan, descs):
+        plan.set_shots_elastic(descs)
+
+    def grad_fields_shapes(self):
+        nx, nz = self.gridsize
+        return [(nx - 1, nz), (nx, nz - 1), (nx, nz), (nx, nz), (nx - 1, nz - 1)]
+
+    def finish_gradient(self, raw, shots_in_batch, per_shot):
+        """Chain rule through the staggered averages (ela_gradient.jl:155-165)
+        and muting (:166-178).  Gradients are "rho", "lambda", "mu"."""
+        T = self.dtype
+        mp = self.matprop
+        rho = np.asarray(mp.rho, dtype=T)
+        mu = np.asarray(mp.mu, dtype=T)
+        gp = self.gradparams
+        groups = raw if per_shot else [raw]
+        tot = {k: np.zeros(self.gridsize, dtype=T) for k in ("rho", "lambda", "mu")}
+        for k, (grih, grjh, glam, gmu, gmhh) in enumerate(groups):
+            g = {"rho": (back_interp(mp.interp_method_rho, rho, grih, 1)
+                         + back_interp(mp.interp_method_rho, rho, grjh, 2)).astype(T),
+                 "lambda": glam.copy(),
+                 "mu": (gmu + back_interp(mp.interp_method_mu, mu, gmhh, [1, 2])).astype(T)}
+            if per_shot:
+                shot = shots_in_batch[k]
+                for arr in g.values():
+                    mute_around_points(arr, shot.srcs.positions, self.spacing, gp.mute_radius_src)
+                    mute_around_points(arr, shot.recs.positions, self.spacing, gp.mute_radius_rec)
+            for key in tot:
+                tot[key] += g[key]
+        return tot
+
+
 # ---------------------------------------------------------------------------
 # build / run (apis/build.jl, forward.jl, misfit.jl, gradient.jl)
 # ---------------------------------------------------------------------------
 def build_wavesim(params, matprop, *, runparams: RunParameters, gradparams: GradParameters | None = None,
-                  gradient: bool = False):
-    """build_wavesim (apis/build.jl:18-40)."""
+                  gradient: bool = False, **kwargs):
+    """build_wavesim (apis/build.jl:18-40); extra keywords go to the wave
+    simulation constructor (elastic: sincinterp)."""
     if gradparams is None and gradient:
         gradparams = GradParameters()
     elif gradparams is not None:
@@ -351,11 +492,14 @@ def build_wavesim(params, matprop, *, runparams: RunParameters, gradparams: Grad
         cls = AcousticCDCPMLWaveSimulation
     elif isinstance(matprop, VpRhoAcousticVDMaterialProperties):
         cls = AcousticVDStaggeredCPMLWaveSimulation
+    elif isinstance(matprop, ElasticIsoMaterialProperties):
+        cls = ElasticIsoCPMLWaveSimulation
     else:
         raise TypeError(f"no wave simulation for material properties {type(matprop).__name__}")
     if not isinstance(params.boundcond, CPMLBoundaryConditionParameters):
         raise NotImplementedError("the mi355x backend implements the C-PML boundary models")
-    return cls(params, matprop, params.boundcond, runparams=runparams, gradparams=gradparams, gradient=gradient)
+    return cls(params, matprop, params.boundcond, runparams=runparams, gradparams=gradparams, gradient=gradient,
+               **kwargs)
 
 
 def check_sim_consistency(wavesim, matprop, shots):
@@ -364,10 +508,16 @@ def check_sim_consistency(wavesim, matprop, shots):
     for sh in shots:
         if (type(sh.srcs), type(sh.recs)) != t0:
             raise ValueError("Types of shots are inconsistent.")
-    ok = ((isinstance(wavesim, AcousticCDCPMLWaveSimulation) and isinstance(matprop, VpAcousticCDMaterialProperties))
-          or (isinstance(wavesim, AcousticVDStaggeredCPMLWaveSimulation)
-              and isinstance(matprop, VpRhoAcousticVDMaterialProperties)))
-    if not ok or not all(isinstance(sh, ScalarShot) for sh in shots):
+    if isinstance(wavesim, ElasticIsoCPMLWaveSimulation):
+        ok = (isinstance(matprop, ElasticIsoMaterialProperties)
+              and all(isinstance(sh, (MomentTensorShot, ExternalForceShot)) for sh in shots))
+    else:
+        ok = (((isinstance(wavesim, AcousticCDCPMLWaveSimulation)
+                and isinstance(matprop, VpAcousticCDMaterialProperties))
+               or (isinstance(wavesim, AcousticVDStaggeredCPMLWaveSimulation)
+                   and isinstance(matprop, VpRhoAcousticVDMaterialProperties)))
+              and all(isinstance(sh, ScalarShot) for sh in shots))
+    if not ok:
         raise ValueError("Types of WaveSimulation, MaterialProperties and Sources/Receivers are inconsistent")
 
 
@@ -378,8 +528,8 @@ def run_swforward(wavesim, matprop, shots):
     for batch in wavesim.batches(len(shots)):
         descs = [wavesim.shot_desc(shots[s]) for s in batch]
         plan = wavesim._ensure_plan(len(batch))
-        plan.set_shots(descs)
-        traces = np.zeros(sum(d["nt"] * d["rec_idx"].shape[0] for d in descs), dtype=wavesim.dtype)
+        wavesim.upload_shots(plan, descs)
+        traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=wavesim.dtype)
         plan.forward(traces, save_state=False)
         _scatter_traces(traces, descs, [shots[s] for s in batch])
     return None
@@ -388,15 +538,15 @@ def run_swforward(wavesim, matprop, shots):
 def _scatter_traces(traces, descs, shots):
     off = 0
     for d, sh in zip(descs, shots):
-        nt, nr = d["nt"], d["rec_idx"].shape[0]
-        blk = traces[off:off + nt * nr].reshape((nt, nr), order="F")
-        sh.recs.seismograms[:, :] = blk.astype(sh.recs.seismograms.dtype)
-        off += nt * nr
+        n = int(np.prod(d["tshape"]))
+        blk = traces[off:off + n].reshape(d["tshape"], order="F")
+        sh.recs.seismograms[...] = blk.astype(sh.recs.seismograms.dtype)
+        off += n
 
 
 def swforward(params_or_wavesim, matprop, shots, *, runparams: RunParameters | None = None):
     """swforward! (forward.jl:114-156): fills shot.recs.seismograms."""
-    if isinstance(params_or_wavesim, InputParametersAcoustic):
+    if isinstance(params_or_wavesim, (InputParametersAcoustic, InputParametersElastic)):
         assert runparams is not None
         ws = build_wavesim(params_or_wavesim, matprop, runparams=runparams, gradient=False)
     else:
@@ -428,8 +578,8 @@ def run_swgradient(wavesim, matprop, shots, misfit):
         bshots = [shots[s] for s in batch]
         descs = [wavesim.shot_desc(sh) for sh in bshots]
         plan = wavesim._ensure_plan(len(batch))
-        plan.set_shots(descs)
-        traces = np.zeros(sum(d["nt"] * d["rec_idx"].shape[0] for d in descs), dtype=T)
+        wavesim.upload_shots(plan, descs)
+        traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=T)
         plan.forward(traces, save_state=True)
         _scatter_traces(traces, descs, bshots)
         adj = []
@@ -462,7 +612,7 @@ def run_swgradient(wavesim, matprop, shots, misfit):
 def swgradient(params_or_wavesim, matprop, shots, misfit, *, runparams: RunParameters | None = None,
                gradparams: GradParameters | None = None):
     """swgradient! (gradient.jl:31-88)."""
-    if isinstance(params_or_wavesim, InputParametersAcoustic):
+    if isinstance(params_or_wavesim, (InputParametersAcoustic, InputParametersElastic)):
         ws = build_wavesim(params_or_wavesim, matprop, runparams=runparams or RunParameters(),
                            gradparams=gradparams or GradParameters(), gradient=True)
     else:
@@ -472,6 +622,7 @@ def swgradient(params_or_wavesim, matprop, shots, misfit, *, runparams: RunParam
 
 __all__ = [
     "build_wavesim", "swforward", "swmisfit", "swgradient", "run_swforward", "run_swgradient",
-    "AcousticCDCPMLWaveSimulation", "AcousticVDStaggeredCPMLWaveSimulation", "L2Misfit",
+    "AcousticCDCPMLWaveSimulation", "AcousticVDStaggeredCPMLWaveSimulation", "ElasticIsoCPMLWaveSimulation",
+    "L2Misfit",
     "find_nearest_grid_points", "interp_stag", "back_interp", "mute_around_points",
 ]

@@ -2,6 +2,7 @@
 
 Reference: src/models/acoustic/acou_params.jl:9-35,
 src/models/acoustic/acou_material_properties.jl:10-81,
+src/models/elastic/ela_params.jl:8-34, ela_material_properties.jl:9-22,
 src/models/bdc_params.jl:16-27, src/models/genparameters.jl:25-78.
 
 `T` in the reference is the element type of the arrays; here it is the numpy
@@ -53,6 +54,11 @@ class InputParametersAcoustic:
     @property
     def ndim(self):
         return len(self.gridsize)
+
+
+class InputParametersElastic(InputParametersAcoustic):
+    """InputParametersElastic, src/models/elastic/ela_params.jl:8-34 (same
+    fields and checks as the acoustic parameters)."""
 
 
 class VpAcousticCDMaterialProperties:
@@ -110,3 +116,26 @@ class GradParameters:
 
     def __post_init__(self):
         assert self.check_freq > 0
+
+
+class ElasticIsoMaterialProperties:
+    """ElasticIsoMaterialProperties, src/models/elastic/ela_material_properties.jl:9-22:
+    Lame parameters lam (λ) and mu (μ), density rho (ρ), and the averaging
+    used for the staggered points (ρ arithmetic, λ harmonic, μ arithmetic by
+    default; λ is never averaged by the 2D scheme)."""
+
+    def __init__(self, lam, mu, rho, interp_method_rho: str = ARITHMETIC, interp_method_lam: str = HARMONIC,
+                 interp_method_mu: str = ARITHMETIC):
+        self.lam = np.asarray(lam)
+        self.mu = np.asarray(mu)
+        self.rho = np.asarray(rho)
+        for m in (interp_method_rho, interp_method_lam, interp_method_mu):
+            assert m in (ARITHMETIC, HARMONIC)
+        self.interp_method_rho = interp_method_rho
+        self.interp_method_lam = interp_method_lam
+        self.interp_method_mu = interp_method_mu
+
+    # the reference's field names
+    λ = property(lambda self: self.lam)
+    μ = property(lambda self: self.mu)
+    ρ = property(lambda self: self.rho)

@@ -36,15 +36,18 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    """ctypes mirrors of swx_plan_desc / swx_shot_desc / swx_plan_info have
-    the sizes the C compiler gives the header's structs."""
+    """ctypes mirrors of the header's structs have the sizes (and offsets)
+    the C compiler gives them."""
     import subprocess
     import tempfile
     from seismicwaves import _native
     src = r'''
+#include <stddef.h>
 #include <stdio.h>
 #include "swx.h"
-int main(void){printf("%zu %zu %zu\n", sizeof(swx_plan_desc), sizeof(swx_shot_desc), sizeof(swx_plan_info));return 0;}
+int main(void){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(swx_plan_desc), sizeof(swx_shot_desc),
+  sizeof(swx_plan_info), sizeof(swx_taps), sizeof(swx_elastic_shot_desc),
+  offsetof(swx_elastic_shot_desc, rec));return 0;}
 '''
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "s.c")
@@ -53,4 +56,5 @@ int main(void){printf("%zu %zu %zu\n", sizeof(swx_plan_desc), sizeof(swx_shot_de
         subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), c, "-o", exe], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     assert [int(x) for x in out] == [ctypes.sizeof(_native.PlanDesc), ctypes.sizeof(_native.ShotDesc),
-                                     ctypes.sizeof(_native.PlanInfo)]
+                                     ctypes.sizeof(_native.PlanInfo), ctypes.sizeof(_native.Taps),
+                                     ctypes.sizeof(_native.ElasticShotDesc), _native.ElasticShotDesc.rec.offset]
