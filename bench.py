@@ -222,7 +222,7 @@ class Runner:
 # ---------------------------------------------------------------------------
 # CPU baseline: oracle OpenMP restatement of the reference's threads backend
 # ---------------------------------------------------------------------------
-def cpu_baseline(wl, nshots_sample=2, nt_sample=400):
+def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
     import oracle
     oracle.build()
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
