@@ -124,8 +124,58 @@ def workload_vd2d_batched_small():
     return wl
 
 
+def workload_el2d_batched(nt=5000):
+    """Batched 2D isotropic elastic gradient w.r.t. rho, lambda, mu:
+    1500 x 600, dh = 5 m, dt = 0.4 ms, nt = 5000, free surface on top, C-PML
+    (20 cells) elsewhere.  vp: 8 seeded layers 1800-4500 m/s, vs = vp/1.8
+    with +-10% noise, Gardner density.  32 shots of 15 Hz Ricker sources at
+    seeded x along z = 10 m: 16 explosive moment tensors (Mxx = Mzz = 1e10)
+    and 16 vertical forces, run as two calls (one source kind per call, as
+    the reference requires); 200-600 receivers per shot at z = 0;
+    checkpoint every 50 steps.  The starting model is the true one smoothed
+    with sigma = 100 m."""
+    import seismicwaves as sw
+    from seismicwaves import synthetic as syn
+    nx, nz = 1500, 600
+    h = (5.0, 5.0)
+    dt, f0, halo, nshots = 4e-4, 15.0, 20, 32
+    rng = np.random.default_rng(4)
+    vp = syn.layered(nx, nz, h, rng, vmin=1800.0, vmax=4500.0)
+    vs = vp / 1.8 * (1 + 0.1 * rng.uniform(-1, 1, vp.shape))
+    rho = syn.gardner(vp, rng)
+    mu = rho * vs ** 2
+    lam = rho * vp ** 2 - 2 * mu
+    sm = lambda m: syn.gaussian_smooth(m, 100.0 / h[0])  # noqa: E731
+    f32 = np.float32
+    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0).astype(f32)
+    xs = rng.uniform(halo * h[0], (nx - 1 - halo) * h[0], nshots)
+    nrecs = rng.integers(200, 601, nshots)
+    groups = [[], []]
+    for k, (x, nr) in enumerate(zip(xs, nrecs)):
+        rec = syn.surface_receivers(int(nr), nx, h, halo, 0.0)
+        src = np.array([[x, 10.0]])
+        if k < nshots // 2:
+            srcs = sw.MomentTensorSources(src, tf[:, None], [sw.MomentTensor2D(1e10, 1e10, 0.0)], f0)
+            groups[0].append(sw.MomentTensorShot(srcs, sw.VectorReceivers(rec, nt, dtype=f32)))
+        else:
+            stf = np.zeros((nt, 2, 1), dtype=f32)
+            stf[:, 1, 0] = 1e10 * tf
+            groups[1].append(sw.ExternalForceShot(sw.ExternalForceSources(src, stf, f0),
+                                                  sw.VectorReceivers(rec, nt, dtype=f32)))
+    params = sw.InputParametersElastic(nt, dt, (nx, nz), h, sw.CPMLBoundaryConditionParameters(halo=halo))
+    mpt = sw.ElasticIsoMaterialProperties(lam.astype(f32), mu.astype(f32), rho.astype(f32))
+    mp0 = sw.ElasticIsoMaterialProperties(sm(lam).astype(f32), sm(mu).astype(f32), sm(rho).astype(f32))
+    return dict(name=f"elastic_2d_gradient_1500x600_32shots_nt{nt}", params=params, physics="el2d",
+                mp_true=mpt, mp0=mp0, shots=groups[0] + groups[1], groups=groups, check_freq=50,
+                run={"erroronPPW": False},
+                config={"nx": nx, "nz": nz, "dh_m": 5.0, "dt_s": dt, "f0_hz": f0, "shots": nshots, "nt": nt,
+                        "receivers": f"{int(nrecs.min())}-{int(nrecs.max())}", "check_freq": 50, "halo": halo,
+                        "sources": "16 explosive moment tensors + 16 vertical forces (two calls)"})
+
+
 WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_cd2d_grad_1000,
-             "vd2d_batched_small": workload_vd2d_batched_small}
+             "vd2d_batched_small": workload_vd2d_batched_small, "el2d_batched": workload_el2d_batched,
+             "el2d_batched_small": lambda: workload_el2d_batched(nt=500)}
 
 
 # ---------------------------------------------------------------------------
@@ -145,9 +195,26 @@ def alg_bytes(physics, nx, nz, halo, e, nts, S):
     strip = _strip_cells(nx, nz, halo) * 2 * e
     fwd_b = fwd_n = adj_b = adj_n = 0
     for nt in nts:
-        kl = (nt - 1) // S
-        last = nt - kl * S              # steps of the last segment: stored during the forward
-        rep = nt - last                 # steps replayed from checkpoints
+        if physics == "vd2d":
+            # the final nt - floor(nt/S)*S steps are stored during the
+            # forward; every earlier segment replays S-1 steps
+            last = nt - (nt // S) * S
+            rep = (nt // S) * (S - 1)
+        else:
+            kl = (nt - 1) // S
+            last = nt - kl * S              # steps of the last segment: stored during the forward
+            rep = nt - last                 # steps replayed from checkpoints
+        if physics == "el2d":
+            # forward: stress r ux, uz, lambda, mu, mu_ihjh, w 3 sigma (8e) + displacement
+            # r 3 sigma, uold, ucur (2 each), dt2/rho x2, w 2 u (11e); + w u history (2e).
+            # Adjoint: the same 19e + correlation r 3 history levels x 2 + 5 grad RMW (16e)
+            last = nt - (nt // S) * S
+            rep = (nt // S) * (S - 1)
+            fwd_b += nt * 19 * e * N + last * 2 * e * N + rep * 21 * e * N
+            fwd_n += 2 * (nt + rep)
+            adj_b += nt * 35 * e * N
+            adj_n += 2 * nt
+            continue
         if physics == "vd2d":
             # forward: r p, vx, vz, m0, m1x, m1z; w p, vx, vz (9e); + w p-history (10e)
             fwd_b += (nt - last) * 9 * e * N + last * 10 * e * N + nt * strip
@@ -168,6 +235,33 @@ def alg_bytes(physics, nx, nz, halo, e, nts, S):
     return {0: (fwd_b, fwd_n), 1: (adj_b, adj_n)}
 
 
+def kernel_names(physics, variant):
+    """Names of the forward and adjoint step kernels the plan's autotuner
+    picked (variant codes of swx_plan_info.kernel_variant)."""
+    if physics == "vd2d":
+        def name(kind, v):
+            return f"vd2d_{kind}_march<float, {v - 100}>" if v > 100 else f"vd2d_{kind}_kernel<float, {v}>"
+        return name("fwd", variant[0]), name("adj", variant[1])
+    if physics == "el2d":
+        return ("el_stress_kernel<float> + el_disp_kernel<float> (forward)",
+                "el_stress_kernel<float> + el_disp_kernel<float> (adjoint)")
+    return "cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)"
+
+
+def traffic_from_profiles(workload, kernel):
+    """HBM bytes per dispatch of `kernel` on `workload` as recorded in the
+    rocprof summaries under profiles/ (a line `TRAFFIC <workload> <kernel>
+    <bytes>`), or None when no summary covers it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "*.md"))):
+        for line in open(path):
+            parts = line.strip().strip("`").split()
+            if len(parts) >= 4 and parts[0] == "TRAFFIC" and parts[1] == workload and \
+                    " ".join(parts[2:-1]) == kernel:
+                return int(float(parts[-1]))
+    return None
+
+
 # ---------------------------------------------------------------------------
 # Gradient driver on an already-populated plan
 # ---------------------------------------------------------------------------
@@ -182,8 +276,8 @@ class Runner:
         self.descs = [self.ws.shot_desc(sh) for sh in shots]
         self.misfits = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in shots]
         self.plan = self.ws._ensure_plan(len(self.descs))
-        self.plan.set_shots(self.descs)
-        self.ntr = sum(d["nt"] * d["rec_idx"].shape[0] for d in self.descs)
+        self.ws.upload_shots(self.plan, self.descs)
+        self.ntr = sum(int(np.prod(d["tshape"])) for d in self.descs)
         self.traces = np.zeros(self.ntr, dtype=np.float32)
         size = sum(int(np.prod(s)) for s in self.ws.grad_fields_shapes())
         self.grad = np.zeros(size, dtype=np.float32)
@@ -192,10 +286,10 @@ class Runner:
         self.plan.forward(self.traces, True)
         off, adj = 0, []
         for d, m in zip(self.descs, self.misfits):
-            nt, nr = d["nt"], d["rec_idx"].shape[0]
-            syn_ = self.traces[off:off + nt * nr].reshape((nt, nr), order="F")
+            n = int(np.prod(d["tshape"]))
+            syn_ = self.traces[off:off + n].reshape(d["tshape"], order="F")
             adj.append(np.asfortranarray(-(syn_.astype(np.float64) - m.observed), dtype=np.float32).ravel(order="F"))
-            off += nt * nr
+            off += n
         self.plan.adjoint(np.concatenate(adj), self.grad, False)
 
     def timed_kernels(self):
@@ -234,6 +328,8 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
     halo, rcoef, ft = p.boundcond.halo, p.boundcond.rcoef, p.boundcond.freeboundtop
     T = np.float32
     cf = wl["check_freq"]
+    if wl["physics"] == "el2d":
+        return cpu_baseline_elastic(wl, cores, nt_sample=300)
     vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
     shots = wl["shots"][:nshots_sample]
     rng = np.random.default_rng(0)
@@ -263,6 +359,35 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
                       f"reference's threads backend (not the Julia binary); {el:.2f} s"}
 
 
+def cpu_baseline_elastic(wl, cores, nt_sample=300):
+    """The oracle's FP32 OpenMP elastic gradient on one moment-tensor shot of
+    the workload, nt_sample steps."""
+    import oracle
+    p = wl["params"]
+    h, dt = p.gridspacing, float(p.dt)
+    bc = p.boundcond
+    T = np.float32
+    mp = wl["mp0"]
+    sh = wl["groups"][0][0]
+    m = sh.srcs.momtens[0]
+    osh = dict(src_pos=sh.srcs.positions, srctf=sh.srcs.tf[:nt_sample], moments=np.array([[m.Mxx, m.Mzz, m.Mxz]]),
+               f0=float(sh.srcs.domfreq), rec_pos=sh.recs.positions, nt=nt_sample)
+    fields, cpml, kind, fams, stf, mom = oracle._el_prepare(mp.lam, mp.mu, mp.rho, h, dt, bc.halo, bc.rcoef,
+                                                           bc.freeboundtop, osh, True, T)
+    nrec = sh.recs.positions.shape[0]
+    adj = (np.random.default_rng(0).standard_normal((nt_sample, 2, nrec)) * 1e-3).astype(T)
+    t0 = time.perf_counter()
+    oracle.el2d_gradient(fields, h, dt, bc.halo, bc.freeboundtop, cpml, nt_sample, kind, fams, mom, stf, adj,
+                         wl["check_freq"], T)
+    el = time.perf_counter() - t0
+    nx, nz = p.gridsize
+    return {"value": round(2.0 * nx * nz * nt_sample / el / 1e9, 4), "unit": "GCell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"FP32 elastic gradient of the same model: 1 moment-tensor shot x nt={nt_sample} "
+                      f"(check_freq={wl['check_freq']}), OpenMP threads={cores}; C restatement of the "
+                      f"reference's threads backend (not the Julia binary); {el:.2f} s"}
+
+
 # ---------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
@@ -288,21 +413,34 @@ def main():
     # observed data from the true model (outside the timed region)
     sw.swforward(p, wl["mp_true"], wl["shots"], runparams=sw.RunParameters(**wl["run"]))
     log("observed data ready")
-    run = Runner(wl, wl["shots"])
-    nts = [d["nt"] for d in run.descs]
+    # one plan per call: the elastic workload runs its two source kinds as
+    # two calls, as the reference requires
+    runs = [Runner(wl, g) for g in wl.get("groups", [wl["shots"]])]
+    nts = [d["nt"] for r in runs for d in r.descs]
     for _ in range(args.warmup):
-        run.step()
+        for r in runs:
+            r.step()
     log("warmup done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run.step()
+        for r in runs:
+            r.step()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     log(f"timed {args.steps} steps: {ms_step:.1f} ms/step")
-    kms, kn = run.timed_kernels()
-    info = run.plan.info()
+    kms, kn = [0.0, 0.0], [0, 0]
+    dev_ms = 0.0
+    for r in runs:
+        m, n = r.timed_kernels()
+        i = r.plan.info()
+        dev_ms += i.ms_forward + i.ms_adjoint
+        for k in range(2):
+            kms[k] += m[k]
+            kn[k] += n[k]
+    info = runs[0].plan.info()
     S = info.segment
-    run.close()
+    for r in runs:
+        r.close()
     nx, nz = p.gridsize
     cells = nx * nz
     value = 2.0 * cells * sum(nts) / (ms_step / 1e3) / 1e9
@@ -316,8 +454,7 @@ def main():
                   "alg_bytes_per_launch": int(per_launch),
                   "achieved_GBs": round(per_launch / (avg_ms / 1e3) / 1e9, 1), "total_ms": round(kms[k], 1)}
     dom = max(fam, key=lambda k: fam[k]["total_ms"])
-    kern = {"vd2d": ("vd2d_fwd_kernel<float>", "vd2d_adj_kernel<float>"),
-            "cd2d": ("cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)")}[wl["physics"]]
+    kern = kernel_names(wl["physics"], list(info.kernel_variant))
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "GCell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -325,15 +462,16 @@ def main():
         "data": "synthetic (seeded layered model, Ricker sources, observed data from a forward run)",
         "config": {"workload": wl["name"], **wl["config"], "segment_used": S},
         "roofline": {"bound": "hbm", "achieved": fam[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fam[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(fam[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(wl["name"], kern[dom]),
                      "kernel": kern[dom], "avg_launch_us": fam[dom]["avg_launch_us"],
                      "alg_bytes_per_launch": fam[dom]["alg_bytes_per_launch"]},
         "kernels": {kern[k]: fam[k] for k in fam},
-        "device_ms_per_step": round(info.ms_forward + info.ms_adjoint, 1),
+        "device_ms_per_step": round(dev_ms, 1),
     }
     if not args.no_one_shot and len(wl["shots"]) > 1:
         # the same kernels on a plan holding one shot, shots run in sequence
-        sample = wl["shots"][:args.one_shot_sample]
+        sample = wl.get("groups", [wl["shots"]])[0][:args.one_shot_sample]
         one = Runner(wl, sample[:1])
         one.step()    # warm-up
         descs = [one.ws.shot_desc(sh) for sh in sample]
@@ -341,8 +479,8 @@ def main():
         for d, sh in zip(descs, sample):
             one.descs = [d]
             one.misfits = [sw.L2Misfit(sh.recs.seismograms.copy())]
-            one.plan.set_shots([d])
-            one.ntr = d["nt"] * d["rec_idx"].shape[0]
+            one.ws.upload_shots(one.plan, [d])
+            one.ntr = int(np.prod(d["tshape"]))
             one.traces = np.zeros(one.ntr, dtype=np.float32)
             one.step()
         el = time.perf_counter() - t0

@@ -154,6 +154,10 @@ typedef struct {
    * swx_adjoint call. */
   double ms_kernel[2];
   int64_t n_kernel[2];
+  /* Kernel variant the plan's autotuner chose for each family (VD: tile
+   * height 8/16/32 of the 2-D tiled kernels, 102/104 for the z-marching
+   * kernels on 128/256-column strips; 0 where there is one variant). */
+  int32_t kernel_variant[2];
 } swx_plan_info;
 
 /* Version of this ABI (SWX_ABI_VERSION). No reference counterpart. */

@@ -55,6 +55,7 @@ template <typename T>
 struct Vd2dArgs {
   int nx, nz, h;
   int nbx, nbz;                  // tiles per shot along x and z
+  int ch;                        // rows per block of the z-marching kernel
   int64_t N;
   T idx, idz;
   const T *m0, *m1x, *m1z;
@@ -84,10 +85,13 @@ __device__ __forceinline__ T d4(T a, T b, T c, T d) {
   return c1 * a + c2 * b + (-c2) * c + (-c1) * d;
 }
 
-// base of the segment a step belongs to, or -1 if the step stores nothing
+// Base of the history segment a step stores into, or -1.  Forward (smode
+// 1): the final segment of the shot, from its last checkpoint floor(nt/S)*S
+// on (the buffer of LinearCheckpointer, checkpointers.jl:44-56); replay
+// (smode 2): the segment being replayed.
 __device__ __forceinline__ int store_base(int smode, int it, int nts, int S, int cbase) {
   if (smode == 1) {
-    const int seg0 = ((nts - 1) / S) * S;
+    const int seg0 = (nts / S) * S;
     return it > seg0 ? seg0 : -1;
   }
   if (smode == 2) return cbase;
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
       a.p_n[so + c] = pv;
       if (sb >= 0) {
         a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
-        if (a.it == sb + 1) a.pbuf[so + c] = pold;
+        if (a.smode == 1 && a.it == sb + 1) a.pbuf[so + c] = pold;
       }
       if (any_rec) {
         // record_receivers! (acoustic2D_VD_xPU.jl:9-15)
@@ -276,6 +280,176 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
         d = d + ps;
       }
       a.vz_n[so + c] = s_vz[lz + 3][tx + 1] - rm1z[r] * d;
+    }
+  }
+}
+
+// Forward / replay step marching down z.  A block owns a strip of 64*W
+// columns and a chunk of rows [Z0, Z1) and walks it row by row: the new
+// pressure of row j (and of the one-left / two-right halo columns the
+// velocity update needs) is computed from LDS rings of old vx and vz rows,
+// then the velocities of row j-2, whose pressure rows j-3..j are now in the
+// ring of new pressures.  Every input row is read once per block (only the
+// three rows above and below a chunk are read twice), and the loads of row
+// j+1 are issued before the work on row j.  Ring depths are one row more
+// than a step reads, so two barriers per row suffice.
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
+  constexpr int TXM = 64 * W;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int I0 = bx * TXM, Z0 = bz * a.ch, Z1 = min(Z0 + a.ch, nz);
+  const int tx = threadIdx.x;
+  const int64_t so = (int64_t)s * a.N;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+
+  __shared__ T rvx[4][TXM + 6];   // vx^{it-1} rows (slot j&3), x in [I0-3, I0+TXM+3)
+  __shared__ T rvz[5][TXM + 3];   // vz^{it-1} rows (slot j%5), x in [I0-1, I0+TXM+2)
+  __shared__ T rp[5][TXM + 3];    // p^{it} rows (slot j%5),    x in [I0-1, I0+TXM+2)
+  __shared__ int s_pts[2];
+  if (tx == 0) {
+    const int *sr = a.pt.srow + (int64_t)s * (a.pt.nrows + 1);
+    const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+    s_pts[0] = sr[max(Z0 - 1, 0)] < sr[min(Z1 + 2, nz)];
+    s_pts[1] = a.traces != nullptr && rr[Z0] < rr[Z1];
+  }
+
+  // column duties: own column gi; halo columns of vx (6), of vz / p / m0 (3)
+  const int gi = I0 + tx;
+  const int hvx_lx = tx < 3 ? tx : TXM + tx;                 // tx < 6
+  const int hp_lx = tx == 0 ? 0 : TXM + tx;                   // tx < 3
+  const int hvx_x = I0 - 3 + hvx_lx, hp_x = I0 - 1 + hp_lx;
+  auto ld_vx = [&](int x, int j) -> T {
+    return (x >= 0 && x <= nx - 2 && j >= 0 && j <= nz - 1) ? a.vx_o[so + (int64_t)j * nx + x] : T(0);
+  };
+  auto ld_vz = [&](int x, int j) -> T {
+    return (x >= 0 && x <= nx - 1 && j >= 0 && j <= nz - 2) ? a.vz_o[so + (int64_t)j * nx + x] : T(0);
+  };
+  auto ld_f = [&](const T *f, int64_t off, int x, int j) -> T {
+    return (x >= 0 && x <= nx - 1 && j >= 0 && j <= nz - 1) ? f[off + (int64_t)j * nx + x] : T(0);
+  };
+  auto rz5 = [](int j) { return (j + 10) % 5; };
+
+  // vz rows Z0-3 .. Z0-1 straight into the ring
+  for (int j = Z0 - 3; j <= Z0 - 1; j++) {
+    rvz[rz5(j)][tx + 1] = ld_vz(gi, j);
+    if (tx < 3) rvz[rz5(j)][hp_lx] = ld_vz(hp_x, j);
+  }
+  // registers for row j: vx(j), vz(j+1), p(j), m0(j) with halos; m1x, m1z(j-2)
+  T n_vx, n_vxh = 0, n_vz, n_vzh = 0, n_p, n_ph = 0, n_m0, n_m0h = 0, n_m1x, n_m1z;
+  auto issue = [&](int j) {
+    n_vx = ld_vx(gi, j);
+    n_vz = ld_vz(gi, j + 1);
+    n_p = ld_f(a.p_o, so, gi, j);
+    n_m0 = ld_f(a.m0, 0, gi, j);
+    n_m1x = (gi <= nx - 2 && j - 2 >= 0 && j - 2 <= nz - 1) ? a.m1x[(int64_t)(j - 2) * nx + gi] : T(0);
+    n_m1z = (gi <= nx - 1 && j - 2 >= 0 && j - 2 <= nz - 2) ? a.m1z[(int64_t)(j - 2) * nx + gi] : T(0);
+    if (tx < 6) n_vxh = ld_vx(hvx_x, j);
+    if (tx < 3) {
+      n_vzh = ld_vz(hp_x, j + 1);
+      n_ph = ld_f(a.p_o, so, hp_x, j);
+      n_m0h = ld_f(a.m0, 0, hp_x, j);
+    }
+  };
+  issue(Z0 - 1);
+  __syncthreads();
+  const bool any_src = s_pts[0] != 0, any_rec = s_pts[1] != 0;
+  const int sb = store_base(a.smode, a.it, nts, a.S, a.cbase);
+
+  // new pressure at row j, LDS column lx (x = I0-1+lx)
+  auto new_p = [&](int lx, int j, T pold, T m0v, bool owned) -> T {
+    const int x = I0 - 1 + lx;
+    if (x < 0 || x >= nx || j < 0 || j >= nz) return T(0);
+    const int64_t c = (int64_t)j * nx + x;
+    T pv = pold;
+    if (x >= 1 && x <= nx - 2 && j >= 1 && j <= nz - 2) {
+      const T *vr = rvx[(j + 4) & 3];
+      T dvx = d4(vr[lx], vr[lx + 1], vr[lx + 2], vr[lx + 3]) * a.idx;
+      if (x <= h || x >= nx - h - 1) {
+        const int k = x <= h ? x : x - nx + 2 * h + 2;
+        const int64_t qx = (int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + k;
+        const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+        if (owned) a.xix_n[qx] = xi;
+        dvx = dvx + xi;
+      }
+      T dvz = d4(rvz[rz5(j - 2)][lx], rvz[rz5(j - 1)][lx], rvz[rz5(j)][lx], rvz[rz5(j + 1)][lx]) * a.idz;
+      if (j <= h || j >= nz - h - 1) {
+        const int k = j <= h ? j : j - nz + 2 * h + 2;
+        const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + x;
+        const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+        if (owned) a.xiz_n[qz] = xi;
+        dvz = dvz + xi;
+      }
+      pv = pv - m0v * (dvx + dvz);
+    }
+    const int u = any_src ? find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c) : -1;
+    if (u >= 0) {
+      const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
+      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pv += tf[(int64_t)nts * a.pt.usrc_list[k]];
+    }
+    if (owned) {
+      a.p_n[so + c] = pv;
+      if (sb >= 0) {
+        a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
+        if (a.smode == 1 && a.it == sb + 1) a.pbuf[so + c] = pold;
+      }
+      if (any_rec) {
+        const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
+        if (r >= 0) {
+          T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
+          for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = pv;
+        }
+      }
+    }
+    return pv;
+  };
+
+  for (int j = Z0 - 1; j <= Z1 + 1; j++) {
+    // commit row j's loads to the rings and registers
+    rvx[(j + 4) & 3][tx + 3] = n_vx;
+    rvz[rz5(j + 1)][tx + 1] = n_vz;
+    if (tx < 6) rvx[(j + 4) & 3][hvx_lx] = n_vxh;
+    if (tx < 3) rvz[rz5(j + 1)][hp_lx] = n_vzh;
+    const T c_p = n_p, c_ph = n_ph, c_m0 = n_m0, c_m0h = n_m0h, c_m1x = n_m1x, c_m1z = n_m1z;
+    if (j + 1 <= Z1 + 1) issue(j + 1);
+    __syncthreads();
+    // pressure of row j (owned rows write HBM)
+    const bool own_row = j >= Z0 && j < Z1;
+    rp[rz5(j)][tx + 1] = new_p(tx + 1, j, c_p, c_m0, own_row && gi < nx);
+    if (tx < 3) rp[rz5(j)][hp_lx] = new_p(hp_lx, j, c_ph, c_m0h, false);
+    __syncthreads();
+    // velocities of row jv = j-2 (update_vx_CPML! / update_vy_CPML!, :39-75)
+    const int jv = j - 2;
+    if (jv >= Z0 && jv < Z1 && gi < nx) {
+      const int64_t c = (int64_t)jv * nx + gi;
+      if (gi <= nx - 2) {
+        const T *pr = rp[rz5(jv)];
+        T d = d4(pr[tx], pr[tx + 1], pr[tx + 2], pr[tx + 3]) * a.idx;
+        if (gi <= h - 1 || gi >= nx - h - 1) {
+          const int k = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
+          const int64_t q = (int64_t)s * a.spx + (int64_t)jv * (2 * h) + k;
+          const T ps = cx.bh[k] * a.psx_o[q] + cx.ah[k] * d;
+          a.psx_n[q] = ps;
+          d = d + ps;
+        }
+        a.vx_n[so + c] = rvx[(jv + 4) & 3][tx + 3] - c_m1x * d;
+      }
+      if (jv <= nz - 2) {
+        T d = d4(rp[rz5(jv - 1)][tx + 1], rp[rz5(jv)][tx + 1], rp[rz5(jv + 1)][tx + 1], rp[rz5(jv + 2)][tx + 1]) *
+              a.idz;
+        if (jv <= h - 1 || jv >= nz - h - 1) {
+          const int k = jv <= h - 1 ? jv : jv + 1 - nz + 2 * h;
+          const int64_t q = (int64_t)s * a.spz + (int64_t)k * nx + gi;
+          const T ps = cz.bh[k] * a.psz_o[q] + cz.ah[k] * d;
+          a.psz_n[q] = ps;
+          d = d + ps;
+        }
+        a.vz_n[so + c] = rvz[rz5(jv)][tx + 1] - c_m1z * d;
+      }
     }
   }
 }
@@ -479,6 +653,188 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
   }
 }
 
+// Adjoint step marching down z (same strip/chunk decomposition as
+// vd2d_fwd_march).  At row j: the new adjoint velocities of row j (vx also
+// on the two-left / one-right halo columns) from the ring of old adjoint
+// pressure rows j-1..j+2, then the pressure of row j-1 from the rings of new
+// velocities, residual injection and the correlation with the replayed
+// p^{it} rows j-2..j+1 and p^{it-1}.  Rings are deep enough for two barriers
+// per row.
+template <typename T, int W>
+__global__ __launch_bounds__(64 * W) void vd2d_adj_march(Vd2dArgs<T> a) {
+  constexpr int TXM = 64 * W;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts) return;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int I0 = bx * TXM, Z0 = bz * a.ch, Z1 = min(Z0 + a.ch, nz);
+  const int tx = threadIdx.x;
+  const int64_t so = (int64_t)s * a.N;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  const int slot = a.it - a.cbase;
+  const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
+  const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
+
+  __shared__ T rpo[5][TXM + 6];   // old adjoint p rows (slot j%5), x in [I0-3, I0+TXM+3)
+  __shared__ T rpc[6][TXM + 3];   // p^{it} rows (slot j%6),        x in [I0-1, I0+TXM+2)
+  __shared__ T rvx[3][TXM + 3];   // new adjoint vx rows (slot j%3), x in [I0-2, I0+TXM+1)
+  __shared__ T rvz[5][TXM];       // new adjoint vz rows (slot j%5), x in [I0, I0+TXM)
+  __shared__ int s_rec;
+  if (tx == 0) {
+    const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+    s_rec = rr[Z0] < rr[Z1];
+  }
+  const int gi = I0 + tx;
+  const int ho_lx = tx < 3 ? tx : TXM + tx;   // tx < 6: old-p halo columns (x = I0-3+ho_lx)
+  const int hc_lx = tx == 0 ? 0 : TXM + tx;   // tx < 3: p^{it} halo columns (x = I0-1+hc_lx)
+  const int hv_lx = tx < 2 ? tx : TXM + 2;    // tx < 3: vx halo columns (x = I0-2+hv_lx)
+  const int ho_x = I0 - 3 + ho_lx, hc_x = I0 - 1 + hc_lx, hv_x = I0 - 2 + hv_lx;
+  auto r5 = [](int j) { return (j + 10) % 5; };
+  auto r6 = [](int j) { return (j + 12) % 6; };
+  auto r3 = [](int j) { return (j + 6) % 3; };
+  auto ld_p = [&](const T *f, int x, int j) -> T {
+    return (x >= 0 && x <= nx - 1 && j >= 0 && j <= nz - 1) ? f[(int64_t)j * nx + x] : T(0);
+  };
+  auto ld_vx = [&](const T *f, int x, int j) -> T {
+    return (x >= 0 && x <= nx - 2 && j >= 0 && j <= nz - 1) ? f[(int64_t)j * nx + x] : T(0);
+  };
+  auto ld_vz = [&](const T *f, int x, int j) -> T {
+    return (x >= 0 && x <= nx - 1 && j >= 0 && j <= nz - 2) ? f[(int64_t)j * nx + x] : T(0);
+  };
+  const T *pold = a.p_o + so;
+
+  // preload old p rows Z0-3 .. Z0-1 and p^{it} row Z0-1
+  for (int j = Z0 - 3; j <= Z0 - 1; j++) {
+    rpo[r5(j)][tx + 3] = ld_p(pold, gi, j);
+    if (tx < 6) rpo[r5(j)][ho_lx] = ld_p(pold, ho_x, j);
+  }
+  rpc[r6(Z0 - 1)][tx + 1] = ld_p(pc_, gi, Z0 - 1);
+  if (tx < 3) rpc[r6(Z0 - 1)][hc_lx] = ld_p(pc_, hc_x, Z0 - 1);
+
+  // prefetched per row j: old p / p^{it} row j+2; vx, vz, m1x, m1z row j
+  // (vx, m1x also on the halo columns); m0, p^{it-1}, grads of row j-1
+  T n_po, n_poh = 0, n_pc, n_pch = 0, n_vx, n_vxh = 0, n_m1x, n_m1xh = 0, n_vz, n_m1z;
+  T n_m0, n_pm, n_g0, n_g1x, n_g1z;
+  auto issue = [&](int j) {
+    n_po = ld_p(pold, gi, j + 2);
+    n_pc = ld_p(pc_, gi, j + 2);
+    n_vx = ld_vx(a.vx_o + so, gi, j);
+    n_m1x = ld_vx(a.m1x, gi, j);
+    n_vz = ld_vz(a.vz_o + so, gi, j);
+    n_m1z = ld_vz(a.m1z, gi, j);
+    n_m0 = ld_p(a.m0, gi, j - 1);
+    n_pm = ld_p(po_, gi, j - 1);
+    n_g0 = ld_p(a.g0 + so, gi, j - 1);
+    n_g1x = ld_p(a.g1x + so, gi, j - 1);
+    n_g1z = ld_p(a.g1z + so, gi, j - 1);
+    if (tx < 6) n_poh = ld_p(pold, ho_x, j + 2);
+    if (tx < 3) {
+      n_pch = ld_p(pc_, hc_x, j + 2);
+      n_vxh = ld_vx(a.vx_o + so, hv_x, j);
+      n_m1xh = ld_vx(a.m1x, hv_x, j);
+    }
+  };
+  issue(Z0 - 2);
+  __syncthreads();
+
+  // new adjoint vx at row j, ring column lx (x = I0-2+lx)
+  auto new_vx = [&](int lx, int j, T vold, T m1, bool owned) -> T {
+    const int x = I0 - 2 + lx;
+    if (x < 0 || x > nx - 2 || j < 0 || j > nz - 1) return T(0);
+    const T *pr = rpo[r5(j)];
+    T d = d4(pr[lx], pr[lx + 1], pr[lx + 2], pr[lx + 3]) * a.idx;
+    if (x <= h - 1 || x >= nx - h - 1) {
+      const int k = x <= h - 1 ? x : x + 1 - nx + 2 * h;
+      const int64_t qq = (int64_t)s * a.spx + (int64_t)j * (2 * h) + k;
+      const T ps = cx.bh[k] * a.psx_o[qq] + cx.ah[k] * d;
+      if (owned) a.psx_n[qq] = ps;
+      d = d + ps;
+    }
+    const T v = vold - m1 * d;
+    if (owned) a.vx_n[so + (int64_t)j * nx + x] = v;
+    return v;
+  };
+
+  for (int j = Z0 - 2; j <= Z1; j++) {
+    rpo[r5(j + 2)][tx + 3] = n_po;
+    rpc[r6(j + 2)][tx + 1] = n_pc;
+    if (tx < 6) rpo[r5(j + 2)][ho_lx] = n_poh;
+    if (tx < 3) rpc[r6(j + 2)][hc_lx] = n_pch;
+    const T c_vx = n_vx, c_vxh = n_vxh, c_m1x = n_m1x, c_m1xh = n_m1xh, c_vz = n_vz, c_m1z = n_m1z;
+    const T c_m0 = n_m0, c_pm = n_pm, c_g0 = n_g0, c_g1x = n_g1x, c_g1z = n_g1z;
+    if (j + 1 <= Z1) issue(j + 1);
+    __syncthreads();
+    // ---- velocities of row j (adjoint_onestep_CPML!, :163-171) ----------
+    const bool own_row = j >= Z0 && j < Z1;
+    rvx[r3(j)][tx + 2] = new_vx(tx + 2, j, c_vx, c_m1x, own_row && gi < nx);
+    if (tx < 3) rvx[r3(j)][hv_lx] = new_vx(hv_lx, j, c_vxh, c_m1xh, false);
+    {
+      T v = 0;
+      if (gi <= nx - 1 && j >= 0 && j <= nz - 2) {
+        T d = d4(rpo[r5(j - 1)][tx + 3], rpo[r5(j)][tx + 3], rpo[r5(j + 1)][tx + 3], rpo[r5(j + 2)][tx + 3]) *
+              a.idz;
+        if (j <= h - 1 || j >= nz - h - 1) {
+          const int k = j <= h - 1 ? j : j + 1 - nz + 2 * h;
+          const int64_t qq = (int64_t)s * a.spz + (int64_t)k * nx + gi;
+          const T ps = cz.bh[k] * a.psz_o[qq] + cz.ah[k] * d;
+          if (own_row) a.psz_n[qq] = ps;
+          d = d + ps;
+        }
+        v = c_vz - c_m1z * d;
+        if (own_row) a.vz_n[so + (int64_t)j * nx + gi] = v;
+      }
+      rvz[r5(j)][tx] = v;
+    }
+    __syncthreads();
+    // ---- pressure of row jp = j-1, residual injection, correlation -------
+    const int jp = j - 1;
+    if (jp >= Z0 && jp < Z1 && gi < nx) {
+      const int64_t c = (int64_t)jp * nx + gi;
+      T pv = rpo[r5(jp)][tx + 3];
+      if (gi >= 1 && gi <= nx - 2 && jp >= 1 && jp <= nz - 2) {
+        const T *vr = rvx[r3(jp)];
+        T dvx = d4(vr[tx], vr[tx + 1], vr[tx + 2], vr[tx + 3]) * a.idx;
+        if (gi <= h || gi >= nx - h - 1) {
+          const int k = gi <= h ? gi : gi - nx + 2 * h + 2;
+          const int64_t qx = (int64_t)s * a.sxx + (int64_t)jp * (2 * h + 2) + k;
+          const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+          a.xix_n[qx] = xi;
+          dvx = dvx + xi;
+        }
+        T dvz = d4(rvz[r5(jp - 2)][tx], rvz[r5(jp - 1)][tx], rvz[r5(jp)][tx], rvz[r5(jp + 1)][tx]) * a.idz;
+        if (jp <= h || jp >= nz - h - 1) {
+          const int k = jp <= h ? jp : jp - nz + 2 * h + 2;
+          const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi;
+          const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+          a.xiz_n[qz] = xi;
+          dvz = dvz + xi;
+        }
+        pv = pv - c_m0 * (dvx + dvz);
+      }
+      const int u = s_rec ? find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, jp, c) : -1;
+      if (u >= 0) {
+        const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
+        for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++)
+          pv += rs[(int64_t)nts * a.pt.urec_list[k]] * c_m0;
+      }
+      a.p_n[so + c] = pv;
+      const T pcv = rpc[r6(jp)][tx + 1];
+      a.g0[so + c] = c_g0 - (pv * (pcv - c_pm) * a.idt);
+      if (gi <= nx - 2) {
+        const T *cr = rpc[r6(jp)];
+        const T dpx = d4(cr[tx], pcv, cr[tx + 2], cr[tx + 3]) * a.idx;
+        a.g1x[so + c] = c_g1x + rvx[r3(jp)][tx + 2] * dpx;
+      }
+      if (jp <= nz - 2) {
+        const T dpz = d4(rpc[r6(jp - 1)][tx + 1], pcv, rpc[r6(jp + 1)][tx + 1], rpc[r6(jp + 2)][tx + 1]) * a.idz;
+        a.g1z[so + c] = c_g1z + rvz[r5(jp)][tx] * dpz;
+      }
+    }
+  }
+}
+
 template <typename T>
 class Vd2dEngine final : public Engine {
  public:
@@ -513,10 +869,13 @@ class Vd2dEngine final : public Engine {
       ck_one_ = (3 * N_ + spx_ + spz_ + sxx_ + sxz_) * ns_ * e;
       const int64_t avail = (int64_t)c_.budget - state_bytes_;
       // the replay buffer holds S+1 pressure slots
-      S_ = choose_segment(d.nt_max, d.check_freq, slot, ck_one_, avail - slot);
+      S_ = choose_segment(d.nt_max, d.check_freq, slot, ck_one_, avail - slot - 2 * ck_one_);
+      // check_freq = 1 keeps every step (no replay); otherwise a checkpoint
+      // at every multiple of S, as the reference's LinearCheckpointer
+      store_all_ = d.check_freq <= 1 && S_ == d.nt_max;
       K_ = (int)ceil_div(d.nt_max, S_);
       pbuf_.alloc((int64_t)(S_ + 1) * slot);
-      nck_ = std::max(K_ - 2, 0);
+      nck_ = store_all_ ? 0 : d.nt_max / S_;
       ck_.alloc(std::max<int64_t>(nck_ * ck_one_, 1));
     }
     zero(st_);
@@ -550,13 +909,14 @@ class Vd2dEngine final : public Engine {
     zero(st_);
     if (!tz_fwd_) {
       const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", {8, 16, Tiles<T>::fwd_max}, [&](int tz) { launch_fwd(a1, tz); });
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max}),
+                     [&](int tz) { launch_fwd(a1, tz); });
     }
     for (int it = 1; it <= ntmax_; it++) {
       fwd_step(it, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
       if (save && it % S_ == 0) {
         const int k = it / S_;
-        if (k >= 1 && k <= K_ - 2) ckpt_copy(k, it, true);
+        if (k >= 1 && k <= nck_) ckpt_copy(k, it, true);
       }
     }
   }
@@ -566,23 +926,35 @@ class Vd2dEngine final : public Engine {
     if (!tz_adj_) {
       // the top step of the last segment reads only forward-stored history
       const Vd2dArgs<T> a1 = adj_args(ntmax_, (Kmax - 1) * S_);
-      tz_adj_ = tune("SWX_VD2D_TILE_ADJ", {8, 16}, [&](int tz) { launch_adj(a1, tz); });
+      tz_adj_ = tune("SWX_VD2D_TILE_ADJ", candidates({8, 16}), [&](int tz) { launch_adj(a1, tz); });
     }
     zero(ad_);
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
+    const int64_t F = (int64_t)ns_ * N_;
     for (int k = Kmax - 1; k >= 0; k--) {
       const int cb = k * S_;
+      // shots whose checkpoint (k+1)S exists replay segment k (initrecover!/
+      // recover!/getsaved, checkpointers.jl:80-105): p^{cb} from checkpoint
+      // k, cb+1 .. cb+S-1 recomputed, p^{cb+S} from checkpoint k+1
+      const int act = cb + S_ - 1;
       bool need = false;
-      for (int s = 0; s < nsh_; s++) need |= nt_[s] > cb + S_;
+      for (int s = 0; s < nsh_; s++) need |= !store_all_ && nt_[s] > act;
       if (need) {
         if (k == 0) zero(st_);
         else ckpt_copy(k, cb, false);
-        for (int it = cb + 1; it <= cb + S_; it++) fwd_step(it, cb + S_, 2, cb, nullptr);
+        dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 1024), (unsigned)nsh_);
+        copy_active_shots<T><<<g, 256, 0, c_.stream>>>(pbuf_.as<T>(), F3(st_, cb & 1, 0), N_,
+                                                       c_.pts.d_nt.as<int>(), act);
+        SWX_HIP(hipGetLastError());
+        for (int it = cb + 1; it <= cb + S_ - 1; it++) fwd_step(it, act, 2, cb, nullptr);
+        const T *ck1 = reinterpret_cast<const T *>(ck_.as<char>() + (int64_t)k * ck_one_);
+        copy_active_shots<T><<<g, 256, 0, c_.stream>>>(pbuf_.as<T>() + (int64_t)S_ * F, ck1, N_,
+                                                       c_.pts.d_nt.as<int>(), act);
+        SWX_HIP(hipGetLastError());
       }
       const int top = std::min(cb + S_, ntmax_);
       for (int it = top; it >= cb + 1; it--) adj_step(it, cb);
     }
-    const int64_t F = (int64_t)ns_ * N_;
     dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 4096));
     for (int f = 0; f < 3; f++)
       reduce_shots<T><<<g, 256, 0, c_.stream>>>(gout_.as<T>() + f * F, grad_.as<T>() + f * F, N_, nsh_, T(1),
@@ -612,6 +984,7 @@ class Vd2dEngine final : public Engine {
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return pbuf_.bytes(); }
   int segment() const override { return S_; }
+  void variants(int32_t v[2]) const override { v[0] = tz_fwd_; v[1] = tz_adj_; }
 
  private:
   T *M(int f) const { return m_.as<T>() + (int64_t)f * N_; }
@@ -644,7 +1017,8 @@ class Vd2dEngine final : public Engine {
     a.res = c_.pts.d_res.as<T>();
     a.nt_shot = c_.pts.d_nt.as<int>();
     a.it = it;
-    a.pbuf = pbuf_.as<T>(); a.pslot = (int64_t)ns_ * N_; a.S = S_ > 0 ? S_ : 1;
+    a.pbuf = pbuf_.as<T>(); a.pslot = (int64_t)ns_ * N_;
+    a.S = store_all_ ? (1 << 30) : S_;   // store-all: the final segment is the whole run
     const int64_t F = (int64_t)ns_ * N_;
     a.g0 = grad_.as<T>(); a.g1x = a.g0 ? a.g0 + F : nullptr; a.g1z = a.g0 ? a.g0 + 2 * F : nullptr;
     a.idt = T(1.0 / c_.d.dt);
@@ -658,8 +1032,27 @@ class Vd2dEngine final : public Engine {
     return (unsigned)((int64_t)a.nbx * a.nbz * nsh_);
   }
 
-  // Tile-height variants; the autotuner picks one per kernel and plan.
+  // z-marching grid: strips of 64*W columns, chunks sized so that the
+  // launch has about 2048 blocks (8 per CU) when the grid allows it
+  unsigned grid_march(Vd2dArgs<T> &a, int w) const {
+    a.nbx = (int)ceil_div(nx_, 64 * w);
+    const int64_t per = (int64_t)a.nbx * nsh_;
+    const int nbz = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, per), ceil_div(nz_, 32)));
+    a.ch = (int)ceil_div(nz_, nbz);
+    a.nbz = (int)ceil_div(nz_, a.ch);
+    return (unsigned)((int64_t)a.nbx * a.nbz * nsh_);
+  }
+
+  // Forward variants: tile heights 8/16/32 (2-D tiles) and 102/104
+  // (z-marching strips of 128/256 columns); the autotuner picks one.
   void launch_fwd(Vd2dArgs<T> a, int tz) {
+    if (tz == 102 || tz == 104) {
+      const unsigned g = grid_march(a, tz - 100);
+      if (tz == 102) vd2d_fwd_march<T, 2><<<g, 128, 0, c_.stream>>>(a);
+      else vd2d_fwd_march<T, 4><<<g, 256, 0, c_.stream>>>(a);
+      SWX_HIP(hipGetLastError());
+      return;
+    }
     const unsigned g = grid(a, tz);
     if (tz == 8) vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
     else if (tz == 16) vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
@@ -667,6 +1060,13 @@ class Vd2dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
   void launch_adj(Vd2dArgs<T> a, int tz) {
+    if (tz == 102 || tz == 104) {
+      const unsigned g = grid_march(a, tz - 100);
+      if (tz == 102) vd2d_adj_march<T, 2><<<g, 128, 0, c_.stream>>>(a);
+      else vd2d_adj_march<T, 4><<<g, 256, 0, c_.stream>>>(a);
+      SWX_HIP(hipGetLastError());
+      return;
+    }
     const unsigned g = grid(a, tz);
     if (tz == 8) vd2d_adj_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
     else vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
@@ -701,6 +1101,18 @@ class Vd2dEngine final : public Engine {
     c_.ktimer.end(c_.stream, 1);
   }
 
+  // Tuner candidates: the 2-D tile heights, plus the z-marching strips
+  // (codes 102/104) when kMarchInTuner is set.  Any listed code can also be
+  // pinned through the environment.
+  static constexpr bool kMarchInTuner = false;
+  static std::vector<int> candidates(std::vector<int> tiles) {
+    if (kMarchInTuner) {
+      tiles.push_back(102);
+      tiles.push_back(104);
+    }
+    return tiles;
+  }
+
   // Times each candidate tile height on a real launch (a step depends only
   // on the previous parity, so repeating it is harmless; the adjoint tuning
   // runs before the adjoint state and gradients are cleared) and returns the
@@ -711,6 +1123,7 @@ class Vd2dEngine final : public Engine {
       const int t = std::atoi(v);
       for (int c : cands)
         if (c == t) return t;
+      if (t == 102 || t == 104) return t;
     }
     hipEvent_t e0, e1;
     SWX_HIP(hipEventCreate(&e0));
@@ -733,7 +1146,8 @@ class Vd2dEngine final : public Engine {
   }
   int tz_fwd_ = 0, tz_adj_ = 0;
 
-  // checkpoint k (all shots): p, vx, vz, psi_x, psi_z, xi_x, xi_z at parity cb&1
+  // checkpoint k (k = 1 .. floor(nt_max/S), all shots): p, vx, vz, psi_x,
+  // psi_z, xi_x, xi_z at parity cb&1
   void ckpt_copy(int k, int cb, bool save) {
     const int64_t e = sizeof(T);
     const int par = cb & 1;
@@ -765,6 +1179,7 @@ class Vd2dEngine final : public Engine {
   DevBuf grad_, gout_, pbuf_, ck_;
   int64_t ck_one_ = 0, state_bytes_ = 0;
   int S_ = 1, K_ = 1, nck_ = 0;
+  bool store_all_ = true;
   int nsh_ = 0, ntmax_ = 0;
   std::vector<int> nt_;
   bool model_set_ = false;

@@ -50,6 +50,17 @@ __global__ void reduce_shots(T *out, const T *grad, int64_t N, int ns, T scale, 
   }
 }
 
+// Copies n elements per shot (shot slabs of n) from src to dst for the
+// shots being replayed (nt_s > act_min).
+template <typename T>
+__global__ void copy_active_shots(T *dst, const T *src, int64_t n, const int *nt_shot, int act_min) {
+  const int s = blockIdx.y;
+  if (nt_shot[s] <= act_min) return;
+  const int64_t o = (int64_t)s * n;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    dst[o + q] = src[o + q];
+}
+
 // Packs the per-shot C-PML coefficient vectors of the shot descriptors.
 template <typename T>
 std::vector<T> pack_cpml(int nshots, const swx_shot_desc *shots, int ndim, int h) {

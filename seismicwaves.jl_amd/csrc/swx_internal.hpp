@@ -168,6 +168,8 @@ class Engine {
   virtual int64_t bytes_checkpoints() const = 0;
   virtual int64_t bytes_replay() const = 0;
   virtual int segment() const = 0;
+  // Kernel variants chosen by the autotuner (0 if a family has one).
+  virtual void variants(int32_t v[2]) const { v[0] = v[1] = 0; }
 };
 
 struct PlanCore {

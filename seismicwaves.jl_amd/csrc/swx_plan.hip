@@ -375,6 +375,7 @@ int swx_plan_get_info(swx_plan *plan, swx_plan_info *info) {
       info->ms_kernel[k] = plan->core.ms_kernel[k];
       info->n_kernel[k] = plan->core.n_kernel[k];
     }
+    plan->engine->variants(info->kernel_variant);
   });
 }
 

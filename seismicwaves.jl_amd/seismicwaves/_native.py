@@ -62,6 +62,7 @@ class PlanInfo(ctypes.Structure):
         ("bytes_replay", ctypes.c_int64), ("segment", ctypes.c_int32), ("nshots", ctypes.c_int32),
         ("ms_forward", ctypes.c_double), ("ms_adjoint", ctypes.c_double),
         ("ms_kernel", ctypes.c_double * 2), ("n_kernel", ctypes.c_int64 * 2),
+        ("kernel_variant", ctypes.c_int32 * 2),
     ]
 
 

@@ -116,6 +116,29 @@ def test_checkpoint_intervals_agree_bitwise(oracle_mod):
         assert np.array_equal(o["vp"], outs[0]["vp"]) and np.array_equal(o["rho"], outs[0]["rho"])
 
 
+def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
+    """Every forward and adjoint kernel variant the autotuner may pick (2-D
+    tiles of height 8/16/32, z-marching strips of 128/256 columns) gives the
+    same seismograms and gradients bit for bit, and matches the oracle."""
+    p = _problem(np.float64, nts=(300, 257), n=(301, 150))
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o) for o in obs]
+    mp = sw.VpRhoAcousticVDMaterialProperties(p["v0"], p["r0"])
+    outs = []
+    for v, w in (("8", "8"), ("16", "16"), ("32", "102"), ("102", "104"), ("104", "8")):
+        print(f"[vd2d kernel variants forward {v} adjoint {w}]", flush=True)
+        monkeypatch.setenv("SWX_VD2D_TILE_FWD", v)
+        monkeypatch.setenv("SWX_VD2D_TILE_ADJ", w)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=40))
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
+    for g, seis in outs[1:]:
+        assert np.array_equal(g["vp"], outs[0][0]["vp"]) and np.array_equal(g["rho"], outs[0][0]["rho"])
+        assert all(np.array_equal(a, b) for a, b in zip(seis, outs[0][1]))
+    gref = _oracle_grad(oracle_mod, p, obs, 40)[0]
+    assert rel_l2(outs[-1][0]["vp"], gref["vp"]) <= 1e-10
+
+
 @pytest.mark.parametrize("case", ["colocated", "edge_points", "halo0"])
 def test_point_edge_cases(oracle_mod, case):
     dtype = np.float64
