@@ -188,9 +188,10 @@ def _strip_cells(nx, nz, halo):
     return (2 * halo * nz + 2 * halo * nx) + ((2 * halo + 2) * nz + (2 * halo + 2) * nx)
 
 
-def alg_bytes(physics, nx, nz, halo, e, nts, S):
+def alg_bytes(physics, nx, nz, halo, e, nts, S, variant=(0, 0)):
     """Returns {kernel family: (total bytes of one gradient, launches)} for
-    family 0 (forward/replay step kernel) and 1 (adjoint step kernel)."""
+    family 0 (forward/replay step kernel) and 1 (adjoint step kernel);
+    `variant` holds the kernel variants the plan picked."""
     N = nx * nz
     strip = _strip_cells(nx, nz, halo) * 2 * e
     fwd_b = fwd_n = adj_b = adj_n = 0
@@ -205,15 +206,20 @@ def alg_bytes(physics, nx, nz, halo, e, nts, S):
             last = nt - kl * S              # steps of the last segment: stored during the forward
             rep = nt - last                 # steps replayed from checkpoints
         if physics == "el2d":
-            # forward: stress r ux, uz, lambda, mu, mu_ihjh, w 3 sigma (8e) + displacement
-            # r 3 sigma, uold, ucur (2 each), dt2/rho x2, w 2 u (11e); + w u history (2e).
-            # Adjoint: the same 19e + correlation r 3 history levels x 2 + 5 grad RMW (16e)
+            # two-launch step: stress r ux, uz, lambda, mu, mu_ihjh, w 3 sigma (8e) +
+            # displacement r 3 sigma, uold, ucur (2 each), dt2/rho x2, w 2 u (11e) = 19e.
+            # z-marching step (one launch, stresses stay in LDS): r ucur, uold (2 each),
+            # lambda, mu, mu_ihjh, dt2/rho x2, w 2 u = 11e.  + w u history (2e).
+            # Adjoint: the same + correlation r 3 history levels x 2 + 5 grad RMW (16e)
+            march_f, march_a = variant[0] > 100, variant[1] > 100
+            sf = 11 if march_f else 19
+            sa = 11 if march_a else 19
             last = nt - (nt // S) * S
             rep = (nt // S) * (S - 1)
-            fwd_b += nt * 19 * e * N + last * 2 * e * N + rep * 21 * e * N
-            fwd_n += 2 * (nt + rep)
-            adj_b += nt * 35 * e * N
-            adj_n += 2 * nt
+            fwd_b += nt * sf * e * N + last * 2 * e * N + rep * (sf + 2) * e * N
+            fwd_n += (1 if march_f else 2) * (nt + rep)
+            adj_b += nt * (sa + 16) * e * N
+            adj_n += (1 if march_a else 2) * nt
             continue
         if physics == "vd2d":
             # forward: r p, vx, vz, m0, m1x, m1z; w p, vx, vz (9e); + w p-history (10e)
@@ -243,8 +249,11 @@ def kernel_names(physics, variant):
             return f"vd2d_{kind}_march<float, {v - 100}>" if v > 100 else f"vd2d_{kind}_kernel<float, {v}>"
         return name("fwd", variant[0]), name("adj", variant[1])
     if physics == "el2d":
-        return ("el_stress_kernel<float> + el_disp_kernel<float> (forward)",
-                "el_stress_kernel<float> + el_disp_kernel<float> (adjoint)")
+        def name(adj, v):
+            if v > 100:
+                return f"el_march_kernel<float, {v - 100}, {'true' if adj else 'false'}>"
+            return "el_stress_kernel<float> + el_disp_kernel<float> (" + ("adjoint)" if adj else "forward)")
+        return name(False, variant[0]), name(True, variant[1])
     return "cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)"
 
 
@@ -444,7 +453,7 @@ def main():
     nx, nz = p.gridsize
     cells = nx * nz
     value = 2.0 * cells * sum(nts) / (ms_step / 1e3) / 1e9
-    ab = alg_bytes(wl["physics"], nx, nz, p.boundcond.halo, 4, nts, S)
+    ab = alg_bytes(wl["physics"], nx, nz, p.boundcond.halo, 4, nts, S, list(info.kernel_variant))
     fam = {}
     for k, name in ((0, "forward/replay step"), (1, "adjoint step + correlation")):
         b, _ = ab[k]

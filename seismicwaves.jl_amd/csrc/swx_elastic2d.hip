@@ -24,6 +24,7 @@
 // the replayed forward history u^{c-2}, u^{c-1}, u^c; the first step of each
 // replay segment is correlated by a correlate-only launch.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 
 #include "swx_device.hpp"
@@ -53,6 +54,7 @@ struct TapView {
 template <typename T>
 struct ElArgs {
   int nx, nz, h, freetop;
+  int nbx, nbz, ch;             // z-marching launch: strips, chunks, rows per chunk
   int64_t N, nrows;
   T idx, idz, idt2;
   const T *lam, *mu, *muhh, *rdx, *rdz;  // rdx = dt^2/rho_ihalf, rdz = dt^2/rho_jhalf
@@ -62,7 +64,7 @@ struct ElArgs {
   const T *ucur;          // [ns][2][N]: ux | uz at level it-1
   T *unew;                // level it (= slot of it-2, updated in place)
   T *sig;                 // [ns][3][N]: sxx | szz | sxz
-  T *psi; int64_t psi_stride;
+  const T *psi_o; T *psi_n; int64_t psi_stride;   // C-PML memory variables, by step parity
   int64_t psioff[8];
   // points: kind 0 none, 1 moment tensor (stress), 2 force or residual (displacement)
   int kind;
@@ -90,28 +92,28 @@ struct F2 {
 
 // Derivative wrappers with the boundary rules of
 // freesurface_derivatives_4th_mirror.jl:72-233.
-template <typename T>
-__device__ __forceinline__ T d_sxx_dx(F2<T> s, int i, int j, int nx, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_sxx_dx(A s, int i, int j, int nx, T id) {
   if (i == 1) return d4<T>(0, s(i, j), s(i + 1, j), s(i + 2, j), id);
   if (i == nx - 1) return d4<T>(s(i - 1, j), s(i, j), s(i + 1, j), 0, id);
   return d4<T>(s(i - 1, j), s(i, j), s(i + 1, j), s(i + 2, j), id);
 }
-template <typename T>
-__device__ __forceinline__ T d_szz_dz(F2<T> s, int i, int j, int nz, bool ft, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_szz_dz(A s, int i, int j, int nz, bool ft, T id) {
   if (j == 1) return d4<T>(ft ? -s(i, j + 1) : T(0), s(i, j), s(i, j + 1), s(i, j + 2), id);
   if (j == nz - 1) return d4<T>(s(i, j - 1), s(i, j), s(i, j + 1), 0, id);
   return d4<T>(s(i, j - 1), s(i, j), s(i, j + 1), s(i, j + 2), id);
 }
-template <typename T>
-__device__ __forceinline__ T d_sxz_dx(F2<T> s, int i, int j, int nx, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_sxz_dx(A s, int i, int j, int nx, T id) {
   if (i == 1) return d4<T>(0, 0, s(i, j), s(i + 1, j), id);
   if (i == 2) return d4<T>(0, s(i - 1, j), s(i, j), s(i + 1, j), id);
   if (i == nx - 1) return d4<T>(s(i - 2, j), s(i - 1, j), s(i, j), 0, id);
   if (i == nx) return d4<T>(s(i - 2, j), s(i - 1, j), 0, 0, id);
   return d4<T>(s(i - 2, j), s(i - 1, j), s(i, j), s(i + 1, j), id);
 }
-template <typename T>
-__device__ __forceinline__ T d_sxz_dz(F2<T> s, int i, int j, int nz, bool ft, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_sxz_dz(A s, int i, int j, int nz, bool ft, T id) {
   if (j == 1) {
     if (ft) return d4<T>(-s(i, j + 1), -s(i, j), s(i, j), s(i, j + 1), id);
     return d4<T>(0, 0, s(i, j), s(i, j + 1), id);
@@ -121,8 +123,8 @@ __device__ __forceinline__ T d_sxz_dz(F2<T> s, int i, int j, int nz, bool ft, T 
   if (j == nz) return d4<T>(s(i, j - 2), s(i, j - 1), 0, 0, id);
   return d4<T>(s(i, j - 2), s(i, j - 1), s(i, j), s(i, j + 1), id);
 }
-template <typename T>
-__device__ __forceinline__ T d_ux_dx(F2<T> u, int i, int j, int nx, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_ux_dx(A u, int i, int j, int nx, T id) {
   if (i == 1) return d4<T>(0, 0, u(i, j), u(i + 1, j), id);
   if (i == 2) return d4<T>(0, u(i - 1, j), u(i, j), u(i + 1, j), id);
   if (i == nx - 1) return d4<T>(u(i - 2, j), u(i - 1, j), u(i, j), 0, id);
@@ -130,11 +132,10 @@ __device__ __forceinline__ T d_ux_dx(F2<T> u, int i, int j, int nx, T id) {
   return d4<T>(u(i - 2, j), u(i - 1, j), u(i, j), u(i + 1, j), id);
 }
 // duz/dz; at the free surface from sigma_zz = 0: -lam/(lam+2mu) dux/dx
-template <typename T>
-__device__ __forceinline__ T d_uz_dz(F2<T> ux, F2<T> uz, T l, T m, int i, int j, int nx, int nz, bool ft, T idx,
-                                     T idz) {
+template <typename T, typename A, typename B>
+__device__ __forceinline__ T d_uz_dz(A ux, B uz, T l, T m, int i, int j, int nx, int nz, bool ft, T idx, T idz) {
   if (j == 1) {
-    if (ft) return (-l / (l + T(2) * m)) * d_ux_dx(ux, i, j, nx, idx);
+    if (ft) return (-l / (l + T(2) * m)) * d_ux_dx<T>(ux, i, j, nx, idx);
     return d4<T>(0, 0, uz(i, j), uz(i, j + 1), idz);
   }
   if (j == 2) return d4<T>(ft ? uz(i, j - 1) : T(0), uz(i, j - 1), uz(i, j), uz(i, j + 1), idz);
@@ -142,14 +143,14 @@ __device__ __forceinline__ T d_uz_dz(F2<T> ux, F2<T> uz, T l, T m, int i, int j,
   if (j == nz) return d4<T>(uz(i, j - 2), uz(i, j - 1), 0, 0, idz);
   return d4<T>(uz(i, j - 2), uz(i, j - 1), uz(i, j), uz(i, j + 1), idz);
 }
-template <typename T>
-__device__ __forceinline__ T d_ux_dz(F2<T> u, int i, int j, int nz, bool ft, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_ux_dz(A u, int i, int j, int nz, bool ft, T id) {
   if (j == 1) return d4<T>(ft ? u(i, j + 1) : T(0), u(i, j), u(i, j + 1), u(i, j + 2), id);
   if (j == nz - 1) return d4<T>(u(i, j - 1), u(i, j), u(i, j + 1), 0, id);
   return d4<T>(u(i, j - 1), u(i, j), u(i, j + 1), u(i, j + 2), id);
 }
-template <typename T>
-__device__ __forceinline__ T d_uz_dx(F2<T> u, int i, int j, int nx, T id) {
+template <typename T, typename A>
+__device__ __forceinline__ T d_uz_dx(A u, int i, int j, int nx, T id) {
   if (i == 1) return d4<T>(0, u(i, j), u(i + 1, j), u(i + 2, j), id);
   if (i == nx - 1) return d4<T>(u(i - 1, j), u(i, j), u(i + 1, j), 0, id);
   return d4<T>(u(i - 1, j), u(i, j), u(i + 1, j), u(i + 2, j), id);
@@ -188,7 +189,8 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
   const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
   const F2<T> ux{a.ucur + (int64_t)s * 2 * N, nx}, uz{a.ucur + (int64_t)s * 2 * N + N, nx};
   T *sg = a.sig + (int64_t)s * 3 * N;
-  T *ps = a.psi + (int64_t)s * a.psi_stride;
+  const T *pso = a.psi_o + (int64_t)s * a.psi_stride;   // C-PML memory, parity of the read state
+  T *psn = a.psi_n + (int64_t)s * a.psi_stride;         // parity of the written state
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
 
@@ -205,15 +207,17 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
       T dxc = dux, dzc = duz;
       int k = strip_half(i - 1, nx - 1, h);
       if (k) {
-        T &p = ps[a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2)];
-        p = cx.b[k - 1] * p + cx.a[k - 1] * dux;
-        dxc = dux + p;
+        const int64_t q = a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
+        const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * dux;
+        psn[q] = pn;
+        dxc = dux + pn;
       }
       k = strip_half(j - 1, nz - 1, h);
       if (k) {
-        T &p = ps[a.psioff[7] + (i - 1) + (int64_t)(k - 1) * nx];
-        p = cz.b[k - 1] * p + cz.a[k - 1] * duz;
-        dzc = duz + p;
+        const int64_t q = a.psioff[7] + (i - 1) + (int64_t)(k - 1) * nx;
+        const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * duz;
+        psn[q] = pn;
+        dzc = duz + pn;
       }
       vxx = (l + T(2) * m) * dxc + l * dzc;
       vzz = j == 1 ? T(0) : l * dxc + (l + T(2) * m) * dzc;
@@ -227,15 +231,17 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
       T c1 = duzx, c2 = duxz;
       int k = strip_full(i, nx, h);
       if (k) {
-        T &p = ps[a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h)];
-        p = cx.bh[k - 1] * p + cx.ah[k - 1] * duzx;
-        c1 = duzx + p;
+        const int64_t q = a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h);
+        const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * duzx;
+        psn[q] = pn;
+        c1 = duzx + pn;
       }
       k = strip_full(j, nz, h);
       if (k) {
-        T &p = ps[a.psioff[6] + (i - 1) + (int64_t)(k - 1) * (nx - 1)];
-        p = cz.bh[k - 1] * p + cz.ah[k - 1] * duxz;
-        c2 = duxz + p;
+        const int64_t q = a.psioff[6] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
+        const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * duxz;
+        psn[q] = pn;
+        c2 = duxz + pn;
       }
       vxz = a.muhh[c] * (c1 + c2);
       ws = true;
@@ -321,7 +327,8 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
   T *un = a.unew + (int64_t)s * 2 * N;
   const T *sg = a.sig + (int64_t)s * 3 * N;
   const F2<T> sxx{sg, nx}, szz{sg + N, nx}, sxz{sg + 2 * N, nx};
-  T *ps = a.psi + (int64_t)s * a.psi_stride;
+  const T *pso = a.psi_o + (int64_t)s * a.psi_stride;   // C-PML memory, parity of the read state
+  T *psn = a.psi_n + (int64_t)s * a.psi_stride;         // parity of the written state
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
   const bool hx = i <= nx - 1, hz = j <= nz - 1;
@@ -332,15 +339,17 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
     T c1 = d1, c2 = d2;
     int k = strip_full(i, nx, h);
     if (k) {
-      T &p = ps[a.psioff[0] + (k - 1) + (int64_t)(j - 1) * (2 * h)];
-      p = cx.bh[k - 1] * p + cx.ah[k - 1] * d1;
-      c1 = d1 + p;
+      const int64_t q = a.psioff[0] + (k - 1) + (int64_t)(j - 1) * (2 * h);
+      const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * d1;
+      psn[q] = pn;
+      c1 = d1 + pn;
     }
     k = strip_half(j - 1, nz - 1, h);
     if (k) {
-      T &p = ps[a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1)];
-      p = cz.b[k - 1] * p + cz.a[k - 1] * d2;
-      c2 = d2 + p;
+      const int64_t q = a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
+      const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * d2;
+      psn[q] = pn;
+      c2 = d2 + pn;
     }
     oldx = un[c];
     curx = uc[c];
@@ -352,15 +361,17 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
     T c1 = d1, c2 = d2;
     int k = strip_half(i - 1, nx - 1, h);
     if (k) {
-      T &p = ps[a.psioff[1] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2)];
-      p = cx.b[k - 1] * p + cx.a[k - 1] * d1;
-      c1 = d1 + p;
+      const int64_t q = a.psioff[1] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
+      const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * d1;
+      psn[q] = pn;
+      c1 = d1 + pn;
     }
     k = strip_full(j, nz, h);
     if (k) {
-      T &p = ps[a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx];
-      p = cz.bh[k - 1] * p + cz.ah[k - 1] * d2;
-      c2 = d2 + p;
+      const int64_t q = a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx;
+      const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * d2;
+      psn[q] = pn;
+      c2 = d2 + pn;
     }
     oldz = un[N + c];
     curz = uc[N + c];
@@ -413,6 +424,432 @@ __global__ void el_hist_copy(T *dst, const T *src, int64_t n2, const int *nt_sho
   const int64_t o = (int64_t)s * n2;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n2; q += (int64_t)gridDim.x * blockDim.x)
     dst[o + q] = src[o + q];
+}
+
+// Block -> (strip, chunk, shot) map giving each of the 8 XCDs a contiguous
+// run, so neighbouring strips share halo lines in one L2.
+__device__ __forceinline__ int xcd_tile_el(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= per * 8) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Row ring in LDS seen through the reference's 1-based (i, j): row j lives
+// in slot j mod NR, column i at index i - i0.
+template <typename T, int NR, int PITCH>
+struct RingA {
+  const T *b;
+  int i0;
+  __device__ __forceinline__ T operator()(int i, int j) const {
+    return b[(((j % NR) + NR) % NR) * PITCH + (i - i0)];
+  }
+};
+
+// Moment-tensor injection into the normal-stress cells the stencil never
+// updates (columns 1 and nx, row nz, row 1 without free surface): there the
+// reference's stresses keep accumulating (elastic2D_iso_xPU.jl:81-94 on a
+// field update_sxx_szz! does not touch).  Run before el_march_kernel, which
+// reads these cells from HBM.
+template <typename T>
+__global__ void el_edge_inject_kernel(ElArgs<T> a) {
+  const int s = blockIdx.y;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int *rows = a.f0.rows + (int64_t)s * (a.nrows + 1);
+  const int u = rows[0] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= rows[a.nrows]) return;
+  const int64_t c = a.f0.cell[u];
+  const int i = (int)(c % a.nx) + 1, j = (int)(c / a.nx) + 1;
+  const bool nrm = i >= 2 && i <= a.nx - 1 && j >= (a.freetop ? 1 : 2) && j <= a.nz - 1;
+  if (nrm) return;
+  T *sg = a.sig + (int64_t)s * 3 * a.N;
+  const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+  const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+  T vxx = sg[c], vzz = sg[a.N + c];
+  for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++) {
+    const int p = a.f0.epoint[e];
+    const T f = tf[(int64_t)nts * p];
+    vxx += M[3 * p + 0] * a.f0.ecoef[e] * f;
+    vzz += M[3 * p + 1] * a.f0.ecoef[e] * f;
+  }
+  sg[c] = vxx;
+  sg[a.N + c] = vzz;
+}
+
+// One time step marching down z: stresses of row j (own columns plus the
+// halo columns the displacement stencils need) from LDS rings of ucur rows,
+// then the displacements of row j-2, whose stress rows j-4..j are in the
+// stress rings.  The stresses never go to HBM (except the never-updated
+// edge cells, see el_edge_inject_kernel).  A block owns a strip of 64*W
+// columns and a chunk of rows.  Forward (ADJ = false): moment-tensor and
+// force injection, history store.  Adjoint (ADJ = true): residual injection
+// and the correlation of adjoint step corr_step on the own stress cells,
+// with rings of the replayed level corr_step-1.  Two barriers per row.
+template <typename T, int W, bool ADJ>
+__global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
+  constexpr int TXM = 64 * W;
+  constexpr int PU = TXM + 6;   // ux, uz rings: columns [I1-3, I1+TXM+2]
+  constexpr int PS = TXM + 3;   // sxx ring [I1-1, I1+TXM+1]; sxz ring [I1-2, I1+TXM]; history rings
+  constexpr int NH = ADJ ? 5 : 1;
+  const int b = xcd_tile_el(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (ADJ ? a.it > nts : (a.it > nts || nts <= a.act_min)) return;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const bool ft = a.freetop != 0;
+  const int64_t N = a.N;
+  const int I1 = bx * TXM + 1;
+  const int Zd0 = bz * a.ch + 1, Zd1 = min(bz * a.ch + a.ch, nz) + 1;   // owned rows [Zd0, Zd1)
+  const int tx = threadIdx.x;
+  const int i = I1 + tx;
+  const T *ucx = a.ucur + (int64_t)s * 2 * N, *ucz = ucx + N;
+  T *un = a.unew + (int64_t)s * 2 * N;
+  const T *pso = a.psi_o + (int64_t)s * a.psi_stride;
+  T *psn = a.psi_n + (int64_t)s * a.psi_stride;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  const bool corr = ADJ && a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
+
+  __shared__ T rux[6][PU], ruz[5][PU];
+  __shared__ T rsxx[3][PS], rszz[4][TXM], rsxz[5][PS];
+  __shared__ T rhx[NH][PS], rhz[NH][PS];   // level corr_step-1: ux [I1-2, I1+TXM], uz [I1-1, I1+TXM+1]
+  const RingA<T, 6, PU> UX{&rux[0][0], I1 - 3};
+  const RingA<T, 5, PU> UZ{&ruz[0][0], I1 - 3};
+  const RingA<T, 3, PS> SXX{&rsxx[0][0], I1 - 1};
+  const RingA<T, 4, TXM> SZZ{&rszz[0][0], I1};
+  const RingA<T, 5, PS> SXZ{&rsxz[0][0], I1 - 2};
+  const RingA<T, NH, PS> HX{&rhx[0][0], I1 - 2};
+  const RingA<T, NH, PS> HZ{&rhz[0][0], I1 - 1};
+  auto slot = [](int j, int nr) { return ((j % nr) + nr) % nr; };
+
+  // column duties: u halo (tx < 6), sxx halo / sxz halo / history halos (tx < 3)
+  const int hu_lx = tx < 3 ? tx : TXM + tx, hu_x = I1 - 3 + hu_lx;
+  const int hxx_x = tx == 0 ? I1 - 1 : I1 + TXM + tx - 1;
+  const int hxz_x = tx < 2 ? I1 - 2 + tx : I1 + TXM;
+  const int hhx_x = hxz_x;                       // history ux halo: I1-2, I1-1, I1+TXM
+  const int hhz_x = tx == 0 ? I1 - 1 : I1 + TXM + tx - 1;   // history uz halo: I1-1, I1+TXM, I1+TXM+1
+  auto ldx = [&](const T *f, int x, int j) -> T {   // ux-grid field (nx-1) x nz, 1-based
+    return (x >= 1 && x <= nx - 1 && j >= 1 && j <= nz) ? f[(int64_t)(j - 1) * nx + (x - 1)] : T(0);
+  };
+  auto ldz = [&](const T *f, int x, int j) -> T {   // uz-grid field nx x (nz-1)
+    return (x >= 1 && x <= nx && j >= 1 && j <= nz - 1) ? f[(int64_t)(j - 1) * nx + (x - 1)] : T(0);
+  };
+  auto ldn = [&](const T *f, int x, int j) -> T {   // node field nx x nz
+    return (x >= 1 && x <= nx && j >= 1 && j <= nz) ? f[(int64_t)(j - 1) * nx + (x - 1)] : T(0);
+  };
+  auto ldh = [&](const T *f, int x, int j) -> T {   // sxz-grid field (nx-1) x (nz-1)
+    return (x >= 1 && x <= nx - 1 && j >= 1 && j <= nz - 1) ? f[(int64_t)(j - 1) * nx + (x - 1)] : T(0);
+  };
+  const T *h0 = nullptr, *h1 = nullptr, *h2 = nullptr;
+  if (ADJ && corr) {
+    const int sl = a.corr_step - a.cbase + 1;
+    h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
+    h1 = h0 + a.hslot;
+    h2 = h1 + a.hslot;
+  }
+
+  // preload: ucur rows Zd0-3 .. Zd0-1 (ux), Zd0-4 .. Zd0-2 (uz); history alike
+  for (int j = Zd0 - 3; j <= Zd0 - 1; j++) {
+    rux[slot(j, 6)][tx + 3] = ldx(ucx, i, j);
+    if (tx < 6) rux[slot(j, 6)][hu_lx] = ldx(ucx, hu_x, j);
+    if (ADJ && corr) {
+      rhx[slot(j, NH)][tx + 2] = ldx(h1, i, j);
+      if (tx < 3) rhx[slot(j, NH)][hhx_x - (I1 - 2)] = ldx(h1, hhx_x, j);
+    }
+  }
+  for (int j = Zd0 - 4; j <= Zd0 - 2; j++) {
+    ruz[slot(j, 5)][tx + 3] = ldz(ucz, i, j);
+    if (tx < 6) ruz[slot(j, 5)][hu_lx] = ldz(ucz, hu_x, j);
+    if (ADJ && corr) {
+      rhz[slot(j, NH)][tx + 1] = ldz(h1 + N, i, j);
+      if (tx < 3) rhz[slot(j, NH)][hhz_x - (I1 - 1)] = ldz(h1 + N, hhz_x, j);
+    }
+  }
+
+  // prefetch registers of iteration j (loaded during iteration j-1)
+  T n_ux = 0, n_uxh = 0, n_uz = 0, n_uzh = 0, n_l = 0, n_m = 0, n_lh = 0, n_mh = 0, n_mhh = 0, n_mhhh = 0;
+  T n_ox = 0, n_oz = 0, n_rx = 0, n_rz = 0;
+  T n_hx = 0, n_hxh = 0, n_hz = 0, n_hzh = 0, n_h0x = 0, n_h0z = 0, n_h2x = 0, n_h2z = 0;
+  T n_g[5] = {0, 0, 0, 0, 0};
+  auto issue = [&](int j) {
+    n_ux = ldx(ucx, i, j + 2);
+    n_uz = ldz(ucz, i, j + 1);
+    n_l = ldn(a.lam, i, j);
+    n_m = ldn(a.mu, i, j);
+    n_mhh = ldh(a.muhh, i, j);
+    n_ox = ldx(un, i, j - 2);
+    n_oz = ldz(un + N, i, j - 2);
+    n_rx = ldx(a.rdx, i, j - 2);
+    n_rz = ldz(a.rdz, i, j - 2);
+    if (tx < 6) {
+      n_uxh = ldx(ucx, hu_x, j + 2);
+      n_uzh = ldz(ucz, hu_x, j + 1);
+    }
+    if (tx < 3) {
+      n_lh = ldn(a.lam, hxx_x, j);
+      n_mh = ldn(a.mu, hxx_x, j);
+      n_mhhh = ldh(a.muhh, hxz_x, j);
+    }
+    if (ADJ && corr) {
+      n_hx = ldx(h1, i, j + 2);
+      n_hz = ldz(h1 + N, i, j + 1);
+      if (tx < 3) {
+        n_hxh = ldx(h1, hhx_x, j + 2);
+        n_hzh = ldz(h1 + N, hhz_x, j + 1);
+      }
+      n_h0x = ldx(h0, i, j);
+      n_h2x = ldx(h2, i, j);
+      n_h0z = ldz(h0 + N, i, j);
+      n_h2z = ldz(h2 + N, i, j);
+      T *g = a.grad + (int64_t)s * a.gstride;
+      n_g[0] = ldx(g, i, j);
+      n_g[1] = ldz(g + N, i, j);
+      n_g[2] = ldn(g + 2 * N, i, j);
+      n_g[3] = ldn(g + 3 * N, i, j);
+      n_g[4] = ldh(g + 4 * N, i, j);
+    }
+  };
+  issue(Zd0 - 2);
+  __syncthreads();
+
+  // normal stresses at (x, j); raw derivatives returned for the correlation
+  auto normal = [&](int x, int j, T l, T m, bool owned, T &vxx, T &vzz, T &dux, T &duz) {
+    vxx = vzz = dux = duz = 0;
+    if (x < 1 || x > nx || j < 1 || j > nz) return;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    const bool nrm = x >= 2 && x <= nx - 1 && j >= (ft ? 1 : 2) && j <= nz - 1;
+    if (!nrm) {
+      // never updated: zero, or the accumulated injections (forward)
+      if (!ADJ && a.kind == 1) {
+        const T *sg = a.sig + (int64_t)s * 3 * N;
+        vxx = sg[c];
+        vzz = sg[N + c];
+      }
+      return;
+    }
+    dux = d_ux_dx<T>(UX, x, j, nx, a.idx);
+    duz = d_uz_dz<T>(UX, UZ, l, m, x, j, nx, nz, ft, a.idx, a.idz);
+    T dxc = dux, dzc = duz;
+    int k = strip_half(x - 1, nx - 1, h);
+    if (k) {
+      const int64_t q = a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
+      const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * dux;
+      if (owned) psn[q] = pn;
+      dxc = dux + pn;
+    }
+    k = strip_half(j - 1, nz - 1, h);
+    if (k) {
+      const int64_t q = a.psioff[7] + (x - 1) + (int64_t)(k - 1) * nx;
+      const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * duz;
+      if (owned) psn[q] = pn;
+      dzc = duz + pn;
+    }
+    vxx = (l + T(2) * m) * dxc + l * dzc;
+    vzz = j == 1 ? T(0) : l * dxc + (l + T(2) * m) * dzc;
+    if (!ADJ && a.kind == 1) {
+      const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
+      if (u >= 0) {
+        const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+        const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+        for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++) {
+          const int p = a.f0.epoint[e];
+          const T f = tf[(int64_t)nts * p];
+          vxx += M[3 * p + 0] * a.f0.ecoef[e] * f;
+          vzz += M[3 * p + 1] * a.f0.ecoef[e] * f;
+        }
+      }
+    }
+  };
+  // shear stress at (x, j)
+  auto shear = [&](int x, int j, T mhh, bool owned, T &vxz, T &duzx, T &duxz) {
+    vxz = duzx = duxz = 0;
+    if (x < 1 || x > nx - 1 || j < 1 || j > nz - 1) return;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    duzx = d_uz_dx<T>(UZ, x, j, nx, a.idx);
+    duxz = d_ux_dz<T>(UX, x, j, nz, ft, a.idz);
+    T c1 = duzx, c2 = duxz;
+    int k = strip_full(x, nx, h);
+    if (k) {
+      const int64_t q = a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h);
+      const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * duzx;
+      if (owned) psn[q] = pn;
+      c1 = duzx + pn;
+    }
+    k = strip_full(j, nz, h);
+    if (k) {
+      const int64_t q = a.psioff[6] + (x - 1) + (int64_t)(k - 1) * (nx - 1);
+      const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * duxz;
+      if (owned) psn[q] = pn;
+      c2 = duxz + pn;
+    }
+    vxz = mhh * (c1 + c2);
+    if (!ADJ && a.kind == 1) {
+      const int u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, j - 1, c);
+      if (u >= 0) {
+        const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+        const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+        for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++) {
+          const int p = a.f1.epoint[e];
+          vxz += M[3 * p + 2] * a.f1.ecoef[e] * tf[(int64_t)nts * p];
+        }
+      }
+    }
+  };
+
+  const int sb = a.hmode == 1 ? (nts / a.S) * a.S : a.cbase;
+  for (int j = Zd0 - 2; j <= Zd1 + 1; j++) {
+    // commit ucur rows j+2 (ux), j+1 (uz) and the history rows
+    rux[slot(j + 2, 6)][tx + 3] = n_ux;
+    ruz[slot(j + 1, 5)][tx + 3] = n_uz;
+    if (tx < 6) {
+      rux[slot(j + 2, 6)][hu_lx] = n_uxh;
+      ruz[slot(j + 1, 5)][hu_lx] = n_uzh;
+    }
+    if (ADJ && corr) {
+      rhx[slot(j + 2, NH)][tx + 2] = n_hx;
+      rhz[slot(j + 1, NH)][tx + 1] = n_hz;
+      if (tx < 3) {
+        rhx[slot(j + 2, NH)][hhx_x - (I1 - 2)] = n_hxh;
+        rhz[slot(j + 1, NH)][hhz_x - (I1 - 1)] = n_hzh;
+      }
+    }
+    const T c_l = n_l, c_m = n_m, c_lh = n_lh, c_mh = n_mh, c_mhh = n_mhh, c_mhhh = n_mhhh;
+    const T c_ox = n_ox, c_oz = n_oz, c_rx = n_rx, c_rz = n_rz;
+    const T c_h0x = n_h0x, c_h0z = n_h0z, c_h2x = n_h2x, c_h2z = n_h2z;
+    T c_g[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) c_g[q] = n_g[q];
+    if (j + 1 <= Zd1 + 1) issue(j + 1);
+    __syncthreads();
+
+    // ---- stresses of row j --------------------------------------------
+    const bool own_row = j >= Zd0 && j < Zd1;
+    const bool own = own_row && i <= nx;
+    {
+      T vxx, vzz, dux, duz, vxz, duzx, duxz;
+      normal(i, j, c_l, c_m, own, vxx, vzz, dux, duz);
+      shear(i, j, c_mhh, own, vxz, duzx, duxz);
+      rsxx[slot(j, 3)][tx + 1] = vxx;
+      rszz[slot(j, 4)][tx] = vzz;
+      rsxz[slot(j, 5)][tx + 2] = vxz;
+      if (ADJ && corr && own) {
+        // correlate_gradients! (correlate_gradient_xPU.jl:1-82) of step corr_step
+        T *g = a.grad + (int64_t)s * a.gstride;
+        const int64_t c = (int64_t)(j - 1) * nx + (i - 1);
+        if (i <= nx - 1) {
+          const T v = UX(i, j) * (c_h0x - T(2) * HX(i, j) + c_h2x) * a.idt2;
+          g[c] = c_g[0] + ((j == 1 && ft) ? v / T(2) : v);
+        }
+        if (j <= nz - 1) g[N + c] = c_g[1] + UZ(i, j) * (c_h0z - T(2) * HZ(i, j) + c_h2z) * a.idt2;
+        if (i >= 2 && i <= nx - 1 && j >= (ft ? 1 : 2) && j <= nz - 1) {
+          const T exx = d_ux_dx<T>(HX, i, j, nx, a.idx);
+          const T ezz = d_uz_dz<T>(HX, HZ, c_l, c_m, i, j, nx, nz, ft, a.idx, a.idz);
+          const T dv = exx + ezz, dva = dux + duz;
+          if (j == 1 && ft) {
+            g[2 * N + c] = c_g[2] + dv * dva / T(2);
+            g[3 * N + c] = c_g[3] + (exx * dux + ezz * duz);
+          } else {
+            g[2 * N + c] = c_g[2] + dv * dva;
+            g[3 * N + c] = c_g[3] + T(2) * (exx * dux + ezz * duz);
+          }
+        }
+        if (i <= nx - 1 && j <= nz - 1) {
+          const T exz = (d_uz_dx<T>(HZ, i, j, nx, a.idx) + d_ux_dz<T>(HX, i, j, nz, ft, a.idz)) / T(2);
+          const T exza = (duzx + duxz) / T(2);
+          g[4 * N + c] = c_g[4] + T(2) * (exz * exza + exz * exza);
+        }
+      }
+    }
+    if (tx < 3) {
+      T vxx, vzz, d0, d1;
+      normal(hxx_x, j, c_lh, c_mh, false, vxx, vzz, d0, d1);
+      rsxx[slot(j, 3)][hxx_x - (I1 - 1)] = vxx;
+      T vxz, d2, d3;
+      shear(hxz_x, j, c_mhhh, false, vxz, d2, d3);
+      rsxz[slot(j, 5)][hxz_x - (I1 - 2)] = vxz;
+    }
+    __syncthreads();
+
+    // ---- displacements of row jd = j-2 (update_ux!/update_uz!, :1-37) ---
+    const int jd = j - 2;
+    if (jd >= Zd0 && jd < Zd1 && i <= nx) {
+      const int64_t c = (int64_t)(jd - 1) * nx + (i - 1);
+      const bool hx = i <= nx - 1, hz = jd <= nz - 1;
+      T curx = 0, curz = 0, nux = 0, nuz = 0;
+      if (hx) {
+        const T d1 = d_sxx_dx<T>(SXX, i, jd, nx, a.idx);
+        const T d2 = d_sxz_dz<T>(SXZ, i, jd, nz, ft, a.idz);
+        T c1 = d1, c2 = d2;
+        int k = strip_full(i, nx, h);
+        if (k) {
+          const int64_t q = a.psioff[0] + (k - 1) + (int64_t)(jd - 1) * (2 * h);
+          const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * d1;
+          psn[q] = pn;
+          c1 = d1 + pn;
+        }
+        k = strip_half(jd - 1, nz - 1, h);
+        if (k) {
+          const int64_t q = a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
+          const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * d2;
+          psn[q] = pn;
+          c2 = d2 + pn;
+        }
+        curx = UX(i, jd);
+        nux = T(2) * curx - c_ox + c_rx * (c1 + c2);
+      }
+      if (hz) {
+        const T d1 = d_sxz_dx<T>(SXZ, i, jd, nx, a.idx);
+        const T d2 = d_szz_dz<T>(SZZ, i, jd, nz, ft, a.idz);
+        T c1 = d1, c2 = d2;
+        int k = strip_half(i - 1, nx - 1, h);
+        if (k) {
+          const int64_t q = a.psioff[1] + (k - 1) + (int64_t)(jd - 1) * (2 * h + 2);
+          const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * d1;
+          psn[q] = pn;
+          c1 = d1 + pn;
+        }
+        k = strip_full(jd, nz, h);
+        if (k) {
+          const int64_t q = a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx;
+          const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * d2;
+          psn[q] = pn;
+          c2 = d2 + pn;
+        }
+        curz = UZ(i, jd);
+        nuz = T(2) * curz - c_oz + c_rz * (c1 + c2);
+      }
+      if (a.kind == 2) {
+        // point forces (forward) or adjoint residuals, in point order
+        const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+        if (hx) {
+          const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, jd - 1, c);
+          if (u >= 0)
+            for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++)
+              nux += a.f0.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f0.epoint[e])] * c_rx;
+        }
+        if (hz) {
+          const int u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, jd - 1, c);
+          if (u >= 0)
+            for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++)
+              nuz += a.f1.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f1.epoint[e] + 1)] * c_rz;
+        }
+      }
+      if (hx) un[c] = nux;
+      if (hz) un[N + c] = nuz;
+      if (!ADJ && a.hmode && a.it > sb) {
+        T *hs = a.hist + (int64_t)s * 2 * N + c;
+        const int sl = a.it - sb + 1;
+        hs[(int64_t)sl * a.hslot] = nux;
+        hs[(int64_t)sl * a.hslot + N] = nuz;
+        if (a.hmode == 1 && a.it == sb + 1) {
+          hs[0] = c_ox;
+          hs[N] = c_oz;
+          hs[a.hslot] = curx;
+          hs[a.hslot + N] = curz;
+        }
+      }
+    }
+  }
 }
 
 // traces[it, comp, r] = sum over taps of coef * u (record_receivers2D_* and
@@ -533,7 +970,7 @@ class El2dEngine final : public Engine {
     model_.alloc(5 * N_ * e);
     u_.alloc(2 * ns_ * 2 * N_ * e);
     sig_.alloc(ns_ * 3 * N_ * e);
-    psi_.alloc(ns_ * psi_stride_ * e);
+    psi_.alloc(2 * ns_ * psi_stride_ * e);   // two step parities
     cpml_.alloc((int64_t)ns_ * cstride_ * e);
     state_bytes_ = model_.bytes() + u_.bytes() + sig_.bytes() + psi_.bytes();
     if (d.gradient) {
@@ -544,7 +981,7 @@ class El2dEngine final : public Engine {
       gout_.alloc(ns_ * 5 * N_ * e);
       state_bytes_ += au_.bytes() + asig_.bytes() + apsi_.bytes() + grad_.bytes() + gout_.bytes();
       const int64_t slot = ns_ * 2 * N_ * (int64_t)e;
-      ck_bytes_one_ = (int64_t)u_.bytes() + (int64_t)psi_.bytes();
+      ck_bytes_one_ = (int64_t)u_.bytes() + (int64_t)psi_.bytes() / 2;
       const int64_t avail = (int64_t)c_.budget - state_bytes_ - 2 * slot - 2 * ck_bytes_one_;
       S_ = choose_segment(d.nt_max, d.check_freq, slot, ck_bytes_one_, avail);
       // check_freq = 1 keeps every step (no replay); otherwise a checkpoint
@@ -636,6 +1073,24 @@ class El2dEngine final : public Engine {
     build_taps<T>(taps_[2], nshots, nrec.data(), f[2].data(), nx_, nz_, nx_ - 1, nz_, "Receiver", c_.stream);
     build_taps<T>(taps_[3], nshots, nrec.data(), f[3].data(), nx_, nz_, nx_, nz_ - 1, "Receiver", c_.stream);
     maxrec_ = *std::max_element(nrec.begin(), nrec.end());
+    // moment-tensor taps on normal-stress cells the stencil never updates
+    edge_taps_ = false;
+    max_ucell0_ = 0;
+    if (mt) {
+      for (int s = 0; s < nshots; s++) {
+        std::vector<int64_t> cells;
+        const swx_taps &f = shots[s].src[0];
+        for (int k = 0; k < f.ptr[nsrc[s]]; k++) {
+          const int64_t i = f.ij[2 * (int64_t)k], j = f.ij[2 * (int64_t)k + 1];
+          cells.push_back((i - 1) + (j - 1) * (int64_t)nx_);
+          const bool nrm = i >= 2 && i <= nx_ - 1 && j >= (c_.d.freetop ? 1 : 2) && j <= nz_ - 1;
+          edge_taps_ |= !nrm;
+        }
+        std::sort(cells.begin(), cells.end());
+        cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+        max_ucell0_ = std::max<int>(max_ucell0_, (int)cells.size());
+      }
+    }
     auto up = [&](DevBuf &b, const void *src, size_t bytes) {
       b.alloc(std::max<size_t>(bytes, 16));
       if (bytes) SWX_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, c_.stream));
@@ -671,6 +1126,10 @@ class El2dEngine final : public Engine {
 
   void forward(bool save) override {
     SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
+    if (var_fwd_ < 0) {
+      // tuning launches run on the state before it is cleared below
+      var_fwd_ = tune("SWX_EL2D_VARIANT_FWD", [&](int v) { step_fwd_v(v, 1, 0, 0, 0, nullptr); });
+    }
     zero_forward_state(true);
     if (save) SWX_HIP(hipMemsetAsync(hist_.get(), 0, hist_.bytes(), c_.stream));
     T *tr = c_.pts.d_traces.as<T>();
@@ -684,6 +1143,10 @@ class El2dEngine final : public Engine {
   }
 
   void adjoint(bool per_shot) override {
+    if (var_adj_ < 0) {
+      const int cbt = (int)(ceil_div(ntmax_, S_) - 1) * S_;
+      var_adj_ = tune("SWX_EL2D_VARIANT_ADJ", [&](int v) { step_adj_v(v, ntmax_, cbt); });
+    }
     SWX_HIP(hipMemsetAsync(au_.get(), 0, au_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(asig_.get(), 0, asig_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(apsi_.get(), 0, apsi_.bytes(), c_.stream));
@@ -723,7 +1186,6 @@ class El2dEngine final : public Engine {
       ElArgs<T> a = base_args(cb);
       a.ucur = U(au_, cb + 1);
       a.sig = asig_.as<T>();
-      a.psi = apsi_.as<T>();
       a.update = 0;
       a.corr_step = cb + 1;
       a.corr_lim = cb + S_;
@@ -754,6 +1216,7 @@ class El2dEngine final : public Engine {
     }
   }
 
+  void variants(int32_t v[2]) const override { v[0] = var_fwd_; v[1] = var_adj_; }
   int64_t bytes_state() const override { return state_bytes_; }
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return hist_.bytes(); }
@@ -761,6 +1224,8 @@ class El2dEngine final : public Engine {
 
  private:
   T *M(int k) const { return model_.as<T>() + (int64_t)k * N_; }
+  // C-PML memory variables of step parity p (all shots)
+  T *PS(const DevBuf &b, int k) const { return b.as<T>() + (int64_t)(k & 1) * ns_ * psi_stride_; }
   // time level k of a displacement buffer (slot k mod 2)
   T *U(const DevBuf &b, int k) const { return b.as<T>() + (int64_t)(((k % 2) + 2) % 2) * ns_ * 2 * N_; }
 
@@ -810,11 +1275,97 @@ class El2dEngine final : public Engine {
   }
 
   void step_fwd(int it, int act_min, int hmode, int cbase, T *traces) {
+    step_fwd_v(var_fwd_, it, act_min, hmode, cbase, traces);
+  }
+  void step_adj(int it, int cbase) { step_adj_v(var_adj_, it, cbase); }
+
+  // z-marching launch shape: strips of 64*W columns, chunks sized for
+  // about 2048 blocks (8 per CU) when the grid allows it
+  unsigned grid_march(ElArgs<T> &a, int w) const {
+    a.nbx = (int)ceil_div(nx_, 64 * w);
+    const int64_t per = (int64_t)a.nbx * nsh_;
+    const int nbz = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, per), ceil_div(nz_, 32)));
+    a.ch = (int)ceil_div(nz_, nbz);
+    a.nbz = (int)ceil_div(nz_, a.ch);
+    return (unsigned)((int64_t)a.nbx * a.nbz * nsh_);
+  }
+  template <bool ADJ>
+  void launch_march(ElArgs<T> a, int v, int kind) {
+    const unsigned g = grid_march(a, v - 100);
+    c_.ktimer.begin(c_.stream);
+    if (v == 102) el_march_kernel<T, 2, ADJ><<<g, 128, 0, c_.stream>>>(a);
+    else el_march_kernel<T, 4, ADJ><<<g, 256, 0, c_.stream>>>(a);
+    c_.ktimer.end(c_.stream, kind);
+    SWX_HIP(hipGetLastError());
+  }
+
+  // Variants: 0 = stress and displacement launches, 102/104 = one
+  // z-marching launch on 128/256-column strips.  The tuner times the
+  // candidates (only 0 unless kMarchInTuner); the environment can pin one.
+  static constexpr bool kMarchInTuner = false;
+  template <typename F>
+  int tune(const char *env, F step) {
+    std::vector<int> cands = {0};
+    if (kMarchInTuner) {
+      cands.push_back(102);
+      cands.push_back(104);
+    }
+    if (const char *v = std::getenv(env)) {
+      const int t = std::atoi(v);
+      if (t == 0 || t == 102 || t == 104) return t;
+    }
+    if (cands.size() == 1) return cands[0];
+    hipEvent_t e0, e1;
+    SWX_HIP(hipEventCreate(&e0));
+    SWX_HIP(hipEventCreate(&e1));
+    int best = cands.front();
+    float bestms = 1e30f;
+    for (int c : cands) {
+      step(c);
+      SWX_HIP(hipEventRecord(e0, c_.stream));
+      for (int r = 0; r < 3; r++) step(c);
+      SWX_HIP(hipEventRecord(e1, c_.stream));
+      SWX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < bestms) { bestms = ms; best = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+  }
+
+  void step_fwd_v(int v, int it, int act_min, int hmode, int cbase, T *traces) {
+    if (v != 0) {
+      ElArgs<T> a = base_args(it);
+      a.ucur = U(u_, it - 1);
+      a.unew = U(u_, it);
+      a.sig = sig_.as<T>();
+      a.psi_o = PS(psi_, it - 1);
+      a.psi_n = PS(psi_, it);
+      a.act_min = act_min;
+      a.f0 = taps_[0].view<T>();
+      a.f1 = taps_[1].view<T>();
+      a.tf = c_.pts.d_srctf.as<T>();
+      a.ncomp = 2;
+      a.kind = kind_ == SWX_SRC_MOMENT_TENSOR ? 1 : 2;
+      a.hmode = c_.d.gradient ? hmode : 0;
+      a.cbase = cbase;
+      if (a.kind == 1 && edge_taps_) {
+        dim3 g((unsigned)ceil_div(std::max(max_ucell0_, 1), 64), (unsigned)nsh_);
+        el_edge_inject_kernel<T><<<g, 64, 0, c_.stream>>>(a);
+        SWX_HIP(hipGetLastError());
+      }
+      launch_march<false>(a, v, 0);
+      record(it, traces);
+      return;
+    }
     ElArgs<T> a = base_args(it);
     a.ucur = U(u_, it - 1);
     a.unew = U(u_, it);
     a.sig = sig_.as<T>();
-    a.psi = psi_.as<T>();
+    a.psi_o = PS(psi_, it - 1);
+    a.psi_n = PS(psi_, it);
     a.act_min = act_min;
     a.f0 = taps_[0].view<T>();
     a.f1 = taps_[1].view<T>();
@@ -826,22 +1377,45 @@ class El2dEngine final : public Engine {
     a.hmode = c_.d.gradient ? hmode : 0;
     a.cbase = cbase;
     launch_disp(a, 0);
-    if (traces) {
-      const int thr = 2 * maxrec_;
-      dim3 g((unsigned)ceil_div(thr, 128), (unsigned)nsh_);
-      el_record_kernel<T><<<g, 128, 0, c_.stream>>>(U(u_, it), N_, taps_[2].view<T>(), taps_[3].view<T>(),
-                                                     c_.pts.d_nt.as<int>(), c_.pts.d_off_tr.as<int64_t>(), traces,
-                                                     it, maxrec_);
-      SWX_HIP(hipGetLastError());
-    }
+    record(it, traces);
   }
 
-  void step_adj(int it, int cbase) {
+  void record(int it, T *traces) {
+    if (!traces) return;
+    const int thr = 2 * maxrec_;
+    dim3 g((unsigned)ceil_div(thr, 128), (unsigned)nsh_);
+    el_record_kernel<T><<<g, 128, 0, c_.stream>>>(U(u_, it), N_, taps_[2].view<T>(), taps_[3].view<T>(),
+                                                   c_.pts.d_nt.as<int>(), c_.pts.d_off_tr.as<int64_t>(), traces, it,
+                                                   maxrec_);
+    SWX_HIP(hipGetLastError());
+  }
+
+  void step_adj_v(int v, int it, int cbase) {
+    if (v != 0) {
+      ElArgs<T> a = base_args(it);
+      a.ucur = U(au_, it + 1);
+      a.unew = U(au_, it);
+      a.sig = asig_.as<T>();
+      a.psi_o = PS(apsi_, it + 1);
+      a.psi_n = PS(apsi_, it);
+      a.cbase = cbase;
+      a.corr_step = it + 1;
+      a.corr_lim = cbase + S_;
+      a.kind = 2;
+      a.f0 = taps_[2].view<T>();
+      a.f1 = taps_[3].view<T>();
+      a.tf = c_.pts.d_res.as<T>();
+      a.off_tf = c_.pts.d_off_tr.as<int64_t>();
+      a.ncomp = 2;
+      launch_march<true>(a, v, 1);
+      return;
+    }
     ElArgs<T> a = base_args(it);
     a.ucur = U(au_, it + 1);
     a.unew = U(au_, it);
     a.sig = asig_.as<T>();
-    a.psi = apsi_.as<T>();
+    a.psi_o = PS(apsi_, it + 1);
+    a.psi_n = PS(apsi_, it);
     a.cbase = cbase;
     a.corr_step = it + 1;
     a.corr_lim = cbase + S_;
@@ -862,7 +1436,8 @@ class El2dEngine final : public Engine {
   void ckpt_copy(int k, bool save) {
     char *base = ck_.as<char>() + (int64_t)(k - 1) * ck_bytes_one_;
     struct Part { void *dev; int64_t bytes; };
-    Part parts[2] = {{u_.get(), (int64_t)u_.bytes()}, {psi_.get(), (int64_t)psi_.bytes()}};
+    Part parts[2] = {{u_.get(), (int64_t)u_.bytes()},
+                     {PS(psi_, k * S_), (int64_t)psi_.bytes() / 2}};
     int64_t off = 0;
     for (auto &p : parts) {
       if (save)
@@ -883,6 +1458,9 @@ class El2dEngine final : public Engine {
   DevBuf mom_, offm_;
   TapTable taps_[4];
   int kind_ = SWX_SRC_MOMENT_TENSOR, maxrec_ = 0;
+  bool edge_taps_ = false;
+  int max_ucell0_ = 0;
+  int var_fwd_ = -1, var_adj_ = -1;
   int64_t ck_bytes_one_ = 0, state_bytes_ = 0;
   int S_ = 1, K_ = 1, nck_ = 0;
   bool store_all_ = true;

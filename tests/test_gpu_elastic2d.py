@@ -154,3 +154,32 @@ def test_point_edge_cases(oracle_mod):
         assert rel_l2(shot.recs.seismograms, sref[0]) <= 1e-10
         for k in ("rho", "lambda", "mu"):
             assert rel_l2(g[k], gref[k]) <= 1e-10, (sinc, k)
+
+
+@pytest.mark.parametrize("kind,freetop", [("mt", True), ("ef", False)])
+def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
+    """Every forward and adjoint kernel variant the autotuner may pick (the
+    two-kernel stress/displacement step, z-marching strips of 128/256
+    columns) gives the same seismograms and gradients bit for bit, on a grid
+    that needs several strips and a partial last one, and matches the
+    oracle."""
+    p = _problem(np.float64, kind=kind, nts=(200, 170), n=(301, 96), freetop=freetop)
+    obs = oracle_mod.elastic_forward(*p["true"], *_oracle_args(p), p["osh"])
+    mis = [sw.L2Misfit(o) for o in obs]
+    mp = _matprop(p["init"], np.float64)
+    outs = []
+    for v, w in (("0", "0"), ("102", "104"), ("104", "102"), ("0", "104")):
+        print(f"[el2d kernel variants {kind} forward {v} adjoint {w}]", flush=True)
+        monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", v)
+        monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", w)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=23))
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
+    for g, seis in outs[1:]:
+        for k in ("rho", "lambda", "mu"):
+            assert np.array_equal(g[k], outs[0][0][k]), k
+        assert all(np.array_equal(a, b) for a, b in zip(seis, outs[0][1]))
+    osh = [dict(o, observed=ob) for o, ob in zip(p["osh"], obs)]
+    gref, _, _ = oracle_mod.elastic_gradient(*p["init"], *_oracle_args(p), osh, check_freq=23)
+    for k in ("rho", "lambda", "mu"):
+        assert rel_l2(outs[-1][0][k], gref[k]) <= 1e-10, k
