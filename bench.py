@@ -188,11 +188,24 @@ def _strip_cells(nx, nz, halo):
     return (2 * halo * nz + 2 * halo * nx) + ((2 * halo + 2) * nz + (2 * halo + 2) * nx)
 
 
+def _launches_two_step(first, nt, ntmax, S, checkpoints):
+    """Launches of the two-step forward schedule that touch a shot of nt
+    steps: pairs (it, it+1) from `first` on, except a single step where it
+    is a checkpoint level or it+1 would pass ntmax."""
+    n, it = 0, first
+    while it <= min(nt, ntmax):
+        two = (not checkpoints or it % S != 0) and it + 1 <= ntmax
+        n += 1
+        it += 2 if two else 1
+    return n
+
+
 def alg_bytes(physics, nx, nz, halo, e, nts, S, variant=(0, 0)):
     """Returns {kernel family: (total bytes of one gradient, launches)} for
     family 0 (forward/replay step kernel) and 1 (adjoint step kernel);
     `variant` holds the kernel variants the plan picked."""
     N = nx * nz
+    ntmax = max(nts)
     strip = _strip_cells(nx, nz, halo) * 2 * e
     fwd_b = fwd_n = adj_b = adj_n = 0
     for nt in nts:
@@ -221,7 +234,15 @@ def alg_bytes(physics, nx, nz, halo, e, nts, S, variant=(0, 0)):
             adj_b += nt * (sa + 16) * e * N
             adj_n += (1 if march_a else 2) * nt
             continue
-        if physics == "vd2d":
+        if physics == "vd2d" and variant[0] >= 200:
+            # two steps per launch: a launch reads p, vx, vz, m0, m1x, m1z and
+            # writes p, vx, vz once (9e) whether it advances the shot one or two
+            # steps; + w p-history (1e) per stored step
+            nl_f = _launches_two_step(1, nt, ntmax, S, checkpoints=True)
+            nl_r = (nt // S) * ((S - 1 + 1) // 2)
+            fwd_b += (nl_f + nl_r) * (9 * e * N + strip) + (last + rep) * e * N
+            fwd_n += nl_f + nl_r
+        elif physics == "vd2d":
             # forward: r p, vx, vz, m0, m1x, m1z; w p, vx, vz (9e); + w p-history (10e)
             fwd_b += (nt - last) * 9 * e * N + last * 10 * e * N + nt * strip
             fwd_b += rep * 10 * e * N + rep * strip
@@ -246,6 +267,8 @@ def kernel_names(physics, variant):
     picked (variant codes of swx_plan_info.kernel_variant)."""
     if physics == "vd2d":
         def name(kind, v):
+            if v >= 200:
+                return f"vd2d_{kind}2_kernel<float, {v - 200}, {512 if v - 200 > 8 else 256}>"
             return f"vd2d_{kind}_march<float, {v - 100}>" if v > 100 else f"vd2d_{kind}_kernel<float, {v}>"
         return name("fwd", variant[0]), name("adj", variant[1])
     if physics == "el2d":
@@ -406,6 +429,12 @@ def main():
     ap.add_argument("--workload", default="vd2d_batched", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-one-shot", action="store_true", help="skip the plain 1-shot path")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="time only the forward pass with checkpointing (for counter-collection runs): no "
+                         "observed-data pass, no adjoint, no roofline")
+    ap.add_argument("--timed-steps-only", action="store_true",
+                    help="stop after the timed steps (for counter-collection runs, where every dispatch is "
+                         "instrumented): no per-launch timing pass, no roofline")
     ap.add_argument("--one-shot-sample", type=int, default=2, help="shots timed on the 1-shot path")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -420,7 +449,20 @@ def main():
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
     log("workload", wl["name"])
     # observed data from the true model (outside the timed region)
-    sw.swforward(p, wl["mp_true"], wl["shots"], runparams=sw.RunParameters(**wl["run"]))
+    if args.forward_only:
+        runs = [Runner(wl, g) for g in wl.get("groups", [wl["shots"]])]
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for r in runs:
+                r.plan.forward(r.traces, True)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        for r in runs:
+            r.close()
+        print(json.dumps({"workload": wl["name"], "forward_only": True, "steps": args.steps,
+                          "ms_per_forward": round(ms, 3)}))
+        return
+    for g in wl.get("groups", [wl["shots"]]):
+        sw.swforward(p, wl["mp_true"], g, runparams=sw.RunParameters(**wl["run"]))
     log("observed data ready")
     # one plan per call: the elastic workload runs its two source kinds as
     # two calls, as the reference requires
@@ -437,6 +479,11 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     log(f"timed {args.steps} steps: {ms_step:.1f} ms/step")
+    if args.timed_steps_only:
+        for r in runs:
+            r.close()
+        print(json.dumps({"workload": wl["name"], "steps": args.steps, "ms_per_step": round(ms_step, 3)}))
+        return
     kms, kn = [0.0, 0.0], [0, 0]
     dev_ms = 0.0
     for r in runs:

@@ -284,6 +284,7 @@ class Cd2dEngine final : public Engine {
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
     for (int it = 1; it <= ntmax_; it++) {
+      progress("forward step", it, ntmax_);
       step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
       if (save && it % S_ == 0) {
         const int k = it / S_;
@@ -303,6 +304,7 @@ class Cd2dEngine final : public Engine {
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
     const int Kmax = (int)ceil_div(ntmax_, S_);
     for (int k = Kmax - 1; k >= 0; k--) {
+      progress("adjoint segment", Kmax - k, Kmax);
       const int cb = k * S_;
       // replay for shots whose segment k is not their last one
       bool need = false;

@@ -316,6 +316,7 @@ class Cd3dEngine final : public Engine {
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
     for (int it = 1; it <= ntmax_; it++) {
+      progress("forward step", it, ntmax_);
       step_fwd(it, 0, save ? 1 : 0, 0, tr);
       if (save && it % S_ == 0) {
         const int k = it / S_;
@@ -333,6 +334,7 @@ class Cd3dEngine final : public Engine {
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
     const int Kmax = (int)ceil_div(ntmax_, S_);
     for (int k = Kmax - 1; k >= 0; k--) {
+      progress("adjoint segment", Kmax - k, Kmax);
       const int cb = k * S_;
       bool need = false;
       for (int s = 0; s < nsh_; s++) need |= nt_[s] > cb + S_;

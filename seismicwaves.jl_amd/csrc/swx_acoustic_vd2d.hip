@@ -284,6 +284,280 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   }
 }
 
+// Two forward / replay steps in one launch: (p, vx, vz)^{it-1} ->
+// (p, vx, vz)^{it+1}, with the intermediate level it kept on chip.
+//
+// A block owns a TX x TZ tile.  The velocities of step it+1 on the tile
+// need the pressure p^{it+1} one cell left / two right (and up / down),
+// which needs v^{it} three cells out, which needs p^{it} four cells out,
+// which needs v^{it-1} six cells out.  So the block loads v^{it-1} and
+// p^{it-1} on the enlarged rectangles, computes p^{it} on P1 = tile + 4
+// (left/up) / 5 (right/down) in LDS, then v^{it} in place on its
+// rectangles, then p^{it+1} in place on P2 = tile + 1 / 2, then v^{it+1}
+// on the tile, which goes to HBM with p^{it+1} and the C-PML variables.
+// Every cell update is the expression of vd2d_fwd_kernel (same operand
+// order), so the results are bitwise those of two single steps.  C-PML
+// memory variables are pointwise recurrences: a thread keeps the level-it
+// value of each of its cells in a register and the same thread updates that
+// cell again in the second step (P2 and the tile are walked in the P1 and
+// v^{it} enumerations).  Per step the HBM traffic is about half that of a
+// single-step launch; the redundant halo work is ~1.3x.
+//
+// A shot whose last step is it (nt_s == it) stores level it and stops after
+// the first step.  The state is read from one buffer and written to the
+// other, so the level written is it+1 (it for such a shot).
+template <typename T, int TZ, int NB>
+__global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
+  constexpr int PX = TX + 9, PZ = TZ + 9;     // p:  x [I0-4, I0+TX+5), z [J0-4, J0+TZ+5)
+  constexpr int AX = TX + 12, AZ = TZ + 9;    // vx: x [I0-6, I0+TX+6), z [J0-4, J0+TZ+5)
+  constexpr int BX = TX + 9, BZ = TZ + 12;    // vz: x [I0-4, I0+TX+5), z [J0-6, J0+TZ+6)
+  constexpr int X1X = TX + 6, X1Z = TZ + 3;   // vx^{it}: x [I0-3, I0+TX+3), z [J0-1, J0+TZ+2)
+  constexpr int Z1X = TX + 3, Z1Z = TZ + 6;   // vz^{it}: x [I0-1, I0+TX+2), z [J0-3, J0+TZ+3)
+  constexpr int NP = (PX * PZ + NB - 1) / NB;
+  constexpr int NVX = (X1X * X1Z + NB - 1) / NB, NVZ = (Z1X * Z1Z + NB - 1) / NB;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const bool two = a.it + 1 <= nts;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int64_t so = (int64_t)s * a.N;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+
+  __shared__ T s_vx[AZ][AX];
+  __shared__ T s_vz[BZ][BX];
+  __shared__ T s_p[PZ][PX];
+  __shared__ int s_pts[2];
+  if (tid == 0) {
+    const int *sr = a.pt.srow + (int64_t)s * (a.pt.nrows + 1);
+    const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+    s_pts[0] = sr[max(J0 - 4, 0)] < sr[min(J0 + TZ + 5, nz)];
+    s_pts[1] = a.traces != nullptr && rr[min(J0, nz)] < rr[min(J0 + TZ, nz)];
+  }
+
+  // ---- loads: v^{it-1}, p^{it-1} into LDS; factors and C-PML variables of
+  // this thread's cells into registers ------------------------------------
+  for (int q = tid; q < AZ * AX; q += NB) {
+    const int lz = q / AX, lx = q % AX;
+    const int gi = I0 - 6 + lx, gj = J0 - 4 + lz;
+    T v = 0;
+    if (gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) v = a.vx_o[so + (int64_t)gj * nx + gi];
+    s_vx[lz][lx] = v;
+  }
+  for (int q = tid; q < BZ * BX; q += NB) {
+    const int lz = q / BX, lx = q % BX;
+    const int gi = I0 - 4 + lx, gj = J0 - 6 + lz;
+    T v = 0;
+    if (gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
+    s_vz[lz][lx] = v;
+  }
+  T rm0[NP], rxx[NP], rxz[NP];
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    const int q = tid + k * NB;
+    const int pz = q / PX, px = q % PX;
+    const int gi = I0 - 4 + px, gj = J0 - 4 + pz;
+    T vp = 0, vm = 0, x0 = 0, x1 = 0;
+    if (q < PX * PZ && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) {
+      const int64_t c = (int64_t)gj * nx + gi;
+      vp = a.p_o[so + c];
+      vm = a.m0[c];
+      if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= nz - 2) {
+        if (gi <= h || gi >= nx - h - 1) {
+          const int kk = gi <= h ? gi : gi - nx + 2 * h + 2;
+          x0 = a.xix_o[(int64_t)s * a.sxx + (int64_t)gj * (2 * h + 2) + kk];
+        }
+        if (gj <= h || gj >= nz - h - 1) {
+          const int kk = gj <= h ? gj : gj - nz + 2 * h + 2;
+          x1 = a.xiz_o[(int64_t)s * a.sxz + (int64_t)kk * nx + gi];
+        }
+      }
+    }
+    if (q < PX * PZ) s_p[pz][px] = vp;
+    rm0[k] = vm;
+    rxx[k] = x0;
+    rxz[k] = x1;
+  }
+  T rm1x[NVX], rpx[NVX];
+#pragma unroll
+  for (int k = 0; k < NVX; k++) {
+    const int q = tid + k * NB;
+    const int gi = I0 - 3 + q % X1X, gj = J0 - 1 + q / X1X;
+    T vm = 0, ps = 0;
+    if (q < X1X * X1Z && gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) {
+      vm = a.m1x[(int64_t)gj * nx + gi];
+      if (gi <= h - 1 || gi >= nx - h - 1) {
+        const int kk = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
+        ps = a.psx_o[(int64_t)s * a.spx + (int64_t)gj * (2 * h) + kk];
+      }
+    }
+    rm1x[k] = vm;
+    rpx[k] = ps;
+  }
+  T rm1z[NVZ], rpz[NVZ];
+#pragma unroll
+  for (int k = 0; k < NVZ; k++) {
+    const int q = tid + k * NB;
+    const int gi = I0 - 1 + q % Z1X, gj = J0 - 3 + q / Z1X;
+    T vm = 0, ps = 0;
+    if (q < Z1X * Z1Z && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) {
+      vm = a.m1z[(int64_t)gj * nx + gi];
+      if (gj <= h - 1 || gj >= nz - h - 1) {
+        const int kk = gj <= h - 1 ? gj : gj + 1 - nz + 2 * h;
+        ps = a.psz_o[(int64_t)s * a.spz + (int64_t)kk * nx + gi];
+      }
+    }
+    rm1z[k] = vm;
+    rpz[k] = ps;
+  }
+  __syncthreads();
+
+  const bool any_src = s_pts[0] != 0, any_rec = s_pts[1] != 0;
+  // pressure update of step `st` at this thread's P1 cells (inner: P2 only),
+  // in place in s_p (update_p_CPML! + inject_sources!, then the tile's
+  // recording, history and, when `fin`, the final state)
+  auto p_phase = [&](int st, bool inner, bool fin) {
+    const int sb = store_base(a.smode, st, nts, a.S, a.cbase);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int q = tid + k * NB;
+      const int pz = q / PX, px = q % PX;
+      if (q >= PX * PZ) continue;
+      if (inner && (px < 3 || px >= TX + 6 || pz < 3 || pz >= TZ + 6)) continue;
+      const int gi = I0 - 4 + px, gj = J0 - 4 + pz;
+      if (gi < 0 || gi >= nx || gj < 0 || gj >= nz) continue;   // stays 0
+      const int64_t c = (int64_t)gj * nx + gi;
+      const T pold = s_p[pz][px];
+      T pv = pold;
+      if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= nz - 2) {
+        T dvx = d4(s_vx[pz][px], s_vx[pz][px + 1], s_vx[pz][px + 2], s_vx[pz][px + 3]) * a.idx;
+        if (gi <= h || gi >= nx - h - 1) {
+          const int kk = gi <= h ? gi : gi - nx + 2 * h + 2;
+          const T xi = cx.b[kk] * rxx[k] + cx.a[kk] * dvx;
+          rxx[k] = xi;
+          dvx = dvx + xi;
+        }
+        T dvz = d4(s_vz[pz][px], s_vz[pz + 1][px], s_vz[pz + 2][px], s_vz[pz + 3][px]) * a.idz;
+        if (gj <= h || gj >= nz - h - 1) {
+          const int kk = gj <= h ? gj : gj - nz + 2 * h + 2;
+          const T xi = cz.b[kk] * rxz[k] + cz.a[kk] * dvz;
+          rxz[k] = xi;
+          dvz = dvz + xi;
+        }
+        pv = pv - rm0[k] * (dvx + dvz);
+      }
+      const int u = any_src ? find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, gj, c) : -1;
+      if (u >= 0) {
+        const T *tf = a.srctf + a.pt.off_tf[s] + (st - 1);
+        for (int e = a.pt.usrc_ptr[u]; e < a.pt.usrc_ptr[u + 1]; e++) pv += tf[(int64_t)nts * a.pt.usrc_list[e]];
+      }
+      s_p[pz][px] = pv;
+      const bool owned = px >= 4 && px < TX + 4 && pz >= 4 && pz < TZ + 4;
+      if (owned) {
+        if (sb >= 0) {
+          a.pbuf[(int64_t)(st - sb) * a.pslot + so + c] = pv;
+          if (a.smode == 1 && st == sb + 1) a.pbuf[so + c] = pold;
+        }
+        if (any_rec) {
+          const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, gj, c);
+          if (r >= 0) {
+            T *tr = a.traces + a.pt.off_tr[s] + (st - 1);
+            for (int e = a.pt.urec_ptr[r]; e < a.pt.urec_ptr[r + 1]; e++) tr[(int64_t)nts * a.pt.urec_list[e]] = pv;
+          }
+        }
+        if (fin) {
+          a.p_n[so + c] = pv;
+          if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= nz - 2) {
+            if (gi <= h || gi >= nx - h - 1) {
+              const int kk = gi <= h ? gi : gi - nx + 2 * h + 2;
+              a.xix_n[(int64_t)s * a.sxx + (int64_t)gj * (2 * h + 2) + kk] = rxx[k];
+            }
+            if (gj <= h || gj >= nz - h - 1) {
+              const int kk = gj <= h ? gj : gj - nz + 2 * h + 2;
+              a.xiz_n[(int64_t)s * a.sxz + (int64_t)kk * nx + gi] = rxz[k];
+            }
+          }
+        }
+      }
+    }
+  };
+  // velocity update at this thread's v^{it} cells (inner: the tile only),
+  // in place in s_vx / s_vz, or to HBM when `fin`
+  auto v_phase = [&](bool inner, bool fin) {
+#pragma unroll
+    for (int k = 0; k < NVX; k++) {
+      const int q = tid + k * NB;
+      const int az = q / X1X, ax = q % X1X;
+      if (q >= X1X * X1Z) continue;
+      if (inner && (ax < 3 || ax >= TX + 3 || az < 1 || az >= TZ + 1)) continue;
+      const int gi = I0 - 3 + ax, gj = J0 - 1 + az;
+      if (gi < 0 || gi > nx - 2 || gj < 0 || gj > nz - 1) continue;   // stays 0
+      T d = d4(s_p[az + 3][ax], s_p[az + 3][ax + 1], s_p[az + 3][ax + 2], s_p[az + 3][ax + 3]) * a.idx;
+      const bool strip = gi <= h - 1 || gi >= nx - h - 1;
+      const int kk = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
+      if (strip) {
+        const T ps = cx.bh[kk] * rpx[k] + cx.ah[kk] * d;
+        rpx[k] = ps;
+        d = d + ps;
+      }
+      const T v = s_vx[az + 3][ax + 3] - rm1x[k] * d;
+      const bool owned = ax >= 3 && ax < TX + 3 && az >= 1 && az < TZ + 1;
+      if (fin) {
+        if (owned) {
+          const int64_t c = (int64_t)gj * nx + gi;
+          a.vx_n[so + c] = v;
+          if (strip) a.psx_n[(int64_t)s * a.spx + (int64_t)gj * (2 * h) + kk] = rpx[k];
+        }
+      } else {
+        s_vx[az + 3][ax + 3] = v;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NVZ; k++) {
+      const int q = tid + k * NB;
+      const int bz2 = q / Z1X, bx2 = q % Z1X;
+      if (q >= Z1X * Z1Z) continue;
+      if (inner && (bx2 < 1 || bx2 >= TX + 1 || bz2 < 3 || bz2 >= TZ + 3)) continue;
+      const int gi = I0 - 1 + bx2, gj = J0 - 3 + bz2;
+      if (gi < 0 || gi > nx - 1 || gj < 0 || gj > nz - 2) continue;   // stays 0
+      T d = d4(s_p[bz2][bx2 + 3], s_p[bz2 + 1][bx2 + 3], s_p[bz2 + 2][bx2 + 3], s_p[bz2 + 3][bx2 + 3]) * a.idz;
+      const bool strip = gj <= h - 1 || gj >= nz - h - 1;
+      const int kk = gj <= h - 1 ? gj : gj + 1 - nz + 2 * h;
+      if (strip) {
+        const T ps = cz.bh[kk] * rpz[k] + cz.ah[kk] * d;
+        rpz[k] = ps;
+        d = d + ps;
+      }
+      const T v = s_vz[bz2 + 3][bx2 + 3] - rm1z[k] * d;
+      const bool owned = bx2 >= 1 && bx2 < TX + 1 && bz2 >= 3 && bz2 < TZ + 3;
+      if (fin) {
+        if (owned) {
+          const int64_t c = (int64_t)gj * nx + gi;
+          a.vz_n[so + c] = v;
+          if (strip) a.psz_n[(int64_t)s * a.spz + (int64_t)kk * nx + gi] = rpz[k];
+        }
+      } else {
+        s_vz[bz2 + 3][bx2 + 3] = v;
+      }
+    }
+  };
+
+  p_phase(a.it, false, !two);   // p^{it} on P1
+  __syncthreads();
+  if (!two) {
+    v_phase(true, true);        // v^{it} on the tile, to HBM
+    return;
+  }
+  v_phase(false, false);        // v^{it} on its rectangles, in LDS
+  __syncthreads();
+  p_phase(a.it + 1, true, true);   // p^{it+1} on P2
+  __syncthreads();
+  v_phase(true, true);          // v^{it+1} on the tile, to HBM
+}
+
 // Forward / replay step marching down z.  A block owns a strip of 64*W
 // columns and a chunk of rows [Z0, Z1) and walks it row by row: the new
 // pressure of row j (and of the one-left / two-right halo columns the
@@ -909,14 +1183,19 @@ class Vd2dEngine final : public Engine {
     zero(st_);
     if (!tz_fwd_) {
       const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max}),
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max, 208, 216, 232}),
                      [&](int tz) { launch_fwd(a1, tz); });
     }
-    for (int it = 1; it <= ntmax_; it++) {
-      fwd_step(it, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      if (save && it % S_ == 0) {
-        const int k = it / S_;
-        if (k >= 1 && k <= nck_) ckpt_copy(k, it, true);
+    for (int it = 1; it <= ntmax_;) {
+      progress("forward step", it, ntmax_);
+      // a two-step launch never steps over a checkpoint level
+      const int n = steps_per_launch(tz_fwd_) == 2 && it % S_ != 0 && it + 1 <= ntmax_ ? 2 : 1;
+      fwd_step(it, n, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
+      it += n;
+      const int last = it - 1;
+      if (save && last % S_ == 0) {
+        const int k = last / S_;
+        if (k >= 1 && k <= nck_) ckpt_copy(k, last, true);
       }
     }
   }
@@ -932,6 +1211,7 @@ class Vd2dEngine final : public Engine {
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
     const int64_t F = (int64_t)ns_ * N_;
     for (int k = Kmax - 1; k >= 0; k--) {
+      progress("adjoint segment", Kmax - k, Kmax);
       const int cb = k * S_;
       // shots whose checkpoint (k+1)S exists replay segment k (initrecover!/
       // recover!/getsaved, checkpointers.jl:80-105): p^{cb} from checkpoint
@@ -943,10 +1223,14 @@ class Vd2dEngine final : public Engine {
         if (k == 0) zero(st_);
         else ckpt_copy(k, cb, false);
         dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 1024), (unsigned)nsh_);
-        copy_active_shots<T><<<g, 256, 0, c_.stream>>>(pbuf_.as<T>(), F3(st_, cb & 1, 0), N_,
+        copy_active_shots<T><<<g, 256, 0, c_.stream>>>(pbuf_.as<T>(), F3(st_, slot(cb), 0), N_,
                                                        c_.pts.d_nt.as<int>(), act);
         SWX_HIP(hipGetLastError());
-        for (int it = cb + 1; it <= cb + S_ - 1; it++) fwd_step(it, act, 2, cb, nullptr);
+        for (int it = cb + 1; it <= cb + S_ - 1;) {
+          const int n = steps_per_launch(tz_fwd_) == 2 && it + 1 <= cb + S_ - 1 ? 2 : 1;
+          fwd_step(it, n, act, 2, cb, nullptr);
+          it += n;
+        }
         const T *ck1 = reinterpret_cast<const T *>(ck_.as<char>() + (int64_t)k * ck_one_);
         copy_active_shots<T><<<g, 256, 0, c_.stream>>>(pbuf_.as<T>() + (int64_t)S_ * F, ck1, N_,
                                                        c_.pts.d_nt.as<int>(), act);
@@ -997,7 +1281,14 @@ class Vd2dEngine final : public Engine {
 
   void zero(DevBuf *b) {
     for (int k = 0; k < 3; k++) SWX_HIP(hipMemsetAsync(b[k].get(), 0, b[k].bytes(), c_.stream));
+    if (b == st_) flip_ = 0;
   }
+
+  // Forward state buffer (0/1) holding time level k.  Single-step launches
+  // read level it-1 and write level it into the other buffer; a two-step
+  // launch also writes into the other buffer, so it toggles flip_.
+  int slot(int k) const { return ((k % 2) + 2 + flip_) % 2; }
+  static int steps_per_launch(int code) { return code >= 200 ? 2 : 1; }
 
   Vd2dArgs<T> args(const DevBuf *b, int it, int rpar, int wpar) const {
     Vd2dArgs<T> a{};
@@ -1053,6 +1344,14 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipGetLastError());
       return;
     }
+    if (tz == 208 || tz == 216 || tz == 232) {
+      const unsigned g = grid(a, tz - 200);
+      if (tz == 208) vd2d_fwd2_kernel<T, 8, 256><<<g, dim3(TX, 4), 0, c_.stream>>>(a);
+      else if (tz == 216) vd2d_fwd2_kernel<T, 16, 512><<<g, dim3(TX, 8), 0, c_.stream>>>(a);
+      else vd2d_fwd2_kernel<T, 32, 512><<<g, dim3(TX, 8), 0, c_.stream>>>(a);
+      SWX_HIP(hipGetLastError());
+      return;
+    }
     const unsigned g = grid(a, tz);
     if (tz == 8) vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
     else if (tz == 16) vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
@@ -1074,7 +1373,7 @@ class Vd2dEngine final : public Engine {
   }
 
   Vd2dArgs<T> fwd_args(int it, int act_min, int smode, int cbase, T *traces) const {
-    Vd2dArgs<T> a = args(st_, it, (it - 1) & 1, it & 1);
+    Vd2dArgs<T> a = args(st_, it, slot(it - 1), slot(it));
     a.act_min = act_min;
     a.smode = smode;
     a.cbase = cbase;
@@ -1087,10 +1386,18 @@ class Vd2dEngine final : public Engine {
     return a;
   }
 
-  void fwd_step(int it, int act_min, int smode, int cbase, T *traces) {
+  // n = 2: steps it and it+1 in one launch (two-step kernel variants)
+  void fwd_step(int it, int n, int act_min, int smode, int cbase, T *traces) {
     const Vd2dArgs<T> a = fwd_args(it, act_min, smode, cbase, traces);
     c_.ktimer.begin(c_.stream);
-    launch_fwd(a, tz_fwd_);
+    if (n == 2) {
+      launch_fwd(a, tz_fwd_);
+      flip_ ^= 1;
+    } else {
+      // the last step before a checkpoint (or of a replay) runs the
+      // single-step kernel of the same tile height
+      launch_fwd(a, steps_per_launch(tz_fwd_) == 2 ? tz_fwd_ - 200 : tz_fwd_);
+    }
     c_.ktimer.end(c_.stream, 0);
   }
 
@@ -1101,9 +1408,10 @@ class Vd2dEngine final : public Engine {
     c_.ktimer.end(c_.stream, 1);
   }
 
-  // Tuner candidates: the 2-D tile heights, plus the z-marching strips
-  // (codes 102/104) when kMarchInTuner is set.  Any listed code can also be
-  // pinned through the environment.
+  // Tuner candidates: the 2-D tile heights (single step 8/16/32, two steps
+  // per launch 208/216/232 for heights 8/16/32), plus the z-marching strips (codes
+  // 102/104) when kMarchInTuner is set.  Any listed code can also be pinned
+  // through the environment.
   static constexpr bool kMarchInTuner = false;
   static std::vector<int> candidates(std::vector<int> tiles) {
     if (kMarchInTuner) {
@@ -1125,6 +1433,7 @@ class Vd2dEngine final : public Engine {
         if (c == t) return t;
       if (t == 102 || t == 104) return t;
     }
+    if (cands.size() == 1) return cands.front();
     hipEvent_t e0, e1;
     SWX_HIP(hipEventCreate(&e0));
     SWX_HIP(hipEventCreate(&e1));
@@ -1138,6 +1447,7 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipEventSynchronize(e1));
       float ms = 0;
       SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= (float)steps_per_launch(c);   // per time step
       if (ms < bestms) { bestms = ms; best = c; }
     }
     (void)hipEventDestroy(e0);
@@ -1145,12 +1455,13 @@ class Vd2dEngine final : public Engine {
     return best;
   }
   int tz_fwd_ = 0, tz_adj_ = 0;
+  int flip_ = 0;
 
   // checkpoint k (k = 1 .. floor(nt_max/S), all shots): p, vx, vz, psi_x,
   // psi_z, xi_x, xi_z at parity cb&1
   void ckpt_copy(int k, int cb, bool save) {
     const int64_t e = sizeof(T);
-    const int par = cb & 1;
+    const int par = slot(cb);
     char *base = ck_.as<char>() + (int64_t)(k - 1) * ck_one_;
     struct Part { void *dev; int64_t bytes; };
     const Part parts[5] = {

@@ -1134,6 +1134,7 @@ class El2dEngine final : public Engine {
     if (save) SWX_HIP(hipMemsetAsync(hist_.get(), 0, hist_.bytes(), c_.stream));
     T *tr = c_.pts.d_traces.as<T>();
     for (int it = 1; it <= ntmax_; it++) {
+      progress("forward step", it, ntmax_);
       step_fwd(it, 0, save ? 1 : 0, 0, tr);
       if (save && it % S_ == 0) {
         const int k = it / S_;
@@ -1154,6 +1155,7 @@ class El2dEngine final : public Engine {
     const int Kmax = (int)ceil_div(ntmax_, S_);
     const int64_t n2 = 2 * N_;
     for (int k = Kmax - 1; k >= 0; k--) {
+      progress("adjoint segment", Kmax - k, Kmax);
       const int cb = k * S_;
       // shots whose checkpoint (k+1)S exists replay segment k: levels cb-1
       // and cb from checkpoint k, cb+1 .. cb+S-1 recomputed, cb+S-1 and

@@ -194,6 +194,10 @@ std::unique_ptr<Engine> make_elastic2d(PlanCore &core);
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int mod3(int k) { return ((k % 3) + 3) % 3; }
 
+// Progress lines on stderr when the environment sets SWX_PROGRESS: about
+// ten per time loop (forward steps, adjoint segments).
+void progress(const char *phase, int done, int total);
+
 // Chooses the replay segment length S for linear checkpointing given the
 // bytes per replay slot (all shots) and per checkpoint (all shots), the
 // user check_freq (1 = keep every step) and the memory still available.

@@ -1,6 +1,7 @@
 // C ABI entry points of libswx.so (declared in include/swx.h): plan life
 // cycle, validation, shot tables and dispatch to the physics engines.
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <map>
 #include <numeric>
@@ -67,6 +68,16 @@ void KernelTimer::collect(double ms[2], int64_t n[2]) {
     n[kind] += 1;
   }
   reset();
+}
+
+void progress(const char *phase, int done, int total) {
+  static const bool on = std::getenv("SWX_PROGRESS") != nullptr;
+  if (!on || total <= 0) return;
+  const int every = std::max(1, total / 10);
+  if (done % every == 0 || done == total) {
+    std::fprintf(stderr, "[swx] %s %d/%d\n", phase, done, total);
+    std::fflush(stderr);
+  }
 }
 
 int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail) {

@@ -118,19 +118,24 @@ def test_checkpoint_intervals_agree_bitwise(oracle_mod):
 
 def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
     """Every forward and adjoint kernel variant the autotuner may pick (2-D
-    tiles of height 8/16/32, z-marching strips of 128/256 columns) gives the
-    same seismograms and gradients bit for bit, and matches the oracle."""
+    tiles of height 8/16/32, two steps per launch on tiles of height
+    8/16/32, z-marching strips of 128/256 columns) gives the same seismograms
+    and gradients bit for bit, and matches the oracle.  The uneven shots
+    (one ends on an odd step) and the odd replay length (39) make the
+    two-step launches end with single steps; check_freq = 1 stores every
+    level."""
     p = _problem(np.float64, nts=(300, 257), n=(301, 150))
     obs = _observed(oracle_mod, p)
     mis = [sw.L2Misfit(o) for o in obs]
     mp = sw.VpRhoAcousticVDMaterialProperties(p["v0"], p["r0"])
     outs = []
-    for v, w in (("8", "8"), ("16", "16"), ("32", "102"), ("102", "104"), ("104", "8")):
-        print(f"[vd2d kernel variants forward {v} adjoint {w}]", flush=True)
+    for v, w, cf in (("8", "8", 40), ("16", "16", 40), ("32", "102", 40), ("102", "104", 40), ("104", "8", 40),
+                     ("208", "16", 40), ("216", "8", 40), ("232", "16", 40), ("216", "16", 1), ("208", "8", 7)):
+        print(f"[vd2d kernel variants forward {v} adjoint {w} check_freq {cf}]", flush=True)
         monkeypatch.setenv("SWX_VD2D_TILE_FWD", v)
         monkeypatch.setenv("SWX_VD2D_TILE_ADJ", w)
         g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
-                          gradparams=sw.GradParameters(check_freq=40))
+                          gradparams=sw.GradParameters(check_freq=cf))
         outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
     for g, seis in outs[1:]:
         assert np.array_equal(g["vp"], outs[0][0]["vp"]) and np.array_equal(g["rho"], outs[0][0]["rho"])
