@@ -124,6 +124,26 @@ def workload_vd2d_batched_small():
     return wl
 
 
+def workload_vd2d_batched_nt10():
+    """The batched VD workload with every shot's nt cut by ten (314..499):
+    the same grid, shots, receivers and checkpoint interval, so a launch
+    steps the same cells of the same shots and the share of active shots
+    over the run is the same; for counter-collection runs, where every
+    dispatch is instrumented."""
+    wl = workload_vd2d_batched()
+    import seismicwaves as sw
+    shots = []
+    for sh in wl["shots"]:
+        nt = sh.recs.seismograms.shape[0] // 10
+        shots.append(sw.ScalarShot(sw.ScalarSources(sh.srcs.positions, sh.srcs.tf[:nt], sh.srcs.domfreq),
+                                   sw.ScalarReceivers(sh.recs.positions, nt, dtype=np.float32)))
+    wl["shots"] = shots
+    nts = [s.recs.seismograms.shape[0] for s in shots]
+    wl["name"] = f"acoustic_vd_2d_gradient_2000x800_64shots_nt{min(nts)}-{max(nts)}"
+    wl["config"].update({"nt_min": min(nts), "nt_max": max(nts), "sum_nt": sum(nts)})
+    return wl
+
+
 def workload_el2d_batched(nt=5000):
     """Batched 2D isotropic elastic gradient w.r.t. rho, lambda, mu:
     1500 x 600, dh = 5 m, dt = 0.4 ms, nt = 5000, free surface on top, C-PML
@@ -175,6 +195,7 @@ def workload_el2d_batched(nt=5000):
 
 WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_cd2d_grad_1000,
              "vd2d_batched_small": workload_vd2d_batched_small, "el2d_batched": workload_el2d_batched,
+             "vd2d_batched_nt10": workload_vd2d_batched_nt10,
              "el2d_batched_small": lambda: workload_el2d_batched(nt=500)}
 
 
@@ -269,7 +290,10 @@ def kernel_names(physics, variant):
         def name(kind, v):
             if v >= 200:
                 return f"vd2d_{kind}2_kernel<float, {v - 200}, {512 if v - 200 > 8 else 256}>"
-            return f"vd2d_{kind}_march<float, {v - 100}>" if v > 100 else f"vd2d_{kind}_kernel<float, {v}>"
+            if v > 100:
+                return f"vd2d_{kind}_march<float, {v - 100}>"
+            # interior tiles and the other tiles: two kernels per step
+            return f"vd2d_{kind}_lean<float, {v}> + vd2d_{kind}_kernel<float, {v}>"
         return name("fwd", variant[0]), name("adj", variant[1])
     if physics == "el2d":
         def name(adj, v):

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   if (kx >= 0) {
     const int64_t q = (int64_t)s * a.spx + (int64_t)j * (2 * h) + kx;
     const T d = (pc_[c + 1] - pc) * a.idx;
-    psx = cx.bh[kx] * a.psix_o[q] + cx.ah[kx] * d;
+    psx = cx.bh(kx) * a.psix_o[q] + cx.ah(kx) * d;
     a.psix_n[q] = psx;
   }
   // psi_z (update_psi_y!, acoustic2D_xPU.jl:14-25)
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   if (kz >= 0) {
     const int64_t q = (int64_t)s * a.spz + (int64_t)kz * nx + i;
     const T d = (pc_[c + nx] - pc) * a.idz;
-    psz = cz.bh[kz] * a.psiz_o[q] + cz.ah[kz] * d;
+    psz = cz.bh(kz) * a.psiz_o[q] + cz.ah(kz) * d;
     a.psiz_n[q] = psz;
   }
 
@@ -112,20 +112,20 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       // left neighbour's new psi, recomputed from the old buffer
       const int kn = kx - 1;
       const int64_t qn = (int64_t)s * a.spx + (int64_t)j * (2 * h) + kn;
-      const T psn = cx.bh[kn] * a.psix_o[qn] + cx.ah[kn] * ((pc - pl) * a.idx);
+      const T psn = cx.bh(kn) * a.psix_o[qn] + cx.ah(kn) * ((pc - pl) * a.idx);
       const T dpsi = (psx - psn) * a.idx;
       const int64_t qx = (int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + kx;
-      const T xi = cx.b[kx] * a.xix[qx] + cx.a[kx] * (d2x + dpsi);
+      const T xi = cx.b(kx) * a.xix[qx] + cx.a(kx) * (d2x + dpsi);
       a.xix[qx] = xi;
       lx = d2x + dpsi + xi;
     }
     if (j < h || j >= nz - h) {
       const int kn = kz - 1;
       const int64_t qn = (int64_t)s * a.spz + (int64_t)kn * nx + i;
-      const T psn = cz.bh[kn] * a.psiz_o[qn] + cz.ah[kn] * ((pc - pd) * a.idz);
+      const T psn = cz.bh(kn) * a.psiz_o[qn] + cz.ah(kn) * ((pc - pd) * a.idz);
       const T dpsi = (psz - psn) * a.idz;
       const int64_t qz = (int64_t)s * a.sxz + (int64_t)kz * nx + i;
-      const T xi = cz.b[kz] * a.xiz[qz] + cz.a[kz] * (d2z + dpsi);
+      const T xi = cz.b(kz) * a.xiz[qz] + cz.a(kz) * (d2z + dpsi);
       a.xiz[qz] = xi;
       lz = d2z + dpsi + xi;
     }

@@ -130,18 +130,18 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
       T psx = 0, psy = 0, psz = 0;
       if (kx >= 0) {
         const int64_t q = (int64_t)s * a.sps[0] + ((int64_t)k * ny + j) * (2 * h) + kx;
-        psx = cx.bh[kx] * a.psi_o[0][q] + cx.ah[kx] * ((pr - pc) * a.id[0]);
+        psx = cx.bh(kx) * a.psi_o[0][q] + cx.ah(kx) * ((pr - pc) * a.id[0]);
         a.psi_n[0][q] = psx;
       }
       if (ky >= 0) {
         const int64_t q = (int64_t)s * a.sps[1] + ((int64_t)k * (2 * h) + ky) * nx + i;
-        psy = cy.bh[ky] * a.psi_o[1][q] + cy.ah[ky] * ((pu - pc) * a.id[1]);
+        psy = cy.bh(ky) * a.psi_o[1][q] + cy.ah(ky) * ((pu - pc) * a.id[1]);
         a.psi_n[1][q] = psy;
       }
       const int kz = psi_owner(k, nz, h);
       if (kz >= 0) {
         const int64_t q = (int64_t)s * a.sps[2] + (int64_t)kz * sxy + cxy;
-        psz = cz.bh[kz] * a.psi_o[2][q] + cz.ah[kz] * ((pp - pc) * a.id[2]);
+        psz = cz.bh(kz) * a.psi_o[2][q] + cz.ah(kz) * ((pp - pc) * a.id[2]);
         a.psi_n[2][q] = psz;
       }
       const bool interior = xin && yin && k >= 1 && k <= nz - 2;
@@ -155,20 +155,20 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
         if (xcpml) {
           const int kn = kx - 1;
           const int64_t qn = (int64_t)s * a.sps[0] + ((int64_t)k * ny + j) * (2 * h) + kn;
-          const T psn = cx.bh[kn] * a.psi_o[0][qn] + cx.ah[kn] * ((pc - pl) * a.id[0]);
+          const T psn = cx.bh(kn) * a.psi_o[0][qn] + cx.ah(kn) * ((pc - pl) * a.id[0]);
           const T dpsi = (psx - psn) * a.id[0];
           const int64_t qx = (int64_t)s * a.sxi[0] + ((int64_t)k * ny + j) * (2 * h + 2) + kx;
-          const T xi = cx.b[kx] * a.xi[0][qx] + cx.a[kx] * (d2x + dpsi);
+          const T xi = cx.b(kx) * a.xi[0][qx] + cx.a(kx) * (d2x + dpsi);
           a.xi[0][qx] = xi;
           lx = d2x + dpsi + xi;
         }
         if (ycpml) {
           const int kn = ky - 1;
           const int64_t qn = (int64_t)s * a.sps[1] + ((int64_t)k * (2 * h) + kn) * nx + i;
-          const T psn = cy.bh[kn] * a.psi_o[1][qn] + cy.ah[kn] * ((pc - pd) * a.id[1]);
+          const T psn = cy.bh(kn) * a.psi_o[1][qn] + cy.ah(kn) * ((pc - pd) * a.id[1]);
           const T dpsi = (psy - psn) * a.id[1];
           const int64_t qy = (int64_t)s * a.sxi[1] + ((int64_t)k * (2 * h + 2) + ky) * nx + i;
-          const T xi = cy.b[ky] * a.xi[1][qy] + cy.a[ky] * (d2y + dpsi);
+          const T xi = cy.b(ky) * a.xi[1][qy] + cy.a(ky) * (d2y + dpsi);
           a.xi[1][qy] = xi;
           ly = d2y + dpsi + xi;
         }
@@ -179,11 +179,11 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
             psn = psz_prev;
           } else {
             const int64_t qn = (int64_t)s * a.sps[2] + (int64_t)kn * sxy + cxy;
-            psn = cz.bh[kn] * a.psi_o[2][qn] + cz.ah[kn] * ((pc - pm) * a.id[2]);
+            psn = cz.bh(kn) * a.psi_o[2][qn] + cz.ah(kn) * ((pc - pm) * a.id[2]);
           }
           const T dpsi = (psz - psn) * a.id[2];
           const int64_t qz = (int64_t)s * a.sxi[2] + (int64_t)kz * sxy + cxy;
-          const T xi = cz.b[kz] * a.xi[2][qz] + cz.a[kz] * (d2z + dpsi);
+          const T xi = cz.b(kz) * a.xi[2][qz] + cz.a(kz) * (d2z + dpsi);
           a.xi[2][qz] = xi;
           lz = d2z + dpsi + xi;
         }

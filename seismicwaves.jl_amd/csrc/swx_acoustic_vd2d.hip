@@ -105,6 +105,130 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b & 7) * per + (b >> 3);
 }
 
+// Is tile (I0, J0) of shot s an interior tile of the forward step: every
+// cell the tile step reads or updates is inside the grid and outside the
+// C-PML strips, and no source (new-pressure rows) or receiver (tile rows)
+// lies in its rows?
+template <int TZ, typename T>
+__device__ __forceinline__ bool vd2d_fwd_is_lean(const Vd2dArgs<T> &a, int s, int I0, int J0) {
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (!(I0 - 3 >= 0 && I0 - 1 > h && I0 + TX + 3 <= nx - 1 && I0 + TX + 1 < nx - h - 1 && J0 - 3 >= 0 &&
+        J0 - 1 > h && J0 + TZ + 3 <= nz - 1 && J0 + TZ + 1 < nz - h - 1))
+    return false;
+  const int *sr = a.pt.srow + (int64_t)s * (a.pt.nrows + 1);
+  const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+  return sr[J0 - 1] == sr[J0 + TZ + 2] && (a.traces == nullptr || rr[J0] == rr[J0 + TZ]);
+}
+
+// Forward / replay step on the interior tiles (vd2d_fwd_is_lean): the
+// updates of vd2d_fwd_kernel, in the same operand order, without per-cell
+// tests, on 32-bit offsets from per-shot bases.  Blocks of other tiles
+// return at once; vd2d_fwd_kernel, launched right after on the same grid,
+// steps those.
+template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
+  constexpr int R = TZ / BY;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  if (!vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
+  const int nx = a.nx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int64_t so = (int64_t)s * a.N;
+  __shared__ T s_vx[TZ + 3][TX + 6];
+  __shared__ T s_vz[TZ + 6][TX + 3];
+  __shared__ T s_p[TZ + 3][TX + 3];
+  const T *__restrict__ po = a.p_o + so;
+  const T *__restrict__ vxo = a.vx_o + so;
+  const T *__restrict__ vzo = a.vz_o + so;
+  for (int lz = ty; lz < TZ + 3; lz += BY) {
+    const int c = (J0 - 1 + lz) * nx + I0 - 3 + tx;
+    s_vx[lz][tx] = vxo[c];
+    if (tx < 6) s_vx[lz][TX + tx] = vxo[c + TX];
+  }
+  for (int lz = ty; lz < TZ + 6; lz += BY) {
+    const int c = (J0 - 3 + lz) * nx + I0 - 1 + tx;
+    s_vz[lz][tx] = vzo[c];
+    if (tx < 3) s_vz[lz][TX + tx] = vzo[c + TX];
+  }
+  T rp[R], rm0[R], rm1x[R], rm1z[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + ty + BY * r) * nx + I0 + tx;
+    rp[r] = po[c];
+    rm0[r] = a.m0[c];
+    rm1x[r] = a.m1x[c];
+    rm1z[r] = a.m1z[c];
+  }
+  // cross-shaped halo of the new pressure: columns I0-1, I0+TX, I0+TX+1
+  // on the tile rows, rows J0-1, J0+TZ, J0+TZ+1 on the tile columns
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
+  T hp[HR], hm0[HR];
+  int hlx[HR], hlz[HR];
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    int lx = -1, lz = -1;
+    if (q < HX) {
+      const int col = q / TZ;
+      lz = 1 + q % TZ;
+      lx = col == 0 ? 0 : TX + col;
+    } else if (q < H) {
+      const int row = (q - HX) / TX;
+      lx = 1 + (q - HX) % TX;
+      lz = row == 0 ? 0 : TZ + row;
+    }
+    T vp = 0, vm = 0;
+    if (lx >= 0) {
+      const int c = (J0 - 1 + lz) * nx + I0 - 1 + lx;
+      vp = po[c];
+      vm = a.m0[c];
+    }
+    hlx[k] = lx;
+    hlz[k] = lz;
+    hp[k] = vp;
+    hm0[k] = vm;
+  }
+  __syncthreads();
+  const int sb = store_base(a.smode, a.it, nts, a.S, a.cbase);
+  T *__restrict__ pn = a.p_n + so;
+  auto lean_p = [&](int lx, int lz, T pold, T m0v) -> T {
+    const T dvx = d4(s_vx[lz][lx], s_vx[lz][lx + 1], s_vx[lz][lx + 2], s_vx[lz][lx + 3]) * a.idx;
+    const T dvz = d4(s_vz[lz][lx], s_vz[lz + 1][lx], s_vz[lz + 2][lx], s_vz[lz + 3][lx]) * a.idz;
+    return pold - m0v * (dvx + dvz);
+  };
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r + 1;
+    const T pv = lean_p(tx + 1, lz, rp[r], rm0[r]);
+    s_p[lz][tx + 1] = pv;
+    const int c = (J0 + lz - 1) * nx + I0 + tx;
+    pn[c] = pv;
+    if (sb >= 0) {
+      T *hs = a.pbuf + so;
+      hs[(int64_t)(a.it - sb) * a.pslot + c] = pv;
+      if (a.smode == 1 && a.it == sb + 1) hs[c] = rp[r];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++)
+    if (hlx[k] >= 0) s_p[hlz[k]][hlx[k]] = lean_p(hlx[k], hlz[k], hp[k], hm0[k]);
+  __syncthreads();
+  T *__restrict__ vxn = a.vx_n + so;
+  T *__restrict__ vzn = a.vz_n + so;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r;
+    const int c = (J0 + lz) * nx + I0 + tx;
+    const T dx = d4(s_p[lz + 1][tx], s_p[lz + 1][tx + 1], s_p[lz + 1][tx + 2], s_p[lz + 1][tx + 3]) * a.idx;
+    vxn[c] = s_vx[lz + 1][tx + 3] - rm1x[r] * dx;
+    const T dz = d4(s_p[lz][tx + 1], s_p[lz + 1][tx + 1], s_p[lz + 2][tx + 1], s_p[lz + 3][tx + 1]) * a.idz;
+    vzn[c] = s_vz[lz + 3][tx + 1] - rm1z[r] * dz;
+  }
+}
+
 // Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.
 template <typename T, int TZ>
 __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
@@ -123,6 +247,9 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   __shared__ T s_vx[TZ + 3][TX + 6];   // vx^{it-1}: x in [I0-3, I0+TX+3), z in [J0-1, J0+TZ+2)
   __shared__ T s_vz[TZ + 6][TX + 3];   // vz^{it-1}: x in [I0-1, I0+TX+2), z in [J0-3, J0+TZ+3)
   __shared__ T s_p[TZ + 3][TX + 3];    // p^{it}:    x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2)
+
+  // interior tiles are stepped by vd2d_fwd_lean
+  if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
 
   // ---- phase 0: every load that does not depend on LDS -------------------
   for (int q = tid; q < (TZ + 3) * (TX + 6); q += NT) {
@@ -205,7 +332,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
       if (gi2 <= h || gi2 >= nx - h - 1) {
         const int k = gi2 <= h ? gi2 : gi2 - nx + 2 * h + 2;
         const int64_t qx = (int64_t)s * a.sxx + (int64_t)gj2 * (2 * h + 2) + k;
-        const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+        const T xi = cx.b(k) * a.xix_o[qx] + cx.a(k) * dvx;
         if (owned) a.xix_n[qx] = xi;
         dvx = dvx + xi;
       }
@@ -213,7 +340,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
       if (gj2 <= h || gj2 >= nz - h - 1) {
         const int k = gj2 <= h ? gj2 : gj2 - nz + 2 * h + 2;
         const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi2;
-        const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+        const T xi = cz.b(k) * a.xiz_o[qz] + cz.a(k) * dvz;
         if (owned) a.xiz_n[qz] = xi;
         dvz = dvz + xi;
       }
@@ -264,7 +391,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
       if (gi <= h - 1 || gi >= nx - h - 1) {
         const int k = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
         const int64_t q = (int64_t)s * a.spx + (int64_t)gj * (2 * h) + k;
-        const T ps = cx.bh[k] * a.psx_o[q] + cx.ah[k] * d;
+        const T ps = cx.bh(k) * a.psx_o[q] + cx.ah(k) * d;
         a.psx_n[q] = ps;
         d = d + ps;
       }
@@ -275,7 +402,7 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
       if (gj <= h - 1 || gj >= nz - h - 1) {
         const int k = gj <= h - 1 ? gj : gj + 1 - nz + 2 * h;
         const int64_t q = (int64_t)s * a.spz + (int64_t)k * nx + gi;
-        const T ps = cz.bh[k] * a.psz_o[q] + cz.ah[k] * d;
+        const T ps = cz.bh(k) * a.psz_o[q] + cz.ah(k) * d;
         a.psz_n[q] = ps;
         d = d + ps;
       }
@@ -436,14 +563,14 @@ __global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
         T dvx = d4(s_vx[pz][px], s_vx[pz][px + 1], s_vx[pz][px + 2], s_vx[pz][px + 3]) * a.idx;
         if (gi <= h || gi >= nx - h - 1) {
           const int kk = gi <= h ? gi : gi - nx + 2 * h + 2;
-          const T xi = cx.b[kk] * rxx[k] + cx.a[kk] * dvx;
+          const T xi = cx.b(kk) * rxx[k] + cx.a(kk) * dvx;
           rxx[k] = xi;
           dvx = dvx + xi;
         }
         T dvz = d4(s_vz[pz][px], s_vz[pz + 1][px], s_vz[pz + 2][px], s_vz[pz + 3][px]) * a.idz;
         if (gj <= h || gj >= nz - h - 1) {
           const int kk = gj <= h ? gj : gj - nz + 2 * h + 2;
-          const T xi = cz.b[kk] * rxz[k] + cz.a[kk] * dvz;
+          const T xi = cz.b(kk) * rxz[k] + cz.a(kk) * dvz;
           rxz[k] = xi;
           dvz = dvz + xi;
         }
@@ -499,7 +626,7 @@ __global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
       const bool strip = gi <= h - 1 || gi >= nx - h - 1;
       const int kk = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
       if (strip) {
-        const T ps = cx.bh[kk] * rpx[k] + cx.ah[kk] * d;
+        const T ps = cx.bh(kk) * rpx[k] + cx.ah(kk) * d;
         rpx[k] = ps;
         d = d + ps;
       }
@@ -527,7 +654,7 @@ __global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
       const bool strip = gj <= h - 1 || gj >= nz - h - 1;
       const int kk = gj <= h - 1 ? gj : gj + 1 - nz + 2 * h;
       if (strip) {
-        const T ps = cz.bh[kk] * rpz[k] + cz.ah[kk] * d;
+        const T ps = cz.bh(kk) * rpz[k] + cz.ah(kk) * d;
         rpz[k] = ps;
         d = d + ps;
       }
@@ -646,7 +773,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
       if (x <= h || x >= nx - h - 1) {
         const int k = x <= h ? x : x - nx + 2 * h + 2;
         const int64_t qx = (int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + k;
-        const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+        const T xi = cx.b(k) * a.xix_o[qx] + cx.a(k) * dvx;
         if (owned) a.xix_n[qx] = xi;
         dvx = dvx + xi;
       }
@@ -654,7 +781,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
       if (j <= h || j >= nz - h - 1) {
         const int k = j <= h ? j : j - nz + 2 * h + 2;
         const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + x;
-        const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+        const T xi = cz.b(k) * a.xiz_o[qz] + cz.a(k) * dvz;
         if (owned) a.xiz_n[qz] = xi;
         dvz = dvz + xi;
       }
@@ -706,7 +833,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
         if (gi <= h - 1 || gi >= nx - h - 1) {
           const int k = gi <= h - 1 ? gi : gi + 1 - nx + 2 * h;
           const int64_t q = (int64_t)s * a.spx + (int64_t)jv * (2 * h) + k;
-          const T ps = cx.bh[k] * a.psx_o[q] + cx.ah[k] * d;
+          const T ps = cx.bh(k) * a.psx_o[q] + cx.ah(k) * d;
           a.psx_n[q] = ps;
           d = d + ps;
         }
@@ -718,13 +845,164 @@ __global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
         if (jv <= h - 1 || jv >= nz - h - 1) {
           const int k = jv <= h - 1 ? jv : jv + 1 - nz + 2 * h;
           const int64_t q = (int64_t)s * a.spz + (int64_t)k * nx + gi;
-          const T ps = cz.bh[k] * a.psz_o[q] + cz.ah[k] * d;
+          const T ps = cz.bh(k) * a.psz_o[q] + cz.ah(k) * d;
           a.psz_n[q] = ps;
           d = d + ps;
         }
         a.vz_n[so + c] = rvz[rz5(jv)][tx + 1] - c_m1z * d;
       }
     }
+  }
+}
+
+// Is tile (I0, J0) of shot s an interior tile of the adjoint step: every
+// cell the tile step reads or updates (old adjoint pressure three cells out,
+// velocities two out, replayed pressure one/two out) is inside the grid and
+// outside the C-PML strips, and no receiver lies in its rows?
+template <int TZ, typename T>
+__device__ __forceinline__ bool vd2d_adj_is_lean(const Vd2dArgs<T> &a, int s, int I0, int J0) {
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (!(I0 - 3 > h && I0 + TX + 3 < nx - h - 1 && J0 - 3 > h && J0 + TZ + 3 < nz - h - 1)) return false;
+  const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+  return rr[J0] == rr[J0 + TZ];
+}
+
+// Adjoint step on the interior tiles (vd2d_adj_is_lean): the updates and
+// correlations of vd2d_adj_kernel, in the same operand order, without
+// per-cell tests.  Blocks of other tiles return at once; vd2d_adj_kernel,
+// launched right after on the same grid, steps those.
+template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
+  constexpr int R = TZ / BY;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  if (!vd2d_adj_is_lean<TZ>(a, s, I0, J0)) return;
+  const int nx = a.nx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int64_t so = (int64_t)s * a.N;
+  const int slot = a.it - a.cbase;
+  const T *__restrict__ pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
+  const T *__restrict__ po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
+  const T *__restrict__ pa = a.p_o + so;
+  const T *__restrict__ vxo = a.vx_o + so;
+  const T *__restrict__ vzo = a.vz_o + so;
+
+  __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
+  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
+  __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
+  __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
+
+  for (int lz = ty; lz < TZ + 6; lz += BY) {
+    const int c = (J0 - 3 + lz) * nx + I0 - 3;
+    if (lz >= 3 && lz < TZ + 3) {
+      s_p[lz][tx] = pa[c + tx];
+      if (tx < 6) s_p[lz][TX + tx] = pa[c + TX + tx];
+    } else {
+      s_p[lz][tx + 3] = pa[c + 3 + tx];
+    }
+  }
+  for (int lz = ty; lz < TZ + 3; lz += BY) {
+    const int c = (J0 - 1 + lz) * nx + I0 - 1;
+    if (lz >= 1 && lz < TZ + 1) {
+      s_pc[lz][tx] = pc_[c + tx];
+      if (tx < 3) s_pc[lz][TX + tx] = pc_[c + TX + tx];
+    } else {
+      s_pc[lz][tx + 1] = pc_[c + 1 + tx];
+    }
+  }
+  T rvx[R], rvz[R], rm1x[R], rm1z[R], rm0[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + ty + BY * r) * nx + I0 + tx;
+    rvx[r] = vxo[c];
+    rvz[r] = vzo[c];
+    rm1x[r] = a.m1x[c];
+    rm1z[r] = a.m1z[c];
+    rm0[r] = a.m0[c];
+    rpo[r] = po_[c];
+    rg0[r] = a.g0[so + c];
+    rg1x[r] = a.g1x[so + c];
+    rg1z[r] = a.g1z[so + c];
+  }
+  // velocity halo: v~x columns I0-2, I0-1, I0+TX on tile rows; v~z rows
+  // J0-2, J0-1, J0+TZ on tile columns
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
+  T hv[HR], hm[HR];
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    T v = 0, m = 0;
+    if (q < HX) {
+      const int col = q / TZ, lz = q % TZ;
+      const int c = (J0 + lz) * nx + I0 - 2 + (col < 2 ? col : TX + 2);
+      v = vxo[c];
+      m = a.m1x[c];
+    } else if (q < H) {
+      const int row = (q - HX) / TX, lx = (q - HX) % TX;
+      const int c = (J0 - 2 + (row < 2 ? row : TZ + 2)) * nx + I0 + lx;
+      v = vzo[c];
+      m = a.m1z[c];
+    }
+    hv[k] = v;
+    hm[k] = m;
+  }
+  __syncthreads();
+
+  auto new_vx = [&](int lx, int lz, T vold, T m1) -> T {
+    const T d = d4(s_p[lz + 3][lx], s_p[lz + 3][lx + 1], s_p[lz + 3][lx + 2], s_p[lz + 3][lx + 3]) * a.idx;
+    return vold - m1 * d;
+  };
+  auto new_vz = [&](int lx, int lz, T vold, T m1) -> T {
+    const T d = d4(s_p[lz][lx + 3], s_p[lz + 1][lx + 3], s_p[lz + 2][lx + 3], s_p[lz + 3][lx + 3]) * a.idz;
+    return vold - m1 * d;
+  };
+  T *__restrict__ vxn = a.vx_n + so;
+  T *__restrict__ vzn = a.vz_n + so;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r;
+    const int c = (J0 + lz) * nx + I0 + tx;
+    const T vx = new_vx(tx + 2, lz, rvx[r], rm1x[r]);
+    const T vz = new_vz(tx, lz + 2, rvz[r], rm1z[r]);
+    vxn[c] = vx;
+    vzn[c] = vz;
+    s_vx[lz][tx + 2] = vx;
+    s_vz[lz + 2][tx] = vz;
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    if (q < HX) {
+      const int col = q / TZ, lz = q % TZ;
+      const int lx = col < 2 ? col : TX + 2;
+      s_vx[lz][lx] = new_vx(lx, lz, hv[k], hm[k]);
+    } else if (q < H) {
+      const int row = (q - HX) / TX, lx = (q - HX) % TX;
+      const int lz = row < 2 ? row : TZ + 2;
+      s_vz[lz][lx] = new_vz(lx, lz, hv[k], hm[k]);
+    }
+  }
+  __syncthreads();
+
+  T *__restrict__ pn = a.p_n + so;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int lz = ty + BY * r;
+    const int c = (J0 + lz) * nx + I0 + tx;
+    T pv = s_p[lz + 3][tx + 3];
+    const T dvx = d4(s_vx[lz][tx], s_vx[lz][tx + 1], s_vx[lz][tx + 2], s_vx[lz][tx + 3]) * a.idx;
+    const T dvz = d4(s_vz[lz][tx], s_vz[lz + 1][tx], s_vz[lz + 2][tx], s_vz[lz + 3][tx]) * a.idz;
+    pv = pv - rm0[r] * (dvx + dvz);
+    pn[c] = pv;
+    const T pcv = s_pc[lz + 1][tx + 1];
+    a.g0[so + c] = rg0[r] - (pv * (pcv - rpo[r]) * a.idt);
+    const T dpx = d4(s_pc[lz + 1][tx], pcv, s_pc[lz + 1][tx + 2], s_pc[lz + 1][tx + 3]) * a.idx;
+    a.g1x[so + c] = rg1x[r] + s_vx[lz][tx + 2] * dpx;
+    const T dpz = d4(s_pc[lz][tx + 1], pcv, s_pc[lz + 2][tx + 1], s_pc[lz + 3][tx + 1]) * a.idz;
+    a.g1z[so + c] = rg1z[r] + s_vz[lz + 2][tx] * dpz;
   }
 }
 
@@ -743,6 +1021,8 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  // interior tiles are stepped by vd2d_adj_lean
+  if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) return;
   const int slot = a.it - a.cbase;
   const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
   const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
@@ -833,7 +1113,7 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
     if (gi2 <= h - 1 || gi2 >= nx - h - 1) {
       const int k = gi2 <= h - 1 ? gi2 : gi2 + 1 - nx + 2 * h;
       const int64_t qq = (int64_t)s * a.spx + (int64_t)gj2 * (2 * h) + k;
-      const T ps = cx.bh[k] * a.psx_o[qq] + cx.ah[k] * d;
+      const T ps = cx.bh(k) * a.psx_o[qq] + cx.ah(k) * d;
       if (owned) a.psx_n[qq] = ps;
       d = d + ps;
     }
@@ -848,7 +1128,7 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
     if (gj2 <= h - 1 || gj2 >= nz - h - 1) {
       const int k = gj2 <= h - 1 ? gj2 : gj2 + 1 - nz + 2 * h;
       const int64_t qq = (int64_t)s * a.spz + (int64_t)k * nx + gi2;
-      const T ps = cz.bh[k] * a.psz_o[qq] + cz.ah[k] * d;
+      const T ps = cz.bh(k) * a.psz_o[qq] + cz.ah(k) * d;
       if (owned) a.psz_n[qq] = ps;
       d = d + ps;
     }
@@ -890,7 +1170,7 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
       if (gi <= h || gi >= nx - h - 1) {
         const int k = gi <= h ? gi : gi - nx + 2 * h + 2;
         const int64_t qx = (int64_t)s * a.sxx + (int64_t)gj * (2 * h + 2) + k;
-        const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+        const T xi = cx.b(k) * a.xix_o[qx] + cx.a(k) * dvx;
         a.xix_n[qx] = xi;
         dvx = dvx + xi;
       }
@@ -898,7 +1178,7 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
       if (gj <= h || gj >= nz - h - 1) {
         const int k = gj <= h ? gj : gj - nz + 2 * h + 2;
         const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi;
-        const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+        const T xi = cz.b(k) * a.xiz_o[qz] + cz.a(k) * dvz;
         a.xiz_n[qz] = xi;
         dvz = dvz + xi;
       }
@@ -1022,7 +1302,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_adj_march(Vd2dArgs<T> a) {
     if (x <= h - 1 || x >= nx - h - 1) {
       const int k = x <= h - 1 ? x : x + 1 - nx + 2 * h;
       const int64_t qq = (int64_t)s * a.spx + (int64_t)j * (2 * h) + k;
-      const T ps = cx.bh[k] * a.psx_o[qq] + cx.ah[k] * d;
+      const T ps = cx.bh(k) * a.psx_o[qq] + cx.ah(k) * d;
       if (owned) a.psx_n[qq] = ps;
       d = d + ps;
     }
@@ -1052,7 +1332,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_adj_march(Vd2dArgs<T> a) {
         if (j <= h - 1 || j >= nz - h - 1) {
           const int k = j <= h - 1 ? j : j + 1 - nz + 2 * h;
           const int64_t qq = (int64_t)s * a.spz + (int64_t)k * nx + gi;
-          const T ps = cz.bh[k] * a.psz_o[qq] + cz.ah[k] * d;
+          const T ps = cz.bh(k) * a.psz_o[qq] + cz.ah(k) * d;
           if (own_row) a.psz_n[qq] = ps;
           d = d + ps;
         }
@@ -1073,7 +1353,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_adj_march(Vd2dArgs<T> a) {
         if (gi <= h || gi >= nx - h - 1) {
           const int k = gi <= h ? gi : gi - nx + 2 * h + 2;
           const int64_t qx = (int64_t)s * a.sxx + (int64_t)jp * (2 * h + 2) + k;
-          const T xi = cx.b[k] * a.xix_o[qx] + cx.a[k] * dvx;
+          const T xi = cx.b(k) * a.xix_o[qx] + cx.a(k) * dvx;
           a.xix_n[qx] = xi;
           dvx = dvx + xi;
         }
@@ -1081,7 +1361,7 @@ __global__ __launch_bounds__(64 * W) void vd2d_adj_march(Vd2dArgs<T> a) {
         if (jp <= h || jp >= nz - h - 1) {
           const int k = jp <= h ? jp : jp - nz + 2 * h + 2;
           const int64_t qz = (int64_t)s * a.sxz + (int64_t)k * nx + gi;
-          const T xi = cz.b[k] * a.xiz_o[qz] + cz.a[k] * dvz;
+          const T xi = cz.b(k) * a.xiz_o[qz] + cz.a(k) * dvz;
           a.xiz_n[qz] = xi;
           dvz = dvz + xi;
         }
@@ -1352,10 +1632,18 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipGetLastError());
       return;
     }
+    // interior tiles, then the others (each kernel skips the other's tiles)
     const unsigned g = grid(a, tz);
-    if (tz == 8) vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    else if (tz == 16) vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    else vd2d_fwd_kernel<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    if (tz == 8) {
+      vd2d_fwd_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    } else if (tz == 16) {
+      vd2d_fwd_lean<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    } else {
+      vd2d_fwd_lean<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      vd2d_fwd_kernel<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    }
     SWX_HIP(hipGetLastError());
   }
   void launch_adj(Vd2dArgs<T> a, int tz) {
@@ -1366,9 +1654,15 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipGetLastError());
       return;
     }
+    // interior tiles, then the others (each kernel skips the other's tiles)
     const unsigned g = grid(a, tz);
-    if (tz == 8) vd2d_adj_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    else vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    if (tz == 8) {
+      vd2d_adj_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      vd2d_adj_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    } else {
+      vd2d_adj_lean<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    }
     SWX_HIP(hipGetLastError());
   }
 

@@ -25,15 +25,21 @@ __device__ __forceinline__ int find_point(const int *rowptr, const int64_t *cell
   return -1;
 }
 
-// Packed C-PML coefficients of one axis: a (2h+2) | a_h (2h) | b (2h+2) | b_h (2h)
+// Packed C-PML coefficients of one axis: a (2h+2) | a_h (2h) | b (2h+2) |
+// b_h (2h).  One pointer per axis (the four vectors are found by offset), so
+// a kernel keeps few uniform registers for them.
 template <typename T>
 struct AxisCoef {
-  const T *a, *ah, *b, *bh;
+  const T *p;
+  int h;
+  __device__ __forceinline__ T a(int k) const { return p[k]; }
+  __device__ __forceinline__ T ah(int k) const { return p[2 * h + 2 + k]; }
+  __device__ __forceinline__ T b(int k) const { return p[4 * h + 2 + k]; }
+  __device__ __forceinline__ T bh(int k) const { return p[6 * h + 4 + k]; }
 };
 template <typename T>
 __device__ __forceinline__ AxisCoef<T> axis_coef(const T *base, int h, int axis) {
-  const T *p = base + (int64_t)axis * (8 * h + 4);
-  return AxisCoef<T>{p, p + (2 * h + 2), p + (4 * h + 2), p + (6 * h + 4)};
+  return AxisCoef<T>{base + axis * (8 * h + 4), h};
 }
 
 // Sum over shots (in shot order) times `scale`, or a scaled copy per shot.

@@ -208,14 +208,14 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
       int k = strip_half(i - 1, nx - 1, h);
       if (k) {
         const int64_t q = a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
-        const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * dux;
+        const T pn = cx.b(k - 1) * pso[q] + cx.a(k - 1) * dux;
         psn[q] = pn;
         dxc = dux + pn;
       }
       k = strip_half(j - 1, nz - 1, h);
       if (k) {
         const int64_t q = a.psioff[7] + (i - 1) + (int64_t)(k - 1) * nx;
-        const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * duz;
+        const T pn = cz.b(k - 1) * pso[q] + cz.a(k - 1) * duz;
         psn[q] = pn;
         dzc = duz + pn;
       }
@@ -232,14 +232,14 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
       int k = strip_full(i, nx, h);
       if (k) {
         const int64_t q = a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h);
-        const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * duzx;
+        const T pn = cx.bh(k - 1) * pso[q] + cx.ah(k - 1) * duzx;
         psn[q] = pn;
         c1 = duzx + pn;
       }
       k = strip_full(j, nz, h);
       if (k) {
         const int64_t q = a.psioff[6] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
-        const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * duxz;
+        const T pn = cz.bh(k - 1) * pso[q] + cz.ah(k - 1) * duxz;
         psn[q] = pn;
         c2 = duxz + pn;
       }
@@ -340,14 +340,14 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
     int k = strip_full(i, nx, h);
     if (k) {
       const int64_t q = a.psioff[0] + (k - 1) + (int64_t)(j - 1) * (2 * h);
-      const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * d1;
+      const T pn = cx.bh(k - 1) * pso[q] + cx.ah(k - 1) * d1;
       psn[q] = pn;
       c1 = d1 + pn;
     }
     k = strip_half(j - 1, nz - 1, h);
     if (k) {
       const int64_t q = a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
-      const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * d2;
+      const T pn = cz.b(k - 1) * pso[q] + cz.a(k - 1) * d2;
       psn[q] = pn;
       c2 = d2 + pn;
     }
@@ -362,14 +362,14 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
     int k = strip_half(i - 1, nx - 1, h);
     if (k) {
       const int64_t q = a.psioff[1] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
-      const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * d1;
+      const T pn = cx.b(k - 1) * pso[q] + cx.a(k - 1) * d1;
       psn[q] = pn;
       c1 = d1 + pn;
     }
     k = strip_full(j, nz, h);
     if (k) {
       const int64_t q = a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx;
-      const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * d2;
+      const T pn = cz.bh(k - 1) * pso[q] + cz.ah(k - 1) * d2;
       psn[q] = pn;
       c2 = d2 + pn;
     }
@@ -633,14 +633,14 @@ __global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
     int k = strip_half(x - 1, nx - 1, h);
     if (k) {
       const int64_t q = a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
-      const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * dux;
+      const T pn = cx.b(k - 1) * pso[q] + cx.a(k - 1) * dux;
       if (owned) psn[q] = pn;
       dxc = dux + pn;
     }
     k = strip_half(j - 1, nz - 1, h);
     if (k) {
       const int64_t q = a.psioff[7] + (x - 1) + (int64_t)(k - 1) * nx;
-      const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * duz;
+      const T pn = cz.b(k - 1) * pso[q] + cz.a(k - 1) * duz;
       if (owned) psn[q] = pn;
       dzc = duz + pn;
     }
@@ -671,14 +671,14 @@ __global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
     int k = strip_full(x, nx, h);
     if (k) {
       const int64_t q = a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h);
-      const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * duzx;
+      const T pn = cx.bh(k - 1) * pso[q] + cx.ah(k - 1) * duzx;
       if (owned) psn[q] = pn;
       c1 = duzx + pn;
     }
     k = strip_full(j, nz, h);
     if (k) {
       const int64_t q = a.psioff[6] + (x - 1) + (int64_t)(k - 1) * (nx - 1);
-      const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * duxz;
+      const T pn = cz.bh(k - 1) * pso[q] + cz.ah(k - 1) * duxz;
       if (owned) psn[q] = pn;
       c2 = duxz + pn;
     }
@@ -783,14 +783,14 @@ __global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
         int k = strip_full(i, nx, h);
         if (k) {
           const int64_t q = a.psioff[0] + (k - 1) + (int64_t)(jd - 1) * (2 * h);
-          const T pn = cx.bh[k - 1] * pso[q] + cx.ah[k - 1] * d1;
+          const T pn = cx.bh(k - 1) * pso[q] + cx.ah(k - 1) * d1;
           psn[q] = pn;
           c1 = d1 + pn;
         }
         k = strip_half(jd - 1, nz - 1, h);
         if (k) {
           const int64_t q = a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
-          const T pn = cz.b[k - 1] * pso[q] + cz.a[k - 1] * d2;
+          const T pn = cz.b(k - 1) * pso[q] + cz.a(k - 1) * d2;
           psn[q] = pn;
           c2 = d2 + pn;
         }
@@ -804,14 +804,14 @@ __global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
         int k = strip_half(i - 1, nx - 1, h);
         if (k) {
           const int64_t q = a.psioff[1] + (k - 1) + (int64_t)(jd - 1) * (2 * h + 2);
-          const T pn = cx.b[k - 1] * pso[q] + cx.a[k - 1] * d1;
+          const T pn = cx.b(k - 1) * pso[q] + cx.a(k - 1) * d1;
           psn[q] = pn;
           c1 = d1 + pn;
         }
         k = strip_full(jd, nz, h);
         if (k) {
           const int64_t q = a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx;
-          const T pn = cz.bh[k - 1] * pso[q] + cz.ah[k - 1] * d2;
+          const T pn = cz.bh(k - 1) * pso[q] + cz.ah(k - 1) * d2;
           psn[q] = pn;
           c2 = d2 + pn;
         }
