@@ -143,15 +143,22 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
   const T *__restrict__ po = a.p_o + so;
   const T *__restrict__ vxo = a.vx_o + so;
   const T *__restrict__ vzo = a.vz_o + so;
-  for (int lz = ty; lz < TZ + 3; lz += BY) {
+  // every global load of the thread is issued before its first LDS store
+  constexpr int KX = (TZ + 3 + BY - 1) / BY, KZ = (TZ + 6 + BY - 1) / BY;
+  T lx0[KX], lx1[KX], lz0[KZ], lz1[KZ];
+#pragma unroll
+  for (int k = 0; k < KX; k++) {
+    const int lz = ty + k * BY;
     const int c = (J0 - 1 + lz) * nx + I0 - 3 + tx;
-    s_vx[lz][tx] = vxo[c];
-    if (tx < 6) s_vx[lz][TX + tx] = vxo[c + TX];
+    lx0[k] = lz < TZ + 3 ? vxo[c] : T(0);
+    lx1[k] = lz < TZ + 3 && tx < 6 ? vxo[c + TX] : T(0);
   }
-  for (int lz = ty; lz < TZ + 6; lz += BY) {
+#pragma unroll
+  for (int k = 0; k < KZ; k++) {
+    const int lz = ty + k * BY;
     const int c = (J0 - 3 + lz) * nx + I0 - 1 + tx;
-    s_vz[lz][tx] = vzo[c];
-    if (tx < 3) s_vz[lz][TX + tx] = vzo[c + TX];
+    lz0[k] = lz < TZ + 6 ? vzo[c] : T(0);
+    lz1[k] = lz < TZ + 6 && tx < 3 ? vzo[c + TX] : T(0);
   }
   T rp[R], rm0[R], rm1x[R], rm1z[R];
 #pragma unroll
@@ -190,6 +197,22 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
     hlz[k] = lz;
     hp[k] = vp;
     hm0[k] = vm;
+  }
+#pragma unroll
+  for (int k = 0; k < KX; k++) {
+    const int lz = ty + k * BY;
+    if (lz < TZ + 3) {
+      s_vx[lz][tx] = lx0[k];
+      if (tx < 6) s_vx[lz][TX + tx] = lx1[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KZ; k++) {
+    const int lz = ty + k * BY;
+    if (lz < TZ + 6) {
+      s_vz[lz][tx] = lz0[k];
+      if (tx < 3) s_vz[lz][TX + tx] = lz1[k];
+    }
   }
   __syncthreads();
   const int sb = store_base(a.smode, a.it, nts, a.S, a.cbase);
@@ -251,20 +274,28 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   // interior tiles are stepped by vd2d_fwd_lean
   if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
 
-  // ---- phase 0: every load that does not depend on LDS -------------------
-  for (int q = tid; q < (TZ + 3) * (TX + 6); q += NT) {
+  // ---- phase 0: every load that does not depend on LDS, all issued before
+  // the first LDS store -------------------------------------------------------
+  constexpr int QX = (TZ + 3) * (TX + 6), KQX = (QX + NT - 1) / NT;
+  constexpr int QZ = (TZ + 6) * (TX + 3), KQZ = (QZ + NT - 1) / NT;
+  T gvx[KQX], gvz[KQZ];
+#pragma unroll
+  for (int k = 0; k < KQX; k++) {
+    const int q = tid + k * NT;
     const int lz = q / (TX + 6), lx = q % (TX + 6);
     const int gi = I0 - 3 + lx, gj = J0 - 1 + lz;
     T v = 0;
-    if (gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) v = a.vx_o[so + (int64_t)gj * nx + gi];
-    s_vx[lz][lx] = v;
+    if (q < QX && gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) v = a.vx_o[so + (int64_t)gj * nx + gi];
+    gvx[k] = v;
   }
-  for (int q = tid; q < (TZ + 6) * (TX + 3); q += NT) {
+#pragma unroll
+  for (int k = 0; k < KQZ; k++) {
+    const int q = tid + k * NT;
     const int lz = q / (TX + 3), lx = q % (TX + 3);
     const int gi = I0 - 1 + lx, gj = J0 - 3 + lz;
     T v = 0;
-    if (gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
-    s_vz[lz][lx] = v;
+    if (q < QZ && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
+    gvz[k] = v;
   }
   const int gi = I0 + tx;
   T rp[R], rm0[R], rm1x[R], rm1z[R];
@@ -315,6 +346,16 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
     hlz[k] = lz;
     hp[k] = vp;
     hm0[k] = vm;
+  }
+#pragma unroll
+  for (int k = 0; k < KQX; k++) {
+    const int q = tid + k * NT;
+    if (q < QX) s_vx[q / (TX + 6)][q % (TX + 6)] = gvx[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KQZ; k++) {
+    const int q = tid + k * NT;
+    if (q < QZ) s_vz[q / (TX + 3)][q % (TX + 3)] = gvz[k];
   }
   __syncthreads();
 
@@ -467,19 +508,26 @@ __global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
 
   // ---- loads: v^{it-1}, p^{it-1} into LDS; factors and C-PML variables of
   // this thread's cells into registers ------------------------------------
-  for (int q = tid; q < AZ * AX; q += NB) {
+  // all global loads of a thread are issued before its first LDS store
+  constexpr int KA = (AZ * AX + NB - 1) / NB, KB = (BZ * BX + NB - 1) / NB;
+  T ga[KA], gb[KB];
+#pragma unroll
+  for (int k = 0; k < KA; k++) {
+    const int q = tid + k * NB;
     const int lz = q / AX, lx = q % AX;
     const int gi = I0 - 6 + lx, gj = J0 - 4 + lz;
     T v = 0;
-    if (gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) v = a.vx_o[so + (int64_t)gj * nx + gi];
-    s_vx[lz][lx] = v;
+    if (q < AZ * AX && gi >= 0 && gi <= nx - 2 && gj >= 0 && gj <= nz - 1) v = a.vx_o[so + (int64_t)gj * nx + gi];
+    ga[k] = v;
   }
-  for (int q = tid; q < BZ * BX; q += NB) {
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const int q = tid + k * NB;
     const int lz = q / BX, lx = q % BX;
     const int gi = I0 - 4 + lx, gj = J0 - 6 + lz;
     T v = 0;
-    if (gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
-    s_vz[lz][lx] = v;
+    if (q < BZ * BX && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 2) v = a.vz_o[so + (int64_t)gj * nx + gi];
+    gb[k] = v;
   }
   T rm0[NP], rxx[NP], rxz[NP];
 #pragma unroll
@@ -539,6 +587,16 @@ __global__ __launch_bounds__(NB) void vd2d_fwd2_kernel(Vd2dArgs<T> a) {
     }
     rm1z[k] = vm;
     rpz[k] = ps;
+  }
+#pragma unroll
+  for (int k = 0; k < KA; k++) {
+    const int q = tid + k * NB;
+    if (q < AZ * AX) s_vx[q / AX][q % AX] = ga[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const int q = tid + k * NB;
+    if (q < BZ * BX) s_vz[q / BX][q % BX] = gb[k];
   }
   __syncthreads();
 
@@ -895,23 +953,26 @@ __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
   __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
   __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
 
-  for (int lz = ty; lz < TZ + 6; lz += BY) {
+  // every global load of the thread is issued before its first LDS store.
+  // Old adjoint pressure: full rows on the tile rows (x arm), tile columns
+  // on the three rows above and below (z arm); p^{it} alike.
+  constexpr int KP = (TZ + 6 + BY - 1) / BY, KC = (TZ + 3 + BY - 1) / BY;
+  T lp0[KP], lp1[KP], lc0[KC], lc1[KC];
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const int lz = ty + k * BY;
     const int c = (J0 - 3 + lz) * nx + I0 - 3;
-    if (lz >= 3 && lz < TZ + 3) {
-      s_p[lz][tx] = pa[c + tx];
-      if (tx < 6) s_p[lz][TX + tx] = pa[c + TX + tx];
-    } else {
-      s_p[lz][tx + 3] = pa[c + 3 + tx];
-    }
+    const bool arm = lz >= 3 && lz < TZ + 3;
+    lp0[k] = lz < TZ + 6 ? pa[c + (arm ? 0 : 3) + tx] : T(0);
+    lp1[k] = arm && tx < 6 ? pa[c + TX + tx] : T(0);
   }
-  for (int lz = ty; lz < TZ + 3; lz += BY) {
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    const int lz = ty + k * BY;
     const int c = (J0 - 1 + lz) * nx + I0 - 1;
-    if (lz >= 1 && lz < TZ + 1) {
-      s_pc[lz][tx] = pc_[c + tx];
-      if (tx < 3) s_pc[lz][TX + tx] = pc_[c + TX + tx];
-    } else {
-      s_pc[lz][tx + 1] = pc_[c + 1 + tx];
-    }
+    const bool arm = lz >= 1 && lz < TZ + 1;
+    lc0[k] = lz < TZ + 3 ? pc_[c + (arm ? 0 : 1) + tx] : T(0);
+    lc1[k] = arm && tx < 3 ? pc_[c + TX + tx] : T(0);
   }
   T rvx[R], rvz[R], rm1x[R], rm1z[R], rm0[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
 #pragma unroll
@@ -948,6 +1009,26 @@ __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
     }
     hv[k] = v;
     hm[k] = m;
+  }
+#pragma unroll
+  for (int k = 0; k < KP; k++) {
+    const int lz = ty + k * BY;
+    if (lz >= 3 && lz < TZ + 3) {
+      s_p[lz][tx] = lp0[k];
+      if (tx < 6) s_p[lz][TX + tx] = lp1[k];
+    } else if (lz < TZ + 6) {
+      s_p[lz][tx + 3] = lp0[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    const int lz = ty + k * BY;
+    if (lz >= 1 && lz < TZ + 1) {
+      s_pc[lz][tx] = lc0[k];
+      if (tx < 3) s_pc[lz][TX + tx] = lc1[k];
+    } else if (lz < TZ + 3) {
+      s_pc[lz][tx + 1] = lc0[k];
+    }
   }
   __syncthreads();
 
@@ -1032,22 +1113,31 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
   __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
   __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
 
-  // ---- phase 0 -------------------------------------------------------------
-  for (int q = tid; q < (TZ + 6) * (TX + 6); q += NT) {
+  // ---- phase 0: all global loads issued before the first LDS store ----------
+  constexpr int QP = (TZ + 6) * (TX + 6), KQP = (QP + NT - 1) / NT;
+  constexpr int QC = (TZ + 3) * (TX + 3), KQC = (QC + NT - 1) / NT;
+  T gp[KQP], gc[KQC];
+#pragma unroll
+  for (int k = 0; k < KQP; k++) {
+    const int q = tid + k * NT;
     const int lz = q / (TX + 6), lx = q % (TX + 6);
     const bool xarm = lz >= 3 && lz < TZ + 3, zarm = lx >= 3 && lx < TX + 3;
     const int gi = I0 - 3 + lx, gj = J0 - 3 + lz;
     T v = 0;
-    if ((xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) v = a.p_o[so + (int64_t)gj * nx + gi];
-    s_p[lz][lx] = v;
+    if (q < QP && (xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1)
+      v = a.p_o[so + (int64_t)gj * nx + gi];
+    gp[k] = v;
   }
-  for (int q = tid; q < (TZ + 3) * (TX + 3); q += NT) {
+#pragma unroll
+  for (int k = 0; k < KQC; k++) {
+    const int q = tid + k * NT;
     const int lz = q / (TX + 3), lx = q % (TX + 3);
     const bool xarm = lz >= 1 && lz <= TZ, zarm = lx >= 1 && lx <= TX;
     const int gi = I0 - 1 + lx, gj = J0 - 1 + lz;
     T v = 0;
-    if ((xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1) v = pc_[(int64_t)gj * nx + gi];
-    s_pc[lz][lx] = v;
+    if (q < QC && (xarm || zarm) && gi >= 0 && gi <= nx - 1 && gj >= 0 && gj <= nz - 1)
+      v = pc_[(int64_t)gj * nx + gi];
+    gc[k] = v;
   }
   const int gi = I0 + tx;
   T rvx[R], rvz[R], rm1x[R], rm1z[R], rm0[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
@@ -1102,6 +1192,16 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
     hq[k] = q;
     hv[k] = v;
     hm[k] = m;
+  }
+#pragma unroll
+  for (int k = 0; k < KQP; k++) {
+    const int q = tid + k * NT;
+    if (q < QP) s_p[q / (TX + 6)][q % (TX + 6)] = gp[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KQC; k++) {
+    const int q = tid + k * NT;
+    if (q < QC) s_pc[q / (TX + 3)][q % (TX + 3)] = gc[k];
   }
   __syncthreads();
 
@@ -1463,7 +1563,7 @@ class Vd2dEngine final : public Engine {
     zero(st_);
     if (!tz_fwd_) {
       const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max, 208, 216, 232}),
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max}),
                      [&](int tz) { launch_fwd(a1, tz); });
     }
     for (int it = 1; it <= ntmax_;) {
@@ -1633,6 +1733,7 @@ class Vd2dEngine final : public Engine {
       return;
     }
     // interior tiles, then the others (each kernel skips the other's tiles)
+    // interior tiles, then the others (each kernel skips the other's tiles)
     const unsigned g = grid(a, tz);
     if (tz == 8) {
       vd2d_fwd_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
@@ -1655,6 +1756,7 @@ class Vd2dEngine final : public Engine {
       return;
     }
     // interior tiles, then the others (each kernel skips the other's tiles)
+    // interior tiles, then the others (each kernel skips the other's tiles)
     const unsigned g = grid(a, tz);
     if (tz == 8) {
       vd2d_adj_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
@@ -1663,6 +1765,7 @@ class Vd2dEngine final : public Engine {
       vd2d_adj_lean<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
       vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
     }
+    SWX_HIP(hipGetLastError());
     SWX_HIP(hipGetLastError());
   }
 
@@ -1702,10 +1805,11 @@ class Vd2dEngine final : public Engine {
     c_.ktimer.end(c_.stream, 1);
   }
 
-  // Tuner candidates: the 2-D tile heights (single step 8/16/32, two steps
-  // per launch 208/216/232 for heights 8/16/32), plus the z-marching strips (codes
-  // 102/104) when kMarchInTuner is set.  Any listed code can also be pinned
-  // through the environment.
+  // Tuner candidates: the 2-D tile heights 8/16/32, plus the z-marching
+  // strips (codes 102/104) when kMarchInTuner is set.  The environment can
+  // pin any variant, including two steps per launch (208/216/232 for tile
+  // heights 8/16/32), which measured slower per step than one step per
+  // launch on MI355X and is therefore not timed by default.
   static constexpr bool kMarchInTuner = false;
   static std::vector<int> candidates(std::vector<int> tiles) {
     if (kMarchInTuner) {
@@ -1725,7 +1829,7 @@ class Vd2dEngine final : public Engine {
       const int t = std::atoi(v);
       for (int c : cands)
         if (c == t) return t;
-      if (t == 102 || t == 104) return t;
+      if (t == 102 || t == 104 || t == 208 || t == 216 || t == 232) return t;
     }
     if (cands.size() == 1) return cands.front();
     hipEvent_t e0, e1;
