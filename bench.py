@@ -106,6 +106,59 @@ def workload_cd2d_grad_1000():
                         "check_freq": 50, "halo": halo})
 
 
+def workload_cd3d_600x600x400(nshots=2, nt=3000):
+    """3D acoustic constant-density gradient: 600 x 600 x 400 cells of 10 m,
+    dt = 1 ms, nt = 3000, C-PML (20 cells) on all faces, 7 Hz Ricker.  vp:
+    8 seeded layers (1500-5000 m/s with depth) plus 50 ellipsoidal
+    inclusions of +-10%; the starting model is the layered profile smoothed
+    over 100 m in depth.  Shots on a 2 x 4 surface grid at 20 m depth, each
+    with a 100 x 100 patch of receivers at 10 m spacing centred on it;
+    checkpoint every 50 steps.  The first `nshots` of the 8 shots are run as
+    one batch (two fit one plan in HBM with their checkpoints)."""
+    import seismicwaves as sw
+    from seismicwaves import synthetic as syn
+    nx, ny, nz = 600, 600, 400
+    h = (10.0, 10.0, 10.0)
+    dt, f0, halo = 1e-3, 7.0, 20
+    rng = np.random.default_rng(5)
+    z = np.arange(nz) * h[2]
+    ifaces = np.sort(rng.uniform(0.05, 0.95, 7)) * z[-1]
+    speeds = np.sort(rng.uniform(1500.0, 5000.0, 8))
+    prof = speeds[np.searchsorted(ifaces, z)].astype(np.float32)
+    vtrue = np.empty((nx, ny, nz), dtype=np.float32)
+    vtrue[:] = prof
+    X, Y, Z = (np.arange(n) * dh for n, dh in zip((nx, ny, nz), h))
+    for _ in range(50):
+        c = rng.uniform([0, 0, 0], [X[-1], Y[-1], Z[-1]])
+        ax = rng.uniform(60.0, 400.0, 3)
+        sl = [slice(max(0, int((c[d] - ax[d]) / h[d])), min(n, int((c[d] + ax[d]) / h[d]) + 1))
+              for d, n in enumerate((nx, ny, nz))]
+        gx, gy, gz = np.meshgrid(X[sl[0]], Y[sl[1]], Z[sl[2]], indexing="ij")
+        inside = ((gx - c[0]) / ax[0]) ** 2 + ((gy - c[1]) / ax[1]) ** 2 + ((gz - c[2]) / ax[2]) ** 2 <= 1.0
+        vtrue[sl[0], sl[1], sl[2]] *= np.where(inside, np.float32(1.0 + rng.uniform(-0.1, 0.1)), np.float32(1.0))
+    w = np.exp(-0.5 * (np.arange(-30, 31) * h[2] / 100.0) ** 2)
+    sprof = np.convolve(np.pad(prof.astype(np.float64), 30, mode="edge"), w / w.sum(), mode="valid")
+    v0 = np.empty_like(vtrue)
+    v0[:] = sprof.astype(np.float32)
+    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0)[:, None].astype(np.float32)
+    xs = np.array([1500.0, 4500.0])
+    ys = np.array([750.0, 2250.0, 3750.0, 5250.0])
+    pos = [(x, y) for y in ys for x in xs][:nshots]
+    off = (np.arange(100) - 50) * 10.0
+    shots = []
+    for x, y in pos:
+        rec = np.array([[x + a, y + b, 10.0] for a in off for b in off])
+        shots.append(sw.ScalarShot(sw.ScalarSources(np.array([[x, y, 20.0]]), tf, f0),
+                                   sw.ScalarReceivers(rec, nt, dtype=np.float32)))
+    params = sw.InputParametersAcoustic(nt, dt, (nx, ny, nz), h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, freeboundtop=False))
+    return dict(name=f"acoustic_cd_3d_gradient_600x600x400_{nshots}shots_nt{nt}", params=params, physics="cd3d",
+                mp_true=sw.VpAcousticCDMaterialProperties(vtrue), mp0=sw.VpAcousticCDMaterialProperties(v0),
+                shots=shots, check_freq=50, run={},
+                config={"nx": nx, "ny": ny, "nz": nz, "dh_m": 10.0, "dt_s": dt, "f0_hz": f0, "nt": nt,
+                        "shots": nshots, "of_shots": 8, "receivers": 10000, "check_freq": 50, "halo": halo})
+
+
 def workload_vd2d_batched_small():
     """The batched VD workload cut to 8 shots of nt 300..500 (for profiling
     runs, where every dispatch is instrumented)."""
@@ -195,7 +248,7 @@ def workload_el2d_batched(nt=5000):
 
 WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_cd2d_grad_1000,
              "vd2d_batched_small": workload_vd2d_batched_small, "el2d_batched": workload_el2d_batched,
-             "vd2d_batched_nt10": workload_vd2d_batched_nt10,
+             "vd2d_batched_nt10": workload_vd2d_batched_nt10, "cd3d_600x600x400": workload_cd3d_600x600x400,
              "el2d_batched_small": lambda: workload_el2d_batched(nt=500)}
 
 
@@ -203,10 +256,11 @@ WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_c
 # Algorithmic bytes per launch of the step kernels (each array counted once
 # per launch, perfect halo reuse).  e = bytes per value.
 # ---------------------------------------------------------------------------
-def _strip_cells(nx, nz, halo):
+def _strip_cells(n, halo):
     # C-PML memory variables touched per launch: psi (2h per axis) and xi
-    # (2h+2 per axis), each read and written once.
-    return (2 * halo * nz + 2 * halo * nx) + ((2 * halo + 2) * nz + (2 * halo + 2) * nx)
+    # (2h+2 per axis) over the face of each axis, each read and written once.
+    N = int(np.prod(n))
+    return sum((2 * halo + 2 * halo + 2) * (N // nd) for nd in n)
 
 
 def _launches_two_step(first, nt, ntmax, S, checkpoints):
@@ -221,13 +275,13 @@ def _launches_two_step(first, nt, ntmax, S, checkpoints):
     return n
 
 
-def alg_bytes(physics, nx, nz, halo, e, nts, S, variant=(0, 0)):
+def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
     """Returns {kernel family: (total bytes of one gradient, launches)} for
     family 0 (forward/replay step kernel) and 1 (adjoint step kernel);
     `variant` holds the kernel variants the plan picked."""
-    N = nx * nz
+    N = int(np.prod(n))
     ntmax = max(nts)
-    strip = _strip_cells(nx, nz, halo) * 2 * e
+    strip = _strip_cells(n, halo) * 2 * e
     fwd_b = fwd_n = adj_b = adj_n = 0
     for nt in nts:
         if physics == "vd2d":
@@ -301,6 +355,8 @@ def kernel_names(physics, variant):
                 return f"el_march_kernel<float, {v - 100}, {'true' if adj else 'false'}>"
             return "el_stress_kernel<float> + el_disp_kernel<float> (" + ("adjoint)" if adj else "forward)")
         return name(False, variant[0]), name(True, variant[1])
+    if physics == "cd3d":
+        return "cd3d_step_kernel<float> (forward)", "cd3d_step_kernel<float> (adjoint)"
     return "cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)"
 
 
@@ -386,6 +442,8 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
     cf = wl["check_freq"]
     if wl["physics"] == "el2d":
         return cpu_baseline_elastic(wl, cores, nt_sample=300)
+    if wl["physics"] == "cd3d":
+        return cpu_baseline_cd3d(wl, cores)
     vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
     shots = wl["shots"][:nshots_sample]
     rng = np.random.default_rng(0)
@@ -413,6 +471,33 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
             "sample": f"FP32 gradient of the same model: {len(shots)} shot(s) x nt={nt_sample} "
                       f"(check_freq={cf}), sequential shots, OpenMP threads={cores}; C restatement of the "
                       f"reference's threads backend (not the Julia binary); {el:.2f} s"}
+
+
+def cpu_baseline_cd3d(wl, cores, nt_sample=40, cf=20):
+    """The oracle's FP32 OpenMP 3D CD gradient on the workload's first shot,
+    nt_sample steps with a checkpoint every cf steps."""
+    import oracle
+    p = wl["params"]
+    h, dt = p.gridspacing, float(p.dt)
+    bc = p.boundcond
+    T = np.float32
+    vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
+    sh = wl["shots"][0]
+    cp = oracle.init_bdc(vp, dt, h, bc.halo, bc.rcoef, bc.freeboundtop, float(sh.srcs.domfreq), T)
+    si = oracle.find_nearest_grid_points(sh.srcs.positions, h)
+    ri = oracle.find_nearest_grid_points(sh.recs.positions, h)
+    stf = oracle.scale_srctf_cd(sh.srcs.tf[:nt_sample], h, dt, vp, si, T)
+    fact = ((T(dt) ** 2) * vp.astype(T) ** 2).astype(T)
+    adj = (np.random.default_rng(0).standard_normal((nt_sample, ri.shape[0])) * 1e-3).astype(T)
+    t0 = time.perf_counter()
+    oracle.cd3d_gradient(fact, h, dt, bc.halo, cp, nt_sample, si, stf, ri, adj, cf, T)
+    el = time.perf_counter() - t0
+    cells = int(np.prod(p.gridsize))
+    return {"value": round(2.0 * cells * nt_sample / el / 1e9, 4), "unit": "GCell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"FP32 gradient of the same model: 1 shot x nt={nt_sample} (check_freq={cf}), OpenMP "
+                      f"threads={cores}; C restatement of the reference's threads backend (not the Julia "
+                      f"binary); {el:.2f} s"}
 
 
 def cpu_baseline_elastic(wl, cores, nt_sample=300):
@@ -521,10 +606,9 @@ def main():
     S = info.segment
     for r in runs:
         r.close()
-    nx, nz = p.gridsize
-    cells = nx * nz
+    cells = int(np.prod(p.gridsize))
     value = 2.0 * cells * sum(nts) / (ms_step / 1e3) / 1e9
-    ab = alg_bytes(wl["physics"], nx, nz, p.boundcond.halo, 4, nts, S, list(info.kernel_variant))
+    ab = alg_bytes(wl["physics"], p.gridsize, p.boundcond.halo, 4, nts, S, list(info.kernel_variant))
     fam = {}
     for k, name in ((0, "forward/replay step"), (1, "adjoint step + correlation")):
         b, _ = ab[k]
