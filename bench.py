@@ -111,8 +111,9 @@ def workload_cd3d_600x600x400(nshots=2, nt=3000):
     dt = 1 ms, nt = 3000, C-PML (20 cells) on all faces, 7 Hz Ricker.  vp:
     8 seeded layers (1500-5000 m/s with depth) plus 50 ellipsoidal
     inclusions of +-10%; the starting model is the layered profile smoothed
-    over 100 m in depth.  Shots on a 2 x 4 surface grid at 20 m depth, each
-    with a 100 x 100 patch of receivers at 10 m spacing centred on it;
+    over 100 m in depth.  Shots on a 2 x 4 grid at 250 m depth (just below
+    the top C-PML), each with a 100 x 100 patch of receivers at 10 m spacing
+    centred on it at 240 m depth;
     checkpoint every 50 steps.  The first `nshots` of the 8 shots are run as
     one batch (two fit one plan in HBM with their checkpoints)."""
     import seismicwaves as sw
@@ -147,8 +148,8 @@ def workload_cd3d_600x600x400(nshots=2, nt=3000):
     off = (np.arange(100) - 50) * 10.0
     shots = []
     for x, y in pos:
-        rec = np.array([[x + a, y + b, 10.0] for a in off for b in off])
-        shots.append(sw.ScalarShot(sw.ScalarSources(np.array([[x, y, 20.0]]), tf, f0),
+        rec = np.array([[x + a, y + b, 240.0] for a in off for b in off])
+        shots.append(sw.ScalarShot(sw.ScalarSources(np.array([[x, y, 250.0]]), tf, f0),
                                    sw.ScalarReceivers(rec, nt, dtype=np.float32)))
     params = sw.InputParametersAcoustic(nt, dt, (nx, ny, nz), h,
                                         sw.CPMLBoundaryConditionParameters(halo=halo, freeboundtop=False))
@@ -353,7 +354,9 @@ def kernel_names(physics, variant):
         def name(adj, v):
             if v > 100:
                 return f"el_march_kernel<float, {v - 100}, {'true' if adj else 'false'}>"
-            return "el_stress_kernel<float> + el_disp_kernel<float> (" + ("adjoint)" if adj else "forward)")
+            # interior and other blocks of the stress and displacement steps
+            return ("el_stress_lean<float> + el_stress_kernel<float> + el_disp_lean<float> + "
+                    "el_disp_kernel<float> (" + ("adjoint)" if adj else "forward)"))
         return name(False, variant[0]), name(True, variant[1])
     if physics == "cd3d":
         return "cd3d_step_kernel<float> (forward)", "cd3d_step_kernel<float> (adjoint)"
@@ -434,16 +437,16 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     os.environ["OMP_NUM_THREADS"] = str(cores)
     p = wl["params"]
+    if wl["physics"] == "el2d":
+        return cpu_baseline_elastic(wl, cores, nt_sample=300)
+    if wl["physics"] == "cd3d":
+        return cpu_baseline_cd3d(wl, cores)
     nx, nz = p.gridsize
     h = p.gridspacing
     dt = float(p.dt)
     halo, rcoef, ft = p.boundcond.halo, p.boundcond.rcoef, p.boundcond.freeboundtop
     T = np.float32
     cf = wl["check_freq"]
-    if wl["physics"] == "el2d":
-        return cpu_baseline_elastic(wl, cores, nt_sample=300)
-    if wl["physics"] == "cd3d":
-        return cpu_baseline_cd3d(wl, cores)
     vp = np.asarray(wl["mp0"].vp, dtype=np.float64)
     shots = wl["shots"][:nshots_sample]
     rng = np.random.default_rng(0)

@@ -170,6 +170,146 @@ __device__ __forceinline__ int strip_full(int I, int n, int h) {  // half = fals
   return 0;
 }
 
+// Is block (bx, by) of shot s an interior block of the stress (stress =
+// true) or displacement step: every cell of the block takes the plain
+// 4th-order branch of each derivative helper (3 <= i <= n-2 along both
+// axes), lies outside every C-PML strip (h+2 <= i <= n-h-1), and no tap of
+// the launch's point family lies in its rows?
+template <typename T>
+__device__ __forceinline__ bool el_is_lean(const ElArgs<T> &a, int s, int bx, int by, bool stress) {
+  const int i0 = bx * TX + 1, i1 = i0 + TX - 1, j0 = by * TZ + 1, j1 = j0 + TZ - 1;
+  const int lo = max(3, a.h + 2);
+  if (!(i0 >= lo && i1 <= min(a.nx - 2, a.nx - a.h - 1) && j0 >= lo && j1 <= min(a.nz - 2, a.nz - a.h - 1)))
+    return false;
+  if (a.kind == (stress ? 1 : 2)) {
+    const int64_t o = (int64_t)s * (a.nrows + 1);
+    if (a.f0.rows[o + j0 - 1] != a.f0.rows[o + j1]) return false;
+    if (a.f1.rows[o + j0 - 1] != a.f1.rows[o + j1]) return false;
+  }
+  return true;
+}
+
+// Stress step (and, on the adjoint field, the correlation) of an interior
+// block: the expressions of el_stress_kernel, in the same operand order,
+// with every global load issued first.  Other blocks return at once;
+// el_stress_kernel, launched right after on the same grid, steps them.
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_stress_lean(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  const bool upd = a.update && a.it <= nts && nts > a.act_min;
+  const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
+  if (!upd && !corr) return;
+  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;
+  const int nx = a.nx;
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int64_t N = a.N;
+  const int c = (i - 1) + (j - 1) * nx;
+  const T *__restrict__ ux = a.ucur + (int64_t)s * 2 * N;
+  const T *__restrict__ uz = ux + N;
+  // ux along x (i-2..i+1) and z (j-1..j+2), uz along z (j-2..j+1) and x (i-1..i+2)
+  const T x_m2 = ux[c - 2], x_m1 = ux[c - 1], x_0 = ux[c], x_p1 = ux[c + 1];
+  const T x_zm1 = ux[c - nx], x_zp1 = ux[c + nx], x_zp2 = ux[c + 2 * nx];
+  const T z_m2 = uz[c - 2 * nx], z_m1 = uz[c - nx], z_0 = uz[c], z_p1 = uz[c + nx];
+  const T z_xm1 = uz[c - 1], z_xp1 = uz[c + 1], z_xp2 = uz[c + 2];
+  const T l = a.lam[c], m = a.mu[c], mhh = a.muhh[c];
+  T h0x = 0, h1x = 0, h2x = 0, h0z = 0, h1z = 0, h2z = 0;
+  T fx_m2 = 0, fx_m1 = 0, fx_p1 = 0, fx_zm1 = 0, fx_zp1 = 0, fx_zp2 = 0;
+  T fz_m2 = 0, fz_m1 = 0, fz_p1 = 0, fz_xm1 = 0, fz_xp1 = 0, fz_xp2 = 0;
+  T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+  T *g = a.grad + (int64_t)s * a.gstride;
+  if (corr) {
+    const int sl = a.corr_step - a.cbase + 1;   // slot of u^{corr_step}
+    const T *h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
+    const T *h1 = h0 + a.hslot;
+    const T *h2 = h1 + a.hslot;
+    h0x = h0[c]; h1x = h1[c]; h2x = h2[c];
+    h0z = h0[N + c]; h1z = h1[N + c]; h2z = h2[N + c];
+    fx_m2 = h1[c - 2]; fx_m1 = h1[c - 1]; fx_p1 = h1[c + 1];
+    fx_zm1 = h1[c - nx]; fx_zp1 = h1[c + nx]; fx_zp2 = h1[c + 2 * nx];
+    fz_m2 = h1[N + c - 2 * nx]; fz_m1 = h1[N + c - nx]; fz_p1 = h1[N + c + nx];
+    fz_xm1 = h1[N + c - 1]; fz_xp1 = h1[N + c + 1]; fz_xp2 = h1[N + c + 2];
+    g0 = g[c]; g1 = g[N + c]; g2 = g[2 * N + c]; g3 = g[3 * N + c]; g4 = g[4 * N + c];
+  }
+  const T dux = d4<T>(x_m2, x_m1, x_0, x_p1, a.idx);
+  const T duz = d4<T>(z_m2, z_m1, z_0, z_p1, a.idz);
+  const T duzx = d4<T>(z_xm1, z_0, z_xp1, z_xp2, a.idx);
+  const T duxz = d4<T>(x_zm1, x_0, x_zp1, x_zp2, a.idz);
+  if (upd) {
+    T *sg = a.sig + (int64_t)s * 3 * N;
+    const T dxc = dux, dzc = duz;
+    sg[c] = (l + T(2) * m) * dxc + l * dzc;
+    sg[N + c] = l * dxc + (l + T(2) * m) * dzc;
+    const T c1 = duzx, c2 = duxz;
+    sg[2 * N + c] = mhh * (c1 + c2);
+  }
+  if (corr) {
+    // correlate_gradients! of adjoint step corr_step (adjcur = ucur here)
+    const T v = x_0 * (h0x - T(2) * h1x + h2x) * a.idt2;
+    g[c] = g0 + v;
+    g[N + c] = g1 + z_0 * (h0z - T(2) * h1z + h2z) * a.idt2;
+    const T exx = d4<T>(fx_m2, fx_m1, h1x, fx_p1, a.idx);
+    const T ezz = d4<T>(fz_m2, fz_m1, h1z, fz_p1, a.idz);
+    const T dv = exx + ezz, dva = dux + duz;
+    g[2 * N + c] = g2 + dv * dva;
+    g[3 * N + c] = g3 + T(2) * (exx * dux + ezz * duz);
+    const T exz = (d4<T>(fz_xm1, h1z, fz_xp1, fz_xp2, a.idx) + d4<T>(fx_zm1, h1x, fx_zp1, fx_zp2, a.idz)) / T(2);
+    const T exza = (duzx + duxz) / T(2);
+    g[4 * N + c] = g4 + T(2) * (exz * exza + exz * exza);
+  }
+}
+
+// Displacement step of an interior block: the expressions of
+// el_disp_kernel, in the same operand order, with every global load issued
+// first, and the history store.  Other blocks return at once.
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;
+  const int nx = a.nx;
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int64_t N = a.N;
+  const int c = (i - 1) + (j - 1) * nx;
+  const T *__restrict__ uc = a.ucur + (int64_t)s * 2 * N;
+  T *__restrict__ un = a.unew + (int64_t)s * 2 * N;
+  const T *__restrict__ sxx = a.sig + (int64_t)s * 3 * N;
+  const T *__restrict__ szz = sxx + N;
+  const T *__restrict__ sxz = sxx + 2 * N;
+  const T a_m1 = sxx[c - 1], a_0 = sxx[c], a_p1 = sxx[c + 1], a_p2 = sxx[c + 2];
+  const T b_zm2 = sxz[c - 2 * nx], b_zm1 = sxz[c - nx], b_0 = sxz[c], b_zp1 = sxz[c + nx];
+  const T b_xm2 = sxz[c - 2], b_xm1 = sxz[c - 1], b_xp1 = sxz[c + 1];
+  const T z_m1 = szz[c - nx], z_0 = szz[c], z_p1 = szz[c + nx], z_p2 = szz[c + 2 * nx];
+  const T oldx = un[c], oldz = un[N + c], curx = uc[c], curz = uc[N + c];
+  const T rx = a.rdx[c], rz = a.rdz[c];
+  const T d1x = d4<T>(a_m1, a_0, a_p1, a_p2, a.idx);
+  const T d2x = d4<T>(b_zm2, b_zm1, b_0, b_zp1, a.idz);
+  const T nux = T(2) * curx - oldx + rx * (d1x + d2x);
+  const T d1z = d4<T>(b_xm2, b_xm1, b_0, b_xp1, a.idx);
+  const T d2z = d4<T>(z_m1, z_0, z_p1, z_p2, a.idz);
+  const T nuz = T(2) * curz - oldz + rz * (d1z + d2z);
+  un[c] = nux;
+  un[N + c] = nuz;
+  if (a.hmode) {
+    const int base = a.hmode == 1 ? (nts / a.S) * a.S : a.cbase;
+    if (a.it > base) {
+      T *hs = a.hist + (int64_t)s * 2 * N + c;
+      const int sl = a.it - base + 1;
+      hs[(int64_t)sl * a.hslot] = nux;
+      hs[(int64_t)sl * a.hslot + N] = nuz;
+      if (a.hmode == 1 && a.it == base + 1) {
+        hs[0] = oldx;
+        hs[N] = oldz;
+        hs[a.hslot] = curx;
+        hs[a.hslot + N] = curz;
+      }
+    }
+  }
+}
+
 // Stress update of one step (update_sxx_szz! :39-60, update_sxz! :62-79),
 // moment-tensor injection (:81-94) and, on the adjoint field, the fused
 // correlation of adjoint step corr_step (correlate_gradient_xPU.jl:1-82).
@@ -180,6 +320,7 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
   const bool upd = a.update && a.it <= nts && nts > a.act_min;
   const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
   if (!upd && !corr) return;
+  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;   // stepped by el_stress_lean
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
@@ -316,6 +457,7 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
   const int s = blockIdx.z;
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
+  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;   // stepped by el_disp_lean
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
@@ -1264,6 +1406,7 @@ class El2dEngine final : public Engine {
   void launch_stress(const ElArgs<T> &a, int kind) {
     dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
     c_.ktimer.begin(c_.stream);
+    el_stress_lean<T><<<grd, blk, 0, c_.stream>>>(a);
     el_stress_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
@@ -1271,6 +1414,7 @@ class El2dEngine final : public Engine {
   void launch_disp(const ElArgs<T> &a, int kind) {
     dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
     c_.ktimer.begin(c_.stream);
+    el_disp_lean<T><<<grd, blk, 0, c_.stream>>>(a);
     el_disp_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
