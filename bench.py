@@ -310,7 +310,7 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             adj_b += nt * (sa + 16) * e * N
             adj_n += (1 if march_a else 2) * nt
             continue
-        if physics == "vd2d" and variant[0] >= 200:
+        if physics == "vd2d" and 200 <= variant[0] < 300:
             # two steps per launch: a launch reads p, vx, vz, m0, m1x, m1z and
             # writes p, vx, vz once (9e) whether it advances the shot one or two
             # steps; + w p-history (1e) per stored step
@@ -343,12 +343,14 @@ def kernel_names(physics, variant):
     picked (variant codes of swx_plan_info.kernel_variant)."""
     if physics == "vd2d":
         def name(kind, v):
-            if v >= 200:
+            if v >= 300:   # interior and other tiles in one launch
+                return f"vd2d_{kind}_kernel<float, {v - 300}, true>"
+            if v >= 200:   # two steps per launch
                 return f"vd2d_{kind}2_kernel<float, {v - 200}, {512 if v - 200 > 8 else 256}>"
             if v > 100:
                 return f"vd2d_{kind}_march<float, {v - 100}>"
             # interior tiles and the other tiles: two kernels per step
-            return f"vd2d_{kind}_lean<float, {v}> + vd2d_{kind}_kernel<float, {v}>"
+            return f"vd2d_{kind}_lean<float, {v}> + vd2d_{kind}_kernel<float, {v}, false>"
         return name("fwd", variant[0]), name("adj", variant[1])
     if physics == "el2d":
         def name(adj, v):

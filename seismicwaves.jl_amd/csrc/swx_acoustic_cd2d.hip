@@ -70,7 +70,17 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   const int64_t so = (int64_t)s * a.N;
   const int64_t c = (int64_t)j * nx + i;
   const T *pc_ = a.pcur + so;
+  const T *po_ = a.pold + so;
+  T *pn_ = a.pnew + so;
+  const bool interior = (i >= 1 && i <= nx - 2 && j >= 1 && j <= nz - 2);
+  const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
+  // the loads that do not depend on a branch's outcome are issued together
   const T pc = pc_[c];
+  const T pl = interior ? pc_[c - 1] : T(0), pr = interior ? pc_[c + 1] : T(0);
+  const T pd = interior ? pc_[c - nx] : T(0), pu = interior ? pc_[c + nx] : T(0);
+  const T pov = po_[c], fa = a.fact[c];
+  const T gv = corr ? a.grad[so + c] : T(0);
+  const T dv = corr ? a.dbuf[(int64_t)(a.it - a.cbase) * a.dslot + so + c] : T(0);
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0);
   const AxisCoef<T> cz = axis_coef(cb, h, 1);
@@ -98,13 +108,9 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
     a.psiz_n[q] = psz;
   }
 
-  const bool interior = (i >= 1 && i <= nx - 2 && j >= 1 && j <= nz - 2);
-  const T *po_ = a.pold + so;
-  T *pn_ = a.pnew + so;
   T pnv = 0, flap = 0, inj = 0;
   if (interior) {
     // update_p_CPML! (acoustic2D_xPU.jl:27-46)
-    const T pl = pc_[c - 1], pr = pc_[c + 1], pd = pc_[c - nx], pu = pc_[c + nx];
     const T d2x = (pl - T(2) * pc + pr) * a.idx2;
     const T d2z = (pd - T(2) * pc + pu) * a.idz2;
     T lx = d2x, lz = d2z;
@@ -129,8 +135,8 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       a.xiz[qz] = xi;
       lz = d2z + dpsi + xi;
     }
-    flap = a.fact[c] * (lx + lz);
-    pnv = T(2) * pc - po_[c] + flap;
+    flap = fa * (lx + lz);
+    pnv = T(2) * pc - pov + flap;
   }
   // Points of this cell: inject_sources! / record_receivers! (forward,
   // acoustic2D_xPU.jl:48-62) or the prescaled residual injection of the
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
     if (u >= 0) {
       if (!interior) pnv = pn_[c];
       const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
-      const T f = a.fact[c];
+      const T f = fa;
       for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++) pnv += rs[(int64_t)nts * a.pt.urec_list[k]] * f;
       wrote = true;
     }
@@ -181,13 +187,12 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
       const T pn = wrote ? pnv : pn_[c];
       // p^{it} - 2 p^{it-1} + p^{it-2}: in the interior exactly the stencil
       // term plus the injection, formed without cancellation
-      a.dbuf[(int64_t)slot * a.dslot + so + c] = interior ? flap + inj : (po_[c] - T(2) * pc) + pn;
+      a.dbuf[(int64_t)slot * a.dslot + so + c] = interior ? flap + inj : (pov - T(2) * pc) + pn;
     }
   }
-  if (a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim) {
+  if (corr) {
     // correlate_gradient! of the previous adjoint step (adjcur = pcur here)
-    const int slot = a.it - a.cbase;
-    a.grad[so + c] += pc * a.dbuf[(int64_t)slot * a.dslot + so + c];
+    a.grad[so + c] = gv + pc * dv;
   }
 }
 

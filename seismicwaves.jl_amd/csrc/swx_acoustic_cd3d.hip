@@ -117,15 +117,40 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   T pm = (col_ok && K0 >= 1) ? pc_[(int64_t)(K0 - 1) * sxy + cxy] : T(0);
   T pc = col_ok ? pc_[(int64_t)K0 * sxy + cxy] : T(0);
   T psz_prev = 0;        // new psi_z of plane k-1 (valid if k-1 >= K0)
+  // Operands of plane k that do not depend on plane k-1's results are
+  // loaded one plane ahead, so their latency overlaps the previous plane's
+  // work (the loop would otherwise wait on every plane's loads).
+  const int cslot = a.it - a.cbase;
+  struct Plane { T pp, pl, pr, pd, pu, po, fa, g, d; };
+  auto load_plane = [&](int k) -> Plane {
+    Plane q{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!col_ok) return q;
+    const int64_t c = (int64_t)k * sxy + cxy;
+    q.pp = k + 1 < nz ? pc_[c + sxy] : T(0);
+    q.pl = i >= 1 ? pc_[c - 1] : T(0);
+    q.pr = i + 1 < nx ? pc_[c + 1] : T(0);
+    q.pd = j >= 1 ? pc_[c - nx] : T(0);
+    q.pu = j + 1 < ny ? pc_[c + nx] : T(0);
+    q.po = po_[c];
+    q.fa = a.fact[c];
+    if (corr) {
+      q.g = a.grad[so + c];
+      q.d = a.dbuf[(int64_t)cslot * a.dslot + so + c];
+    }
+    return q;
+  };
+  Plane nxt = load_plane(K0);
   for (int k = K0; k < K1; k++) {
     const int64_t c = (int64_t)k * sxy + cxy;
-    T pp = (col_ok && k + 1 < nz) ? pc_[c + sxy] : T(0);
+    const Plane cur = nxt;
+    if (k + 1 < K1) nxt = load_plane(k + 1);
+    const T pp = cur.pp;
     // wave-uniform: any point on this plane's rows of the block?
     const bool any_pt = rowi[jy0 + (int64_t)ny * k] < rowi[jy1 + (int64_t)ny * k];
     const bool any_rec = a.traces && rowr[jy0 + (int64_t)ny * k] < rowr[jy1 + (int64_t)ny * k];
     if (col_ok) {
-      const T pl = i >= 1 ? pc_[c - 1] : T(0), pr = i + 1 < nx ? pc_[c + 1] : T(0);
-      const T pd = j >= 1 ? pc_[c - nx] : T(0), pu = j + 1 < ny ? pc_[c + nx] : T(0);
+      const T pl = cur.pl, pr = cur.pr;
+      const T pd = cur.pd, pu = cur.pu;
       // update_psi_{x,y,z}! (acoustic3D_xPU.jl:1-38)
       T psx = 0, psy = 0, psz = 0;
       if (kx >= 0) {
@@ -187,8 +212,8 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
           a.xi[2][qz] = xi;
           lz = d2z + dpsi + xi;
         }
-        flap = a.fact[c] * (lx + ly + lz);
-        pnv = T(2) * pc - po_[c] + flap;
+        flap = cur.fa * (lx + ly + lz);
+        pnv = T(2) * pc - cur.po + flap;
       }
       bool wrote = interior;
       if (any_pt) {
@@ -205,7 +230,7 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
             }
           } else {
             const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
-            const T f = a.fact[c];
+            const T f = cur.fa;
             for (int q = a.pt.urec_ptr[u]; q < a.pt.urec_ptr[u + 1]; q++) pnv += rs[(int64_t)nts * a.pt.urec_list[q]] * f;
           }
           wrote = true;
@@ -224,12 +249,9 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
         // p^{it} - 2 p^{it-1} + p^{it-2}: in the interior this is exactly the
         // stencil term plus the injection, formed without cancellation
         const T pn = wrote ? pnv : pn_[c];
-        a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (po_[c] - T(2) * pc) + pn;
+        a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pn;
       }
-      if (corr) {
-        const int slot = a.it - a.cbase;
-        a.grad[so + c] += pc * a.dbuf[(int64_t)slot * a.dslot + so + c];
-      }
+      if (corr) a.grad[so + c] = cur.g + pc * cur.d;
       psz_prev = psz;
     }
     pm = pc;

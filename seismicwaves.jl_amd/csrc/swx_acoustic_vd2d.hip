@@ -120,26 +120,17 @@ __device__ __forceinline__ bool vd2d_fwd_is_lean(const Vd2dArgs<T> &a, int s, in
   return sr[J0 - 1] == sr[J0 + TZ + 2] && (a.traces == nullptr || rr[J0] == rr[J0 + TZ]);
 }
 
-// Forward / replay step on the interior tiles (vd2d_fwd_is_lean): the
-// updates of vd2d_fwd_kernel, in the same operand order, without per-cell
-// tests, on 32-bit offsets from per-shot bases.  Blocks of other tiles
-// return at once; vd2d_fwd_kernel, launched right after on the same grid,
-// steps those.
+// Forward / replay step of an interior tile (vd2d_fwd_is_lean): the updates
+// of vd2d_fwd_kernel, in the same operand order, without per-cell tests, on
+// 32-bit offsets from per-shot bases.
 template <typename T, int TZ>
-__global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
+__device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, int nts, int I0, int J0,
+                                                   T (&s_vx)[TZ + 3][TX + 6], T (&s_vz)[TZ + 6][TX + 3],
+                                                   T (&s_p)[TZ + 3][TX + 3]) {
   constexpr int R = TZ / BY;
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts || nts <= a.act_min) return;
-  const int I0 = bx * TX, J0 = bz * TZ;
-  if (!vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
   const int nx = a.nx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
-  __shared__ T s_vx[TZ + 3][TX + 6];
-  __shared__ T s_vz[TZ + 6][TX + 3];
-  __shared__ T s_p[TZ + 3][TX + 3];
   const T *__restrict__ po = a.p_o + so;
   const T *__restrict__ vxo = a.vx_o + so;
   const T *__restrict__ vzo = a.vz_o + so;
@@ -252,8 +243,27 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
   }
 }
 
-// Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.
+// Interior tiles of a forward / replay step; blocks of other tiles return at
+// once and vd2d_fwd_kernel<T, TZ, false>, launched right after on the same
+// grid, steps them.
 template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  if (!vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
+  __shared__ T s_vx[TZ + 3][TX + 6];
+  __shared__ T s_vz[TZ + 6][TX + 3];
+  __shared__ T s_p[TZ + 3][TX + 3];
+  vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p);
+}
+
+// Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.  Interior
+// tiles are stepped inline (FOLD, one launch per step) or left to
+// vd2d_fwd_lean.
+template <typename T, int TZ, bool FOLD>
 __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   constexpr int R = TZ / BY;
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -271,8 +281,10 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   __shared__ T s_vz[TZ + 6][TX + 3];   // vz^{it-1}: x in [I0-1, I0+TX+2), z in [J0-3, J0+TZ+3)
   __shared__ T s_p[TZ + 3][TX + 3];    // p^{it}:    x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2)
 
-  // interior tiles are stepped by vd2d_fwd_lean
-  if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) return;
+  if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) {
+    if (FOLD) vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p);
+    return;
+  }
 
   // ---- phase 0: every load that does not depend on LDS, all issued before
   // the first LDS store -------------------------------------------------------
@@ -925,19 +937,14 @@ __device__ __forceinline__ bool vd2d_adj_is_lean(const Vd2dArgs<T> &a, int s, in
   return rr[J0] == rr[J0 + TZ];
 }
 
-// Adjoint step on the interior tiles (vd2d_adj_is_lean): the updates and
+// Adjoint step of an interior tile (vd2d_adj_is_lean): the updates and
 // correlations of vd2d_adj_kernel, in the same operand order, without
-// per-cell tests.  Blocks of other tiles return at once; vd2d_adj_kernel,
-// launched right after on the same grid, steps those.
+// per-cell tests.
 template <typename T, int TZ>
-__global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
+__device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, int I0, int J0,
+                                                   T (&s_p)[TZ + 6][TX + 6], T (&s_pc)[TZ + 3][TX + 3],
+                                                   T (&s_vx)[TZ][TX + 3], T (&s_vz)[TZ + 3][TX]) {
   constexpr int R = TZ / BY;
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts) return;
-  const int I0 = bx * TX, J0 = bz * TZ;
-  if (!vd2d_adj_is_lean<TZ>(a, s, I0, J0)) return;
   const int nx = a.nx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
@@ -947,11 +954,6 @@ __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
   const T *__restrict__ pa = a.p_o + so;
   const T *__restrict__ vxo = a.vx_o + so;
   const T *__restrict__ vzo = a.vz_o + so;
-
-  __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
-  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
-  __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
-  __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
 
   // every global load of the thread is issued before its first LDS store.
   // Old adjoint pressure: full rows on the tile rows (x arm), tile columns
@@ -1087,9 +1089,28 @@ __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
   }
 }
 
-// Adjoint step it: (p~, v~)^{it+1} -> (p~, v~)^{it}, then the correlation
-// with p^{it}, p^{it-1} of the replayed segment.
+// Interior tiles of an adjoint step; blocks of other tiles return at once
+// and vd2d_adj_kernel<T, TZ, false>, launched right after on the same grid,
+// steps them.
 template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  if (!vd2d_adj_is_lean<TZ>(a, s, I0, J0)) return;
+  __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
+  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
+  __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
+  __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
+  vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz);
+}
+
+// Adjoint step it: (p~, v~)^{it+1} -> (p~, v~)^{it}, then the correlation
+// with p^{it}, p^{it-1} of the replayed segment.  Interior tiles are stepped
+// inline (FOLD, one launch per step) or left to vd2d_adj_lean.
+template <typename T, int TZ, bool FOLD>
 __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
   constexpr int R = TZ / BY;
   const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -1102,16 +1123,17 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
-  // interior tiles are stepped by vd2d_adj_lean
-  if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) return;
-  const int slot = a.it - a.cbase;
-  const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
-  const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
-
   __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
   __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
   __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
   __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
+  if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) {
+    if (FOLD) vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz);
+    return;
+  }
+  const int slot = a.it - a.cbase;
+  const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
+  const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
 
   // ---- phase 0: all global loads issued before the first LDS store ----------
   constexpr int QP = (TZ + 6) * (TX + 6), KQP = (QP + NT - 1) / NT;
@@ -1563,7 +1585,7 @@ class Vd2dEngine final : public Engine {
     zero(st_);
     if (!tz_fwd_) {
       const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max}),
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max, 308, 316, 300 + Tiles<T>::fwd_max}),
                      [&](int tz) { launch_fwd(a1, tz); });
     }
     for (int it = 1; it <= ntmax_;) {
@@ -1583,9 +1605,12 @@ class Vd2dEngine final : public Engine {
   void adjoint(bool per_shot) override {
     const int Kmax = (int)ceil_div(ntmax_, S_);
     if (!tz_adj_) {
-      // the top step of the last segment reads only forward-stored history
-      const Vd2dArgs<T> a1 = adj_args(ntmax_, (Kmax - 1) * S_);
-      tz_adj_ = tune("SWX_VD2D_TILE_ADJ", candidates({8, 16}), [&](int tz) { launch_adj(a1, tz); });
+      // timed on a step every shot takes part in (the shortest shot's last
+      // step); the history slots it reads are in range, their contents do
+      // not matter, and the adjoint state and gradients are cleared below
+      const int itt = *std::min_element(nt_.begin(), nt_.end());
+      const Vd2dArgs<T> a1 = adj_args(itt, ((itt - 1) / S_) * S_);
+      tz_adj_ = tune("SWX_VD2D_TILE_ADJ", candidates({16, 316}), [&](int tz) { launch_adj(a1, tz); });
     }
     zero(ad_);
     SWX_HIP(hipMemsetAsync(grad_.get(), 0, grad_.bytes(), c_.stream));
@@ -1668,7 +1693,7 @@ class Vd2dEngine final : public Engine {
   // read level it-1 and write level it into the other buffer; a two-step
   // launch also writes into the other buffer, so it toggles flip_.
   int slot(int k) const { return ((k % 2) + 2 + flip_) % 2; }
-  static int steps_per_launch(int code) { return code >= 200 ? 2 : 1; }
+  static int steps_per_launch(int code) { return code >= 200 && code < 300 ? 2 : 1; }
 
   Vd2dArgs<T> args(const DevBuf *b, int it, int rpar, int wpar) const {
     Vd2dArgs<T> a{};
@@ -1733,17 +1758,17 @@ class Vd2dEngine final : public Engine {
       return;
     }
     // interior tiles, then the others (each kernel skips the other's tiles)
-    // interior tiles, then the others (each kernel skips the other's tiles)
-    const unsigned g = grid(a, tz);
-    if (tz == 8) {
-      vd2d_fwd_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-      vd2d_fwd_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    } else if (tz == 16) {
-      vd2d_fwd_lean<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-      vd2d_fwd_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    } else {
-      vd2d_fwd_lean<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-      vd2d_fwd_kernel<T, Tiles<T>::fwd_max><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    // 8/16/32: interior tiles, then the others (each kernel skips the other's
+    // tiles); 308/316/332: one launch stepping both kinds of tiles
+    const unsigned g = grid(a, tz % 100);
+    const dim3 blk(TX, BY);
+    switch (tz) {
+      case 8: vd2d_fwd_lean<T, 8><<<g, blk, 0, c_.stream>>>(a); vd2d_fwd_kernel<T, 8, false><<<g, blk, 0, c_.stream>>>(a); break;
+      case 16: vd2d_fwd_lean<T, 16><<<g, blk, 0, c_.stream>>>(a); vd2d_fwd_kernel<T, 16, false><<<g, blk, 0, c_.stream>>>(a); break;
+      case 32: vd2d_fwd_lean<T, 32><<<g, blk, 0, c_.stream>>>(a); vd2d_fwd_kernel<T, 32, false><<<g, blk, 0, c_.stream>>>(a); break;
+      case 308: vd2d_fwd_kernel<T, 8, true><<<g, blk, 0, c_.stream>>>(a); break;
+      case 316: vd2d_fwd_kernel<T, 16, true><<<g, blk, 0, c_.stream>>>(a); break;
+      default: vd2d_fwd_kernel<T, 32, true><<<g, blk, 0, c_.stream>>>(a); break;
     }
     SWX_HIP(hipGetLastError());
   }
@@ -1756,14 +1781,14 @@ class Vd2dEngine final : public Engine {
       return;
     }
     // interior tiles, then the others (each kernel skips the other's tiles)
-    // interior tiles, then the others (each kernel skips the other's tiles)
-    const unsigned g = grid(a, tz);
-    if (tz == 8) {
-      vd2d_adj_lean<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-      vd2d_adj_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-    } else {
-      vd2d_adj_lean<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
-      vd2d_adj_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+    // 8/16: interior tiles, then the others; 308/316: one launch
+    const unsigned g = grid(a, tz % 100);
+    const dim3 blk(TX, BY);
+    switch (tz) {
+      case 8: vd2d_adj_lean<T, 8><<<g, blk, 0, c_.stream>>>(a); vd2d_adj_kernel<T, 8, false><<<g, blk, 0, c_.stream>>>(a); break;
+      case 16: vd2d_adj_lean<T, 16><<<g, blk, 0, c_.stream>>>(a); vd2d_adj_kernel<T, 16, false><<<g, blk, 0, c_.stream>>>(a); break;
+      case 308: vd2d_adj_kernel<T, 8, true><<<g, blk, 0, c_.stream>>>(a); break;
+      default: vd2d_adj_kernel<T, 16, true><<<g, blk, 0, c_.stream>>>(a); break;
     }
     SWX_HIP(hipGetLastError());
     SWX_HIP(hipGetLastError());
@@ -1805,8 +1830,12 @@ class Vd2dEngine final : public Engine {
     c_.ktimer.end(c_.stream, 1);
   }
 
-  // Tuner candidates: the 2-D tile heights 8/16/32, plus the z-marching
-  // strips (codes 102/104) when kMarchInTuner is set.  The environment can
+  // Tuner candidates: the 2-D tile heights 8/16/32 as an interior-tile launch
+  // plus an other-tile launch, the same heights as one launch (308/316/332),
+  // and the z-marching strips (codes 102/104) when kMarchInTuner is set.
+  // The adjoint times tile height 16 only: on the 64-shot batch the height-8
+  // variants measured 15% slower over whole runs, but the short tuning
+  // window could not tell them apart reliably.  The environment can
   // pin any variant, including two steps per launch (208/216/232 for tile
   // heights 8/16/32), which measured slower per step than one step per
   // launch on MI355X and is therefore not timed by default.
@@ -1840,7 +1869,7 @@ class Vd2dEngine final : public Engine {
     for (int c : cands) {
       launch(c);
       SWX_HIP(hipEventRecord(e0, c_.stream));
-      for (int r = 0; r < 3; r++) launch(c);
+      for (int r = 0; r < 5; r++) launch(c);
       SWX_HIP(hipEventRecord(e1, c_.stream));
       SWX_HIP(hipEventSynchronize(e1));
       float ms = 0;

@@ -118,8 +118,9 @@ def test_checkpoint_intervals_agree_bitwise(oracle_mod):
 
 def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
     """Every forward and adjoint kernel variant the autotuner may pick (2-D
-    tiles of height 8/16/32, two steps per launch on tiles of height
-    8/16/32, z-marching strips of 128/256 columns) gives the same seismograms
+    tiles of height 8/16/32 as interior + other-tile launches or as one
+    launch, two steps per launch on tiles of height 8/16/32, z-marching
+    strips of 128/256 columns) gives the same seismograms
     and gradients bit for bit, and matches the oracle.  The uneven shots
     (one ends on an odd step) and the odd replay length (39) make the
     two-step launches end with single steps; check_freq = 1 stores every
@@ -130,7 +131,8 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
     mp = sw.VpRhoAcousticVDMaterialProperties(p["v0"], p["r0"])
     outs = []
     for v, w, cf in (("8", "8", 40), ("16", "16", 40), ("32", "102", 40), ("102", "104", 40), ("104", "8", 40),
-                     ("208", "16", 40), ("216", "8", 40), ("232", "16", 40), ("216", "16", 1), ("208", "8", 7)):
+                     ("208", "16", 40), ("216", "8", 40), ("232", "16", 40), ("216", "16", 1), ("208", "8", 7),
+                     ("316", "316", 40), ("308", "308", 9), ("332", "16", 40)):
         print(f"[vd2d kernel variants forward {v} adjoint {w} check_freq {cf}]", flush=True)
         monkeypatch.setenv("SWX_VD2D_TILE_FWD", v)
         monkeypatch.setenv("SWX_VD2D_TILE_ADJ", w)
