@@ -356,9 +356,11 @@ def kernel_names(physics, variant):
         def name(adj, v):
             if v > 100:
                 return f"el_march_kernel<float, {v - 100}, {'true' if adj else 'false'}>"
-            # interior and other blocks of the stress and displacement steps
-            return ("el_stress_lean<float> + el_stress_kernel<float> + el_disp_lean<float> + "
-                    "el_disp_kernel<float> (" + ("adjoint)" if adj else "forward)"))
+            tail = " (adjoint)" if adj else " (forward)"
+            if v == 1:   # interior and other blocks in one launch per kernel
+                return "el_stress_kernel<float, true> + el_disp_kernel<float, true>" + tail
+            return ("el_stress_lean<float> + el_stress_kernel<float, false> + el_disp_lean<float> + "
+                    "el_disp_kernel<float, false>" + tail)
         return name(False, variant[0]), name(True, variant[1])
     if physics == "cd3d":
         return "cd3d_step_kernel<float> (forward)", "cd3d_step_kernel<float> (adjoint)"

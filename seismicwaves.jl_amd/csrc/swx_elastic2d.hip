@@ -191,16 +191,9 @@ __device__ __forceinline__ bool el_is_lean(const ElArgs<T> &a, int s, int bx, in
 
 // Stress step (and, on the adjoint field, the correlation) of an interior
 // block: the expressions of el_stress_kernel, in the same operand order,
-// with every global load issued first.  Other blocks return at once;
-// el_stress_kernel, launched right after on the same grid, steps them.
+// with every global load issued first.
 template <typename T>
-__global__ __launch_bounds__(TX *TZ) void el_stress_lean(ElArgs<T> a) {
-  const int s = blockIdx.z;
-  const int nts = a.nt_shot[s];
-  const bool upd = a.update && a.it <= nts && nts > a.act_min;
-  const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
-  if (!upd && !corr) return;
-  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;
+__device__ __forceinline__ void el_stress_lean_block(const ElArgs<T> &a, int s, bool upd, bool corr) {
   const int nx = a.nx;
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
@@ -262,13 +255,9 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_lean(ElArgs<T> a) {
 
 // Displacement step of an interior block: the expressions of
 // el_disp_kernel, in the same operand order, with every global load issued
-// first, and the history store.  Other blocks return at once.
+// first, and the history store.
 template <typename T>
-__global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
-  const int s = blockIdx.z;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts || nts <= a.act_min) return;
-  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;
+__device__ __forceinline__ void el_disp_lean_block(const ElArgs<T> &a, int s, int nts) {
   const int nx = a.nx;
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
@@ -310,17 +299,42 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
   }
 }
 
+// Interior blocks of a stress / displacement step; other blocks return at
+// once and el_stress_kernel / el_disp_kernel<T, false>, launched right
+// after on the same grid, step them.
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_stress_lean(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  const bool upd = a.update && a.it <= nts && nts > a.act_min;
+  const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
+  if (!upd && !corr) return;
+  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;
+  el_stress_lean_block(a, s, upd, corr);
+}
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
+  const int s = blockIdx.z;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;
+  el_disp_lean_block(a, s, nts);
+}
+
 // Stress update of one step (update_sxx_szz! :39-60, update_sxz! :62-79),
 // moment-tensor injection (:81-94) and, on the adjoint field, the fused
 // correlation of adjoint step corr_step (correlate_gradient_xPU.jl:1-82).
-template <typename T>
+template <typename T, bool FOLD>
 __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
   const int s = blockIdx.z;
   const int nts = a.nt_shot[s];
   const bool upd = a.update && a.it <= nts && nts > a.act_min;
   const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
   if (!upd && !corr) return;
-  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;   // stepped by el_stress_lean
+  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) {   // interior block
+    if (FOLD) el_stress_lean_block(a, s, upd, corr);
+    return;
+  }
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
@@ -452,12 +466,15 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
 
 // Displacement update (update_ux! :1-18, update_uz! :20-37) into the slot of
 // uold, force / adjoint-residual injection (:96-106) and the forward history.
-template <typename T>
+template <typename T, bool FOLD>
 __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
   const int s = blockIdx.z;
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
-  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;   // stepped by el_disp_lean
+  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) {   // interior block
+    if (FOLD) el_disp_lean_block(a, s, nts);
+    return;
+  }
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
@@ -1334,7 +1351,7 @@ class El2dEngine final : public Engine {
       a.corr_step = cb + 1;
       a.corr_lim = cb + S_;
       a.cbase = cb;
-      launch_stress(a, 1);
+      launch_stress(a, 1, var_adj_ == 1);
     }
     const int64_t G = 5 * N_;
     dim3 g((unsigned)std::min<int64_t>(ceil_div(G, 256), 4096));
@@ -1403,19 +1420,29 @@ class El2dEngine final : public Engine {
     return a;
   }
 
-  void launch_stress(const ElArgs<T> &a, int kind) {
+  // fold: interior and other blocks in one launch; otherwise an interior
+  // launch followed by an other-block launch
+  void launch_stress(const ElArgs<T> &a, int kind, bool fold) {
     dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
     c_.ktimer.begin(c_.stream);
-    el_stress_lean<T><<<grd, blk, 0, c_.stream>>>(a);
-    el_stress_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    if (fold) {
+      el_stress_kernel<T, true><<<grd, blk, 0, c_.stream>>>(a);
+    } else {
+      el_stress_lean<T><<<grd, blk, 0, c_.stream>>>(a);
+      el_stress_kernel<T, false><<<grd, blk, 0, c_.stream>>>(a);
+    }
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
   }
-  void launch_disp(const ElArgs<T> &a, int kind) {
+  void launch_disp(const ElArgs<T> &a, int kind, bool fold) {
     dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
     c_.ktimer.begin(c_.stream);
-    el_disp_lean<T><<<grd, blk, 0, c_.stream>>>(a);
-    el_disp_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    if (fold) {
+      el_disp_kernel<T, true><<<grd, blk, 0, c_.stream>>>(a);
+    } else {
+      el_disp_lean<T><<<grd, blk, 0, c_.stream>>>(a);
+      el_disp_kernel<T, false><<<grd, blk, 0, c_.stream>>>(a);
+    }
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
   }
@@ -1445,20 +1472,22 @@ class El2dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
 
-  // Variants: 0 = stress and displacement launches, 102/104 = one
-  // z-marching launch on 128/256-column strips.  The tuner times the
-  // candidates (only 0 unless kMarchInTuner); the environment can pin one.
+  // Variants: 0 = stress and displacement steps each as an interior-block
+  // launch plus an other-block launch, 1 = the same steps with both kinds of
+  // blocks in one launch each, 102/104 = one z-marching launch on
+  // 128/256-column strips.  The tuner times 0 and 1 (and the z-marching
+  // variants when kMarchInTuner is set); the environment can pin any.
   static constexpr bool kMarchInTuner = false;
   template <typename F>
   int tune(const char *env, F step) {
-    std::vector<int> cands = {0};
+    std::vector<int> cands = {0, 1};
     if (kMarchInTuner) {
       cands.push_back(102);
       cands.push_back(104);
     }
     if (const char *v = std::getenv(env)) {
       const int t = std::atoi(v);
-      if (t == 0 || t == 102 || t == 104) return t;
+      if (t == 0 || t == 1 || t == 102 || t == 104) return t;
     }
     if (cands.size() == 1) return cands[0];
     hipEvent_t e0, e1;
@@ -1482,7 +1511,7 @@ class El2dEngine final : public Engine {
   }
 
   void step_fwd_v(int v, int it, int act_min, int hmode, int cbase, T *traces) {
-    if (v != 0) {
+    if (v > 100) {
       ElArgs<T> a = base_args(it);
       a.ucur = U(u_, it - 1);
       a.unew = U(u_, it);
@@ -1518,11 +1547,11 @@ class El2dEngine final : public Engine {
     a.tf = c_.pts.d_srctf.as<T>();
     a.ncomp = 2;
     a.kind = kind_ == SWX_SRC_MOMENT_TENSOR ? 1 : 0;
-    launch_stress(a, 0);
+    launch_stress(a, 0, v == 1);
     a.kind = kind_ == SWX_SRC_EXTERNAL_FORCE ? 2 : 0;
     a.hmode = c_.d.gradient ? hmode : 0;
     a.cbase = cbase;
-    launch_disp(a, 0);
+    launch_disp(a, 0, v == 1);
     record(it, traces);
   }
 
@@ -1537,7 +1566,7 @@ class El2dEngine final : public Engine {
   }
 
   void step_adj_v(int v, int it, int cbase) {
-    if (v != 0) {
+    if (v > 100) {
       ElArgs<T> a = base_args(it);
       a.ucur = U(au_, it + 1);
       a.unew = U(au_, it);
@@ -1566,7 +1595,7 @@ class El2dEngine final : public Engine {
     a.corr_step = it + 1;
     a.corr_lim = cbase + S_;
     a.kind = 0;
-    launch_stress(a, 1);
+    launch_stress(a, 1, v == 1);
     a.corr_step = 0;
     a.kind = 2;
     a.f0 = taps_[2].view<T>();
@@ -1574,7 +1603,7 @@ class El2dEngine final : public Engine {
     a.tf = c_.pts.d_res.as<T>();
     a.off_tf = c_.pts.d_off_tr.as<int64_t>();
     a.ncomp = 2;
-    launch_disp(a, 1);
+    launch_disp(a, 1, v == 1);
   }
 
   // checkpoint k (k = 1 .. floor(nt_max/S)): both displacement levels

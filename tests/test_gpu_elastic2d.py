@@ -159,8 +159,9 @@ def test_point_edge_cases(oracle_mod):
 @pytest.mark.parametrize("kind,freetop", [("mt", True), ("ef", False)])
 def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     """Every forward and adjoint kernel variant the autotuner may pick (the
-    two-kernel stress/displacement step, z-marching strips of 128/256
-    columns) gives the same seismograms and gradients bit for bit, on a grid
+    stress/displacement step as interior + other-block launches or as one
+    launch each, z-marching strips of 128/256 columns) gives the same
+    seismograms and gradients bit for bit, on a grid
     that needs several strips and a partial last one, and matches the
     oracle."""
     p = _problem(np.float64, kind=kind, nts=(200, 170), n=(301, 96), freetop=freetop)
@@ -168,7 +169,7 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     mis = [sw.L2Misfit(o) for o in obs]
     mp = _matprop(p["init"], np.float64)
     outs = []
-    for v, w in (("0", "0"), ("102", "104"), ("104", "102"), ("0", "104")):
+    for v, w in (("0", "0"), ("1", "1"), ("102", "104"), ("104", "102"), ("0", "104"), ("1", "0")):
         print(f"[el2d kernel variants {kind} forward {v} adjoint {w}]", flush=True)
         monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", v)
         monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", w)
