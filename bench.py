@@ -363,7 +363,7 @@ def kernel_names(physics, variant):
                     "el_disp_kernel<float, false>" + tail)
         return name(False, variant[0]), name(True, variant[1])
     if physics == "cd3d":
-        return "cd3d_step_kernel<float> (forward)", "cd3d_step_kernel<float> (adjoint)"
+        return "cd3d_step_kernel<float, 4, 0> (forward)", "cd3d_step_kernel<float, 4, 0> (adjoint)"
     return "cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)"
 
 

@@ -9,12 +9,14 @@
 //   driver     src/models/acoustic/acou_gradient.jl:4-95
 //   correlate  src/models/acoustic/backends/shared/correlate_gradient_xPU.jl:1-10
 //
-// Kernel shape: a block is a 64 x 8 column tile of the (x, y) plane that
+// Kernel shape: a block is a 64 x 4 column tile of the (x, y) plane that
 // marches along z (the slow axis) over a chunk of planes.  Each thread keeps
 // p(k-1), p(k), p(k+1) in registers and rotates them, so the z neighbours
 // and the current plane are read from HBM once; x/y neighbours come from
 // the L1/L2 lines the neighbouring lanes load anyway.  The z C-PML memory
 // variable of plane k-1 stays in a register for the xi update of plane k.
+// Tiles clear of the x/y C-PML strips, the grid edges and the points take a
+// march with no per-cell tests (cd3d_lean_column) in the same launch.
 //
 // Device layout (x fastest), one slab per shot:
 //   p       [3][ns][N]            p^k lives in slot k mod 3 (pold/pcur/pnew rotation)
@@ -32,12 +34,13 @@
 namespace swx {
 namespace {
 
-constexpr int TX = 64, TY = 8;
+constexpr int TX = 64;
 
 template <typename T>
 struct Cd3dArgs {
   int n[3], h;
   int nbx, nby, nbk, kch;       // tiles along x, y; z chunks; planes per chunk
+  int lean;                     // interior column tiles take cd3d_lean_column
   int64_t N;
   T id[3], id2[3];
   const T *fact;
@@ -72,7 +75,123 @@ __device__ __forceinline__ int psi_owner(int i, int n, int h) {
   return -1;
 }
 
+// Interior column tile of the 3D step (cd3d_is_lean): no x/y C-PML strip or
+// grid edge in reach of the tile, and no source / receiver on its rows in
+// any plane of its z chunk.  The z-march of cd3d_step_kernel with the same
+// expressions (z C-PML and the edge planes handled per plane), without the
+// per-cell x/y tests and point lookups, on 32-bit in-plane offsets.
+template <typename T, int TY>
+__device__ __forceinline__ bool cd3d_is_lean(const Cd3dArgs<T> &a, int s, int bx, int by, int K0, int K1) {
+  const int nx = a.n[0], ny = a.n[1], h = a.h;
+  const int i0 = bx * TX, i1 = i0 + TX - 1, j0 = by * TY, j1 = j0 + TY - 1;
+  if (!(i0 >= max(h, 1) && i1 <= nx - h - 2 && j0 >= max(h, 1) && j1 <= ny - h - 2)) return false;
+  // the planes of the chunk spread over the threads of the block
+  const int *rowp = (a.res == nullptr ? a.pt.srow : a.pt.rrow) + (int64_t)s * (a.pt.nrows + 1);
+  const int *rowr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
+  int hit = 0;
+  for (int k = K0 + threadIdx.y * TX + threadIdx.x; k < K1; k += TX * TY) {
+    const int64_t r0 = j0 + (int64_t)ny * k, r1 = j1 + 1 + (int64_t)ny * k;
+    hit |= rowp[r0] != rowp[r1] || (a.traces && rowr[r0] != rowr[r1]);
+  }
+  return !__syncthreads_or(hit);
+}
+
 template <typename T>
+__device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, int nts, int i, int j, int K0,
+                                                 int K1) {
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2], h = a.h;
+  const int64_t so = (int64_t)s * a.N;
+  const int64_t sxy = (int64_t)nx * ny;
+  const T *__restrict__ pc_ = a.pcur + so;
+  const T *__restrict__ po_ = a.pold + so;
+  T *__restrict__ pn_ = a.pnew + so;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cz = axis_coef(cb, h, 2);
+  int dslot = -1;
+  if (a.dmode == 1) {
+    const int seg0 = ((nts - 1) / a.S) * a.S;
+    if (a.it > seg0) dslot = a.it - seg0 - 1;
+  } else if (a.dmode == 2) {
+    dslot = a.it - a.cbase - 1;
+  }
+  const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
+  const int cslot = a.it - a.cbase;
+  const int cxy = j * nx + i;
+  T pm = K0 >= 1 ? pc_[(int64_t)(K0 - 1) * sxy + cxy] : T(0);
+  T pc = pc_[(int64_t)K0 * sxy + cxy];
+  T psz_prev = 0;
+  struct Plane { T pp, pl, pr, pd, pu, po, fa, g, d; };
+  auto load_plane = [&](int k) -> Plane {
+    Plane q;
+    const int64_t c = (int64_t)k * sxy + cxy;
+    q.pp = k + 1 < nz ? pc_[c + sxy] : T(0);
+    q.pl = pc_[c - 1];
+    q.pr = pc_[c + 1];
+    q.pd = pc_[c - nx];
+    q.pu = pc_[c + nx];
+    q.po = po_[c];
+    q.fa = a.fact[c];
+    q.g = corr ? a.grad[so + c] : T(0);
+    q.d = corr ? a.dbuf[(int64_t)cslot * a.dslot + so + c] : T(0);
+    return q;
+  };
+  Plane nxt = load_plane(K0);
+  for (int k = K0; k < K1; k++) {
+    const int64_t c = (int64_t)k * sxy + cxy;
+    const Plane cur = nxt;
+    if (k + 1 < K1) nxt = load_plane(k + 1);
+    const T pp = cur.pp;
+    T psz = 0;
+    const int kz = psi_owner(k, nz, h);
+    if (kz >= 0) {
+      const int64_t q = (int64_t)s * a.sps[2] + (int64_t)kz * sxy + cxy;
+      psz = cz.bh(kz) * a.psi_o[2][q] + cz.ah(kz) * ((pp - pc) * a.id[2]);
+      a.psi_n[2][q] = psz;
+    }
+    const bool interior = k >= 1 && k <= nz - 2;
+    T pnv = 0, flap = 0;
+    if (interior) {
+      const T d2x = (cur.pl - T(2) * pc + cur.pr) * a.id2[0];
+      const T d2y = (cur.pd - T(2) * pc + cur.pu) * a.id2[1];
+      const T d2z = (pm - T(2) * pc + pp) * a.id2[2];
+      const T lx = d2x, ly = d2y;
+      T lz = d2z;
+      if (k < h || k >= nz - h) {
+        const int kn = kz - 1;
+        T psn;
+        if (k > K0) {
+          psn = psz_prev;
+        } else {
+          const int64_t qn = (int64_t)s * a.sps[2] + (int64_t)kn * sxy + cxy;
+          psn = cz.bh(kn) * a.psi_o[2][qn] + cz.ah(kn) * ((pc - pm) * a.id[2]);
+        }
+        const T dpsi = (psz - psn) * a.id[2];
+        const int64_t qz = (int64_t)s * a.sxi[2] + (int64_t)kz * sxy + cxy;
+        const T xi = cz.b(kz) * a.xi[2][qz] + cz.a(kz) * (d2z + dpsi);
+        a.xi[2][qz] = xi;
+        lz = d2z + dpsi + xi;
+      }
+      flap = cur.fa * (lx + ly + lz);
+      pnv = T(2) * pc - cur.po + flap;
+      pn_[c] = pnv;
+    }
+    if (dslot >= 0) {
+      const T inj = 0;
+      const T pn = interior ? pnv : pn_[c];
+      a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pn;
+    }
+    if (corr) a.grad[so + c] = cur.g + pc * cur.d;
+    psz_prev = psz;
+    pm = pc;
+    pc = pp;
+  }
+}
+
+// TY: tile height (rows of the block); MODE 0 takes interior tiles through
+// cd3d_lean_column and the others through the general march in one launch,
+// MODE 1 runs only the interior tiles and MODE 2 only the others (the two
+// launched one after the other, so each gets its own register budget).
+template <typename T, int TY, int MODE>
 __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   const int b = xcd_tile3(blockIdx.x, gridDim.x);
   int t = b;
@@ -85,6 +204,15 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], h = a.h;
   const int i = bx * TX + threadIdx.x, j = by * TY + threadIdx.y;
   const int K0 = bk * a.kch, K1 = min(K0 + a.kch, nz);
+  if (MODE == 1 || (a.lean && MODE == 0)) {
+    if (cd3d_is_lean<T, TY>(a, s, bx, by, K0, K1)) {
+      if (MODE < 2) cd3d_lean_column(a, s, nts, i, j, K0, K1);
+      return;
+    }
+    if (MODE == 1) return;
+  } else if (MODE == 2) {
+    if (cd3d_is_lean<T, TY>(a, s, bx, by, K0, K1)) return;
+  }
   const bool col_ok = i < nx && j < ny;
   const int64_t so = (int64_t)s * a.N;
   const int64_t sxy = (int64_t)nx * ny;
@@ -312,8 +440,22 @@ class Cd3dEngine final : public Engine {
       nck_ = std::max(K_ - 2, 0);
       ck_.alloc(std::max<int64_t>(nck_ * ck_one_, 1));
     }
-    kch_ = 32;
+    // 64 x 4 tiles in one launch, 16 planes per chunk: measured best of
+    // 64 x {8, 4, 2} tiles, one or two launches and 16 / 32 / 64 planes on
+    // 600 x 600 x 400 (the smaller blocks let more of them share a CU than
+    // the general path's register count would with 64 x 8)
+    kch_ = 16;
     if (const char *v = std::getenv("SWX_CD3D_ZCHUNK")) kch_ = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("SWX_CD3D_LEAN")) lean_ = std::atoi(v) != 0;
+    // launch shape: 8 / 4 / 2 = one launch with 64 x 8 / 64 x 4 / 64 x 2
+    // tiles, 108 / 104 / 102 = interior tiles and the others in two launches
+    if (const char *v = std::getenv("SWX_CD3D_VARIANT")) {
+      const int c = std::atoi(v);
+      SWX_REQUIRE(c % 100 == 8 || c % 100 == 4 || c % 100 == 2, SWX_EINVAL,
+                  "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
+      SWX_REQUIRE(c < 200, SWX_EINVAL, "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
+      variant_ = c;
+    }
     zero_forward_state();
   }
 
@@ -417,9 +559,10 @@ class Cd3dEngine final : public Engine {
     a.h = h_;
     a.N = N_;
     a.nbx = (int)ceil_div(n_[0], TX);
-    a.nby = (int)ceil_div(n_[1], TY);
+    a.nby = 0;  // set per launch shape
     a.kch = kch_;
     a.nbk = (int)ceil_div(n_[2], kch_);
+    a.lean = lean_ ? 1 : 0;
     a.fact = fact_.as<T>();
     a.cpml = cpml_.as<T>();
     a.cstride = cstride_;
@@ -433,10 +576,24 @@ class Cd3dEngine final : public Engine {
     return a;
   }
 
-  void launch(const Cd3dArgs<T> &a, int kind) {
+  template <int TY>
+  void launch_shape(Cd3dArgs<T> a, bool split) {
+    a.nby = (int)ceil_div(n_[1], TY);
     const unsigned nb = (unsigned)((int64_t)a.nbx * a.nby * a.nbk * nsh_);
+    if (split && a.lean) {
+      cd3d_step_kernel<T, TY, 1><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
+      cd3d_step_kernel<T, TY, 2><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
+    } else {
+      cd3d_step_kernel<T, TY, 0><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
+    }
+  }
+
+  void launch(const Cd3dArgs<T> &a, int kind) {
     c_.ktimer.begin(c_.stream);
-    cd3d_step_kernel<T><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
+    const int ty = variant_ % 100;
+    if (ty == 4) launch_shape<4>(a, variant_ > 100);
+    else if (ty == 2) launch_shape<2>(a, variant_ > 100);
+    else launch_shape<8>(a, variant_ > 100);
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
   }
@@ -495,7 +652,9 @@ class Cd3dEngine final : public Engine {
   PlanCore &c_;
   int n_[3], h_, ns_;
   int64_t N_, sps_[3], sxi_[3];
-  int cstride_, kch_ = 32;
+  int cstride_, kch_ = 16;
+  bool lean_ = true;
+  int variant_ = 4;
   DevBuf fact_, p_, psi_[3], xi_[3], cpml_;
   DevBuf a_, apsi_[3], axi_[3], grad_, gout_;
   DevBuf dbuf_, ck_;

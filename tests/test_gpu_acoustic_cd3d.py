@@ -112,3 +112,67 @@ def test_point_edge_cases(oracle_mod):
                       runparams=sw.RunParameters(), gradparams=sw.GradParameters(check_freq=1))
     assert rel_l2(shots[0].recs.seismograms, sref[0]) <= 1e-10
     assert rel_l2(g["vp"], gref["vp"]) <= 1e-10
+
+
+def _interior_problem(dtype, nt=150):
+    """A grid wide enough for 64 x 8 column tiles clear of the x/y C-PML strips
+    (the test-free interior path), with points on some of their rows, so
+    that in the z chunks that hold them the same tiles take the general path."""
+    n, h, dt, f0, halo = (150, 44, 36), (10.0, 10.0, 10.0), 1e-3, 8.0, 8
+    rng = np.random.default_rng(21)
+    vtrue = syn.gaussian_smooth(2000 + 400 * rng.random(n), 2.0)
+    v0 = np.full(n, float(vtrue.mean()))
+    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0)[:, None]
+    src = np.array([[900.0, 200.0, 120.0]])
+    rec = np.array([[x * h[0], 280.0, 30.0] for x in range(70, 131, 5)])
+    shots = [sw.ScalarShot(sw.ScalarSources(src, tf.astype(dtype), f0), sw.ScalarReceivers(rec, nt, dtype=dtype))]
+    osh = [dict(src_pos=src, srctf=tf, f0=f0, rec_pos=rec, nt=nt)]
+    params = sw.InputParametersAcoustic(nt, dt, n, h, sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4))
+    return dict(params=params, vtrue=vtrue, v0=v0, shots=shots, osh=osh, h=h, dt=dt, halo=halo, nt=nt)
+
+
+def test_interior_tiles_against_oracle(oracle_mod):
+    """FP64 seismograms and gradient on a grid whose interior tiles take the
+    test-free path."""
+    print("[cd3d interior tiles float64]", flush=True)
+    p = _interior_problem(np.float64)
+    obs = oracle_mod.acoustic_cd3d_forward(p["vtrue"], p["h"], p["dt"], p["nt"], p["halo"], 1e-4, True, p["osh"])
+    sw.swforward(p["params"], sw.VpAcousticCDMaterialProperties(p["vtrue"]), p["shots"], runparams=sw.RunParameters())
+    assert np.max(np.abs(obs[0])) > 0
+    assert rel_l2(p["shots"][0].recs.seismograms, obs[0]) <= 1e-10
+    osh = [dict(o, observed=ob) for o, ob in zip(p["osh"], obs)]
+    gref, _, _ = oracle_mod.acoustic_cd3d_gradient(p["v0"], p["h"], p["dt"], p["nt"], p["halo"], 1e-4, True, osh,
+                                                   check_freq=30)
+    g = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"]), p["shots"],
+                      [sw.L2Misfit(o) for o in obs], runparams=sw.RunParameters(),
+                      gradparams=sw.GradParameters(check_freq=30))
+    err = rel_l2(g["vp"], gref["vp"])
+    print(f"rel-L2 gradient = {err:.3e}")
+    assert np.max(np.abs(gref["vp"])) > 0
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("chunk", ["32", "7"])
+def test_interior_path_agrees_bitwise(monkeypatch, chunk):
+    """The test-free interior path gives the general path's results bit for
+    bit (seismograms and gradient, FP32), with 64 x 8, 64 x 4 and 64 x 2 tiles, in one
+    launch or two."""
+    p = _interior_problem(np.float32)
+    rng = np.random.default_rng(1)
+    mis = [sw.L2Misfit(rng.standard_normal(sh.recs.seismograms.shape).astype(np.float32) * 1e-3) for sh in p["shots"]]
+    mp = sw.VpAcousticCDMaterialProperties(p["vtrue"].astype(np.float32))
+    monkeypatch.setenv("SWX_CD3D_ZCHUNK", chunk)
+    outs = []
+    for variant, lean in (("8", "1"), ("8", "0"), ("4", "1"), ("4", "0"), ("2", "1"), ("108", "1"), ("104", "1"),
+                          ("102", "1")):
+        print(f"[cd3d z chunk {chunk}, launch shape {variant}, interior path {lean}]", flush=True)
+        monkeypatch.setenv("SWX_CD3D_VARIANT", variant)
+        monkeypatch.setenv("SWX_CD3D_LEAN", lean)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=20))["vp"]
+        sw.swforward(p["params"], mp, p["shots"], runparams=sw.RunParameters())
+        outs.append((g, p["shots"][0].recs.seismograms.copy()))
+    assert np.max(np.abs(outs[0][0])) > 0
+    for g, seis in outs[1:]:
+        assert np.array_equal(outs[0][0], g)
+        assert np.array_equal(outs[0][1], seis)
