@@ -19,6 +19,7 @@
 //   ckpt   [K-2][...]      p^{c-1}, p^c, psi, xi at c = k*S
 //   grad   [ns][nx*nz]     per-shot raw gradient (reduced in shot order)
 #include <algorithm>
+#include <cstdlib>
 
 #include "swx_device.hpp"
 #include "swx_internal.hpp"
@@ -27,7 +28,7 @@ namespace swx {
 namespace {
 
 constexpr int TX = 64;  // one wavefront along the fast axis
-constexpr int TZ = 4;
+constexpr int TZ = 4;   // waves per block; each thread steps R rows TZ apart
 
 template <typename T>
 struct Cd2dStepArgs {
@@ -56,15 +57,85 @@ struct Cd2dStepArgs {
   const T *srctf;
   T *traces;
   const T *res;
+  int nbx, nby;              // tiles of TX x (TZ * R) cells along x, z
+  int lean;                  // interior tiles take cd2d_lean_cells
 };
 
+// Block -> tile map giving each of the 8 XCDs a contiguous run of tiles, so
+// the halo lines neighbouring tiles share are found in one L2.
+__device__ __forceinline__ int xcd_tile2(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= per * 8) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Interior tile: every cell of rows J0..J1 and columns I0..I0+TX-1 takes the
+// plain Laplacian branch (inside the grid, outside the C-PML strips, no
+// memory variable owned) and no point of the launch's families lies on
+// those rows.
 template <typename T>
-__global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
-  const int s = blockIdx.z;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts || nts <= a.act_min) return;
-  const int i = blockIdx.x * TX + threadIdx.x;
-  const int j = blockIdx.y * TZ + threadIdx.y;
+__device__ __forceinline__ bool cd2d_is_lean(const Cd2dStepArgs<T> &a, int s, int I0, int J0, int J1) {
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  if (!(I0 >= max(h, 1) && I0 + TX - 1 <= nx - h - 2 && J0 >= max(h, 1) && J1 <= nz - h - 2)) return false;
+  const int64_t o = (int64_t)s * (a.pt.nrows + 1);
+  const int *rowp = a.res == nullptr ? a.pt.srow : a.pt.rrow;
+  if (rowp[o + J0] != rowp[o + J1 + 1]) return false;
+  if (a.traces && a.pt.rrow[o + J0] != a.pt.rrow[o + J1 + 1]) return false;
+  return true;
+}
+
+// R cells (i, J0 + r * TZ) of an interior tile: every load of the R cells is
+// issued before the first store; the expressions are those of cd2d_cell's
+// interior branch, so results are bitwise equal.
+template <typename T, int R>
+__device__ __forceinline__ void cd2d_lean_cells(const Cd2dStepArgs<T> &a, int s, int nts, int i, int J0) {
+  const int nx = a.nx;
+  const int64_t so = (int64_t)s * a.N;
+  const T *__restrict__ pc_ = a.pcur + so;
+  const T *__restrict__ po_ = a.pold + so;
+  T *__restrict__ pn_ = a.pnew + so;
+  const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
+  int slot = -1;
+  if (a.dmode == 1) {
+    const int seg0 = ((nts - 1) / a.S) * a.S;
+    if (a.it > seg0) slot = a.it - seg0 - 1;
+  } else if (a.dmode == 2) {
+    slot = a.it - a.cbase - 1;
+  }
+  const T *dsrc = corr ? a.dbuf + (int64_t)(a.it - a.cbase) * a.dslot + so : nullptr;
+  T pc[R], pl[R], pr[R], pd[R], pu[R], pov[R], fa[R], gv[R], dv[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + r * TZ) * nx + i;
+    pc[r] = pc_[c];
+    pl[r] = pc_[c - 1];
+    pr[r] = pc_[c + 1];
+    pd[r] = pc_[c - nx];
+    pu[r] = pc_[c + nx];
+    pov[r] = po_[c];
+    fa[r] = a.fact[c];
+    gv[r] = corr ? a.grad[so + c] : T(0);
+    dv[r] = corr ? dsrc[c] : T(0);
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + r * TZ) * nx + i;
+    const T d2x = (pl[r] - T(2) * pc[r] + pr[r]) * a.idx2;
+    const T d2z = (pd[r] - T(2) * pc[r] + pu[r]) * a.idz2;
+    const T lx = d2x, lz = d2z;
+    const T flap = fa[r] * (lx + lz);
+    const T pnv = T(2) * pc[r] - pov[r] + flap;
+    const T inj = 0;
+    pn_[c] = pnv;
+    if (slot >= 0) a.dbuf[(int64_t)slot * a.dslot + so + c] = flap + inj;
+    if (corr) a.grad[so + c] = gv[r] + pc[r] * dv[r];
+  }
+}
+
+// One cell of the step: C-PML memory variables, Laplacian, points, replay
+// history and correlation.
+template <typename T>
+__device__ __forceinline__ void cd2d_cell(const Cd2dStepArgs<T> &a, int s, int nts, int i, int j) {
   const int nx = a.nx, nz = a.nz, h = a.h;
   if (i >= nx || j >= nz) return;
   const int64_t so = (int64_t)s * a.N;
@@ -196,6 +267,24 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   }
 }
 
+// One time step on a tile of TX x (TZ * R) cells per block (a 1D grid of
+// nbx * nby * shots blocks, dealt to the XCDs in contiguous runs).
+template <typename T, int R>
+__global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
+  const int b = xcd_tile2(blockIdx.x, gridDim.x);
+  const int t = b / a.nbx;
+  const int bx = b - t * a.nbx, by = t % a.nby, s = t / a.nby;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int i = bx * TX + threadIdx.x;
+  const int J0 = by * TZ * R;
+  if (a.lean && cd2d_is_lean(a, s, bx * TX, J0, J0 + TZ * R - 1)) {
+    cd2d_lean_cells<T, R>(a, s, nts, i, J0 + threadIdx.y);
+    return;
+  }
+  for (int r = 0; r < R; r++) cd2d_cell(a, s, nts, i, J0 + threadIdx.y + r * TZ);
+}
+
 // grad += adjcur * D for the first step of a segment (the other steps are
 // fused into the adjoint stencil).
 template <typename T>
@@ -222,6 +311,14 @@ class Cd2dEngine final : public Engine {
     sxx_ = (int64_t)(2 * h_ + 2) * nz_;
     sxz_ = (int64_t)nx_ * (2 * h_ + 2);
     cstride_ = 2 * (8 * h_ + 4);
+    // rows per thread (1, 2 or 4) and the interior-tile path; env overrides
+    // for measurement
+    if (const char *v = std::getenv("SWX_CD2D_ROWS")) {
+      const int r = std::atoi(v);
+      SWX_REQUIRE(r == 1 || r == 2 || r == 4, SWX_EINVAL, "SWX_CD2D_ROWS must be 1, 2 or 4");
+      rows_ = r;
+    }
+    if (const char *v = std::getenv("SWX_CD2D_LEAN")) lean_ = std::atoi(v) != 0;
     const size_t e = sizeof(T);
     fact_.alloc(N_ * e);
     p_.alloc(3 * ns_ * N_ * e);
@@ -377,11 +474,20 @@ class Cd2dEngine final : public Engine {
     return a;
   }
 
-  void launch(const Cd2dStepArgs<T> &a) {
-    dim3 blk(TX, TZ);
-    dim3 grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+  template <int R>
+  void launch_rows(Cd2dStepArgs<T> a) {
+    a.nbx = (int)ceil_div(nx_, TX);
+    a.nby = (int)ceil_div(nz_, TZ * R);
+    const unsigned nb = (unsigned)((int64_t)a.nbx * a.nby * nsh_);
+    cd2d_step_kernel<T, R><<<nb, dim3(TX, TZ), 0, c_.stream>>>(a);
+  }
+
+  void launch(Cd2dStepArgs<T> a) {
+    a.lean = lean_ ? 1 : 0;
     c_.ktimer.begin(c_.stream);
-    cd2d_step_kernel<T><<<grd, blk, 0, c_.stream>>>(a);
+    if (rows_ == 4) launch_rows<4>(a);
+    else if (rows_ == 1) launch_rows<1>(a);
+    else launch_rows<2>(a);
     c_.ktimer.end(c_.stream, a.corr ? 1 : 0);
     SWX_HIP(hipGetLastError());
   }
@@ -458,6 +564,8 @@ class Cd2dEngine final : public Engine {
   int nx_, nz_, h_, ns_;
   int64_t N_, spx_, spz_, sxx_, sxz_;
   int cstride_;
+  int rows_ = 2;
+  bool lean_ = true;
   DevBuf fact_, p_, psix_, psiz_, xix_, xiz_, cpml_;
   DevBuf a_, apsix_, apsiz_, axix_, axiz_, grad_, gout_;
   DevBuf dbuf_, ck_;

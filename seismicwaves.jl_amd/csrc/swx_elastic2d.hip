@@ -55,6 +55,7 @@ template <typename T>
 struct ElArgs {
   int nx, nz, h, freetop;
   int nbx, nbz, ch;             // z-marching launch: strips, chunks, rows per chunk
+  int tbx, tbz, xcd;            // block launches: tiles along x, z; XCD-aware tile map
   int64_t N, nrows;
   T idx, idz, idt2;
   const T *lam, *mu, *muhh, *rdx, *rdz;  // rdx = dt^2/rho_ihalf, rdz = dt^2/rho_jhalf
@@ -76,6 +77,27 @@ struct ElArgs {
   // correlation (stress launch on the adjoint field)
   T *grad; int64_t gstride; int update; int corr_step; int corr_lim;
 };
+
+// Block -> (strip, chunk, shot) map giving each of the 8 XCDs a contiguous
+// run, so neighbouring strips share halo lines in one L2.
+__device__ __forceinline__ int xcd_tile_el(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= per * 8) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Tile (bx, by) and shot s of a block launch (a 1D grid of tbx * tbz * shots
+// blocks).  With xcd set, each XCD steps a contiguous run of tiles, so the
+// x-halo lines a tile shares with its neighbours are found in one L2.
+struct ElBlock {
+  int bx, by, s;
+};
+template <typename T>
+__device__ __forceinline__ ElBlock el_block(const ElArgs<T> &a) {
+  const int b = a.xcd ? xcd_tile_el(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int t = b / a.tbx;
+  return ElBlock{b - t * a.tbx, t % a.tbz, t / a.tbz};
+}
 
 // 4th-order staggered difference, freesurface_derivatives_4th_mirror.jl:2-7
 template <typename T>
@@ -193,10 +215,10 @@ __device__ __forceinline__ bool el_is_lean(const ElArgs<T> &a, int s, int bx, in
 // block: the expressions of el_stress_kernel, in the same operand order,
 // with every global load issued first.
 template <typename T>
-__device__ __forceinline__ void el_stress_lean_block(const ElArgs<T> &a, int s, bool upd, bool corr) {
-  const int nx = a.nx;
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+__device__ __forceinline__ void el_stress_lean_block(const ElArgs<T> &a, ElBlock B, bool upd, bool corr) {
+  const int nx = a.nx, s = B.s;
+  const int i = B.bx * TX + threadIdx.x + 1;
+  const int j = B.by * TZ + threadIdx.y + 1;
   const int64_t N = a.N;
   const int c = (i - 1) + (j - 1) * nx;
   const T *__restrict__ ux = a.ucur + (int64_t)s * 2 * N;
@@ -257,10 +279,10 @@ __device__ __forceinline__ void el_stress_lean_block(const ElArgs<T> &a, int s, 
 // el_disp_kernel, in the same operand order, with every global load issued
 // first, and the history store.
 template <typename T>
-__device__ __forceinline__ void el_disp_lean_block(const ElArgs<T> &a, int s, int nts) {
-  const int nx = a.nx;
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+__device__ __forceinline__ void el_disp_lean_block(const ElArgs<T> &a, ElBlock B, int nts) {
+  const int nx = a.nx, s = B.s;
+  const int i = B.bx * TX + threadIdx.x + 1;
+  const int j = B.by * TZ + threadIdx.y + 1;
   const int64_t N = a.N;
   const int c = (i - 1) + (j - 1) * nx;
   const T *__restrict__ uc = a.ucur + (int64_t)s * 2 * N;
@@ -304,21 +326,23 @@ __device__ __forceinline__ void el_disp_lean_block(const ElArgs<T> &a, int s, in
 // after on the same grid, step them.
 template <typename T>
 __global__ __launch_bounds__(TX *TZ) void el_stress_lean(ElArgs<T> a) {
-  const int s = blockIdx.z;
+  const ElBlock B = el_block(a);
+  const int s = B.s;
   const int nts = a.nt_shot[s];
   const bool upd = a.update && a.it <= nts && nts > a.act_min;
   const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
   if (!upd && !corr) return;
-  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) return;
-  el_stress_lean_block(a, s, upd, corr);
+  if (!el_is_lean(a, s, B.bx, B.by, true)) return;
+  el_stress_lean_block(a, B, upd, corr);
 }
 template <typename T>
 __global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
-  const int s = blockIdx.z;
+  const ElBlock B = el_block(a);
+  const int s = B.s;
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
-  if (!el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) return;
-  el_disp_lean_block(a, s, nts);
+  if (!el_is_lean(a, s, B.bx, B.by, false)) return;
+  el_disp_lean_block(a, B, nts);
 }
 
 // Stress update of one step (update_sxx_szz! :39-60, update_sxz! :62-79),
@@ -326,17 +350,18 @@ __global__ __launch_bounds__(TX *TZ) void el_disp_lean(ElArgs<T> a) {
 // correlation of adjoint step corr_step (correlate_gradient_xPU.jl:1-82).
 template <typename T, bool FOLD>
 __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
-  const int s = blockIdx.z;
+  const ElBlock B = el_block(a);
+  const int s = B.s;
   const int nts = a.nt_shot[s];
   const bool upd = a.update && a.it <= nts && nts > a.act_min;
   const bool corr = a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
   if (!upd && !corr) return;
-  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, true)) {   // interior block
-    if (FOLD) el_stress_lean_block(a, s, upd, corr);
+  if (el_is_lean(a, s, B.bx, B.by, true)) {   // interior block
+    if (FOLD) el_stress_lean_block(a, B, upd, corr);
     return;
   }
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int i = B.bx * TX + threadIdx.x + 1;
+  const int j = B.by * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
   if (i > nx || j > nz) return;
   const bool ft = a.freetop != 0;
@@ -468,15 +493,16 @@ __global__ __launch_bounds__(TX *TZ) void el_stress_kernel(ElArgs<T> a) {
 // uold, force / adjoint-residual injection (:96-106) and the forward history.
 template <typename T, bool FOLD>
 __global__ __launch_bounds__(TX *TZ) void el_disp_kernel(ElArgs<T> a) {
-  const int s = blockIdx.z;
+  const ElBlock B = el_block(a);
+  const int s = B.s;
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
-  if (el_is_lean(a, s, blockIdx.x, blockIdx.y, false)) {   // interior block
-    if (FOLD) el_disp_lean_block(a, s, nts);
+  if (el_is_lean(a, s, B.bx, B.by, false)) {   // interior block
+    if (FOLD) el_disp_lean_block(a, B, nts);
     return;
   }
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TZ + threadIdx.y + 1;
+  const int i = B.bx * TX + threadIdx.x + 1;
+  const int j = B.by * TZ + threadIdx.y + 1;
   const int nx = a.nx, nz = a.nz, h = a.h;
   if (i > nx || j > nz) return;
   const bool ft = a.freetop != 0;
@@ -585,13 +611,6 @@ __global__ void el_hist_copy(T *dst, const T *src, int64_t n2, const int *nt_sho
     dst[o + q] = src[o + q];
 }
 
-// Block -> (strip, chunk, shot) map giving each of the 8 XCDs a contiguous
-// run, so neighbouring strips share halo lines in one L2.
-__device__ __forceinline__ int xcd_tile_el(int b, int nb) {
-  const int per = nb >> 3;
-  if (b >= per * 8) return b;
-  return (b & 7) * per + (b >> 3);
-}
 
 // Row ring in LDS seen through the reference's 1-based (i, j): row j lives
 // in slot j mod NR, column i at index i - i0.
@@ -1422,8 +1441,11 @@ class El2dEngine final : public Engine {
 
   // fold: interior and other blocks in one launch; otherwise an interior
   // launch followed by an other-block launch
-  void launch_stress(const ElArgs<T> &a, int kind, bool fold) {
-    dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+  void launch_stress(ElArgs<T> a, int kind, bool fold) {
+    a.tbx = (int)ceil_div(nx_, TX);
+    a.tbz = (int)ceil_div(nz_, TZ);
+    a.xcd = xcd_ ? 1 : 0;
+    dim3 blk(TX, TZ), grd((unsigned)((int64_t)a.tbx * a.tbz * nsh_));
     c_.ktimer.begin(c_.stream);
     if (fold) {
       el_stress_kernel<T, true><<<grd, blk, 0, c_.stream>>>(a);
@@ -1434,8 +1456,11 @@ class El2dEngine final : public Engine {
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
   }
-  void launch_disp(const ElArgs<T> &a, int kind, bool fold) {
-    dim3 blk(TX, TZ), grd((unsigned)ceil_div(nx_, TX), (unsigned)ceil_div(nz_, TZ), (unsigned)nsh_);
+  void launch_disp(ElArgs<T> a, int kind, bool fold) {
+    a.tbx = (int)ceil_div(nx_, TX);
+    a.tbz = (int)ceil_div(nz_, TZ);
+    a.xcd = xcd_ ? 1 : 0;
+    dim3 blk(TX, TZ), grd((unsigned)((int64_t)a.tbx * a.tbz * nsh_));
     c_.ktimer.begin(c_.stream);
     if (fold) {
       el_disp_kernel<T, true><<<grd, blk, 0, c_.stream>>>(a);
@@ -1636,6 +1661,11 @@ class El2dEngine final : public Engine {
   bool edge_taps_ = false;
   int max_ucell0_ = 0;
   int var_fwd_ = -1, var_adj_ = -1;
+  // XCD-aware tile map of the block launches (env SWX_EL2D_XCD=0 turns it off)
+  bool xcd_ = [] {
+    const char *v = std::getenv("SWX_EL2D_XCD");
+    return v == nullptr || std::atoi(v) != 0;
+  }();
   int64_t ck_bytes_one_ = 0, state_bytes_ = 0;
   int S_ = 1, K_ = 1, nck_ = 0;
   bool store_all_ = true;

@@ -127,6 +127,50 @@ def test_gradient_checkpoint_intervals_agree_bitwise(oracle_mod):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+def _deep_points_problem(dtype):
+    """Sources and receivers deep inside the grid, so that tiles clear of the
+    C-PML strips step through the test-free interior path except on the rows
+    that hold points."""
+    p = _grad_problem(dtype, nts=(300, 260))
+    h = p["h"]
+    tf = syn.ricker_tf(300, p["dt"], 8.0, 1.2 / 8.0)[:, None]
+    recs = [np.array([[x * h[0] + 3.0, 70 * h[1]] for x in range(20, 131, 7)]),
+            np.array([[95 * h[0], z * h[1]] for z in range(20, 91, 5)])]
+    p["shots"] = make_shots([[900.0, 500.0], [412.0, 630.0]], tf, recs, 8.0, p["nts"], dtype)
+    return p
+
+
+def test_interior_tiles_and_rows_agree_bitwise(oracle_mod, monkeypatch):
+    """1, 2 or 4 rows per thread, with or without the interior-tile path, give
+    the same seismograms and gradient bit for bit (FP32), and the FP64
+    default matches the oracle."""
+    p = _deep_points_problem(np.float32)
+    obs = _observed(oracle_mod, p)
+    mis = [sw.L2Misfit(o.astype(np.float32)) for o in obs]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"].astype(np.float32))
+    outs = []
+    for rows, lean in (("2", "1"), ("2", "0"), ("1", "1"), ("1", "0"), ("4", "1"), ("4", "0")):
+        print(f"[cd2d rows per thread {rows}, interior path {lean}]", flush=True)
+        monkeypatch.setenv("SWX_CD2D_ROWS", rows)
+        monkeypatch.setenv("SWX_CD2D_LEAN", lean)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=23))["vp"]
+        sw.swforward(p["params"], mp, p["shots"], runparams=sw.RunParameters())
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
+    assert np.max(np.abs(outs[0][0])) > 0
+    for g, seis in outs[1:]:
+        assert np.array_equal(g, outs[0][0])
+        assert all(np.array_equal(a, b) for a, b in zip(seis, outs[0][1]))
+    monkeypatch.delenv("SWX_CD2D_ROWS")
+    monkeypatch.delenv("SWX_CD2D_LEAN")
+    p = _deep_points_problem(np.float64)
+    gref, mref, _ = _oracle_gradient(oracle_mod, p, obs, 23)
+    g = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"]), p["shots"],
+                      [sw.L2Misfit(o) for o in obs], runparams=sw.RunParameters(),
+                      gradparams=sw.GradParameters(check_freq=23))["vp"]
+    assert rel_l2(g, gref) <= 1e-10
+
+
 def test_batched_equals_sequential():
     """One batched launch over three shots = three single-shot runs."""
     p = _grad_problem(np.float32)

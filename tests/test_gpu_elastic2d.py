@@ -160,7 +160,8 @@ def test_point_edge_cases(oracle_mod):
 def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     """Every forward and adjoint kernel variant the autotuner may pick (the
     stress/displacement step as interior + other-block launches or as one
-    launch each, z-marching strips of 128/256 columns) gives the same
+    launch each, with or without the XCD-aware tile map, z-marching strips of
+    128/256 columns) gives the same
     seismograms and gradients bit for bit, on a grid
     that needs several strips and a partial last one, and matches the
     oracle."""
@@ -169,10 +170,12 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     mis = [sw.L2Misfit(o) for o in obs]
     mp = _matprop(p["init"], np.float64)
     outs = []
-    for v, w in (("0", "0"), ("1", "1"), ("102", "104"), ("104", "102"), ("0", "104"), ("1", "0")):
-        print(f"[el2d kernel variants {kind} forward {v} adjoint {w}]", flush=True)
+    for v, w, xcd in (("0", "0", "1"), ("1", "1", "1"), ("1", "1", "0"), ("0", "0", "0"), ("102", "104", "1"),
+                      ("104", "102", "1"), ("0", "104", "1"), ("1", "0", "1")):
+        print(f"[el2d kernel variants {kind} forward {v} adjoint {w} xcd map {xcd}]", flush=True)
         monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", v)
         monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", w)
+        monkeypatch.setenv("SWX_EL2D_XCD", xcd)
         g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
                           gradparams=sw.GradParameters(check_freq=23))
         outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
