@@ -364,7 +364,7 @@ def kernel_names(physics, variant):
         return name(False, variant[0]), name(True, variant[1])
     if physics == "cd3d":
         return "cd3d_step_kernel<float, 4, 0> (forward)", "cd3d_step_kernel<float, 4, 0> (adjoint)"
-    return "cd2d_step_kernel<float> (forward)", "cd2d_step_kernel<float> (adjoint)"
+    return "cd2d_step_kernel<float, 2> (forward)", "cd2d_step_kernel<float, 2> (adjoint)"
 
 
 def traffic_from_profiles(workload, kernel):
