@@ -56,6 +56,8 @@ struct ElArgs {
   int nx, nz, h, freetop;
   int nbx, nbz, ch;             // z-marching launch: strips, chunks, rows per chunk
   int tbx, tbz, xcd;            // block launches: tiles along x, z; XCD-aware tile map
+  int order;                    // 1: the tiles outside [ibx0, ibx1] x [iby0, iby1] first
+  int ibx0, ibx1, iby0, iby1;   // tiles clear of the C-PML strips and grid edges
   int64_t N, nrows;
   T idx, idz, idt2;
   const T *lam, *mu, *muhh, *rdx, *rdz;  // rdx = dt^2/rho_ihalf, rdz = dt^2/rho_jhalf
@@ -92,11 +94,43 @@ __device__ __forceinline__ int xcd_tile_el(int b, int nb) {
 struct ElBlock {
   int bx, by, s;
 };
+//
+// With order set, the first blocks of the launch take the frame of tiles
+// around the interior rectangle [ibx0, ibx1] x [iby0, iby1] (C-PML strips
+// and grid edges: slower, latency-bound blocks) and the rest the interior
+// tiles, each range dealt to the XCDs in contiguous runs.  The slow blocks
+// then start first and overlap the interior ones instead of forming the
+// launch's tail.
 template <typename T>
 __device__ __forceinline__ ElBlock el_block(const ElArgs<T> &a) {
-  const int b = a.xcd ? xcd_tile_el(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const int t = b / a.tbx;
-  return ElBlock{b - t * a.tbx, t % a.tbz, t / a.tbz};
+  if (!a.order) {
+    const int b = a.xcd ? xcd_tile_el(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    const int t = b / a.tbx;
+    return ElBlock{b - t * a.tbx, t % a.tbz, t / a.tbz};
+  }
+  const int nIx = a.ibx1 - a.ibx0 + 1, nIy = a.iby1 - a.iby0 + 1;   // 0 x 0 when empty
+  const int nI = nIx * nIy, F = a.tbx * a.tbz - nI;
+  const int ns = gridDim.x / (a.tbx * a.tbz);
+  const int G = F * ns;
+  int b = blockIdx.x;
+  if (b < G) {
+    b = xcd_tile_el(b, G);
+    const int s = b / F;
+    int f = b - s * F;
+    const int top = a.iby0 * a.tbx;             // rows above the interior
+    if (f < top) return ElBlock{f % a.tbx, f / a.tbx, s};
+    f -= top;
+    const int side = a.tbx - nIx;                // frame tiles of an interior row
+    if (f < nIy * side) {
+      const int r = f / side, c = f - r * side;
+      return ElBlock{c < a.ibx0 ? c : c + nIx, a.iby0 + r, s};
+    }
+    f -= nIy * side;                             // rows below the interior
+    return ElBlock{f % a.tbx, a.iby0 + nIy + f / a.tbx, s};
+  }
+  b = xcd_tile_el(b - G, nI * ns);
+  const int s = b / nI, r = b - s * nI;
+  return ElBlock{a.ibx0 + r % nIx, a.iby0 + r / nIx, s};
 }
 
 // 4th-order staggered difference, freesurface_derivatives_4th_mirror.jl:2-7
@@ -1439,12 +1473,32 @@ class El2dEngine final : public Engine {
     return a;
   }
 
+  // Interior tile rectangle for the frame-first block order: the tiles whose
+  // cells satisfy the geometric part of el_is_lean (lo <= i <= hi along x
+  // and z); the point rows are left to el_is_lean.
+  void set_order(ElArgs<T> &a) const {
+    a.order = order_ ? 1 : 0;
+    const int lo = std::max(3, h_ + 2);
+    const int hx = std::min(nx_ - 2, nx_ - h_ - 1), hz = std::min(nz_ - 2, nz_ - h_ - 1);
+    a.ibx0 = (int)ceil_div(lo - 1, TX);
+    a.ibx1 = hx >= TX ? (hx - TX) / TX : -1;
+    a.iby0 = (int)ceil_div(lo - 1, TZ);
+    a.iby1 = hz >= TZ ? (hz - TZ) / TZ : -1;
+    a.ibx1 = std::min(a.ibx1, a.tbx - 1);
+    a.iby1 = std::min(a.iby1, a.tbz - 1);
+    if (a.ibx1 < a.ibx0 || a.iby1 < a.iby0) {   // no interior tile: all frame
+      a.ibx0 = 0; a.ibx1 = -1;
+      a.iby0 = a.tbz; a.iby1 = a.tbz - 1;
+    }
+  }
+
   // fold: interior and other blocks in one launch; otherwise an interior
   // launch followed by an other-block launch
   void launch_stress(ElArgs<T> a, int kind, bool fold) {
     a.tbx = (int)ceil_div(nx_, TX);
     a.tbz = (int)ceil_div(nz_, TZ);
     a.xcd = xcd_ ? 1 : 0;
+    set_order(a);
     dim3 blk(TX, TZ), grd((unsigned)((int64_t)a.tbx * a.tbz * nsh_));
     c_.ktimer.begin(c_.stream);
     if (fold) {
@@ -1460,6 +1514,7 @@ class El2dEngine final : public Engine {
     a.tbx = (int)ceil_div(nx_, TX);
     a.tbz = (int)ceil_div(nz_, TZ);
     a.xcd = xcd_ ? 1 : 0;
+    set_order(a);
     dim3 blk(TX, TZ), grd((unsigned)((int64_t)a.tbx * a.tbz * nsh_));
     c_.ktimer.begin(c_.stream);
     if (fold) {
@@ -1664,6 +1719,11 @@ class El2dEngine final : public Engine {
   // XCD-aware tile map of the block launches (env SWX_EL2D_XCD=0 turns it off)
   bool xcd_ = [] {
     const char *v = std::getenv("SWX_EL2D_XCD");
+    return v == nullptr || std::atoi(v) != 0;
+  }();
+  // frame-first block order (env SWX_EL2D_ORDER=0 turns it off)
+  bool order_ = [] {
+    const char *v = std::getenv("SWX_EL2D_ORDER");
     return v == nullptr || std::atoi(v) != 0;
   }();
   int64_t ck_bytes_one_ = 0, state_bytes_ = 0;

@@ -160,9 +160,9 @@ def test_point_edge_cases(oracle_mod):
 def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     """Every forward and adjoint kernel variant the autotuner may pick (the
     stress/displacement step as interior + other-block launches or as one
-    launch each, with or without the XCD-aware tile map, z-marching strips of
-    128/256 columns) gives the same
-    seismograms and gradients bit for bit, on a grid
+    launch each; with the XCD-aware tile map, with the C-PML/edge tiles
+    first or with the round-robin block order; z-marching strips of 128/256
+    columns) gives the same seismograms and gradients bit for bit, on a grid
     that needs several strips and a partial last one, and matches the
     oracle."""
     p = _problem(np.float64, kind=kind, nts=(200, 170), n=(301, 96), freetop=freetop)
@@ -170,12 +170,14 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     mis = [sw.L2Misfit(o) for o in obs]
     mp = _matprop(p["init"], np.float64)
     outs = []
-    for v, w, xcd in (("0", "0", "1"), ("1", "1", "1"), ("1", "1", "0"), ("0", "0", "0"), ("102", "104", "1"),
-                      ("104", "102", "1"), ("0", "104", "1"), ("1", "0", "1")):
-        print(f"[el2d kernel variants {kind} forward {v} adjoint {w} xcd map {xcd}]", flush=True)
+    for v, w, xcd, order in (("0", "0", "1", "1"), ("1", "1", "1", "1"), ("1", "1", "1", "0"), ("0", "0", "1", "0"),
+                             ("1", "1", "0", "0"), ("0", "0", "0", "0"), ("102", "104", "1", "1"),
+                             ("104", "102", "1", "1"), ("0", "104", "1", "1"), ("1", "0", "1", "1")):
+        print(f"[el2d kernel variants {kind} forward {v} adjoint {w} xcd map {xcd} frame first {order}]", flush=True)
         monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", v)
         monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", w)
         monkeypatch.setenv("SWX_EL2D_XCD", xcd)
+        monkeypatch.setenv("SWX_EL2D_ORDER", order)
         g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
                           gradparams=sw.GradParameters(check_freq=23))
         outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
