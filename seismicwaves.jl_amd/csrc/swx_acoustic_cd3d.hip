@@ -96,7 +96,9 @@ __device__ __forceinline__ bool cd3d_is_lean(const Cd3dArgs<T> &a, int s, int bx
   return !__syncthreads_or(hit);
 }
 
-template <typename T>
+// CORR: the launch correlates this shot's adjoint field (a compile-time
+// flag, so that every load of the march is unconditional).
+template <typename T, bool CORR>
 __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, int nts, int i, int j, int K0,
                                                  int K1) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], h = a.h;
@@ -114,7 +116,7 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
   } else if (a.dmode == 2) {
     dslot = a.it - a.cbase - 1;
   }
-  const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
+  const bool corr = CORR;
   const int cslot = a.it - a.cbase;
   const int cxy = j * nx + i;
   T pm = K0 >= 1 ? pc_[(int64_t)(K0 - 1) * sxy + cxy] : T(0);
@@ -124,22 +126,21 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
   auto load_plane = [&](int k) -> Plane {
     Plane q;
     const int64_t c = (int64_t)k * sxy + cxy;
-    q.pp = k + 1 < nz ? pc_[c + sxy] : T(0);
+    const T pnext = pc_[k + 1 < nz ? c + sxy : c];   // in bounds on the last plane too
+    q.pp = k + 1 < nz ? pnext : T(0);
     q.pl = pc_[c - 1];
     q.pr = pc_[c + 1];
     q.pd = pc_[c - nx];
     q.pu = pc_[c + nx];
     q.po = po_[c];
     q.fa = a.fact[c];
-    q.g = corr ? a.grad[so + c] : T(0);
-    q.d = corr ? a.dbuf[(int64_t)cslot * a.dslot + so + c] : T(0);
+    q.g = CORR ? a.grad[so + c] : T(0);
+    q.d = CORR ? a.dbuf[(int64_t)cslot * a.dslot + so + c] : T(0);
     return q;
   };
-  Plane nxt = load_plane(K0);
-  for (int k = K0; k < K1; k++) {
+  // One plane of the march; cur holds the operands loaded for plane k.
+  auto step = [&](int k, const Plane &cur) {
     const int64_t c = (int64_t)k * sxy + cxy;
-    const Plane cur = nxt;
-    if (k + 1 < K1) nxt = load_plane(k + 1);
     const T pp = cur.pp;
     T psz = 0;
     const int kz = psi_owner(k, nz, h);
@@ -184,6 +185,18 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
     psz_prev = psz;
     pm = pc;
     pc = pp;
+  };
+  // Two plane buffers used in turn (the loop is unrolled by two): the loads
+  // of plane k+1 stay in flight while plane k is stepped.  With one buffer
+  // copied forward each plane, the copy of registers whose loads are still
+  // pending makes the wave wait for them at the end of every plane.
+  Plane A = load_plane(K0);
+  for (int k = K0; k < K1; k += 2) {
+    const Plane B = load_plane(min(k + 1, K1 - 1));
+    step(k, A);
+    if (k + 1 >= K1) break;
+    A = load_plane(min(k + 2, K1 - 1));
+    step(k + 1, B);
   }
 }
 
@@ -206,7 +219,10 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   const int K0 = bk * a.kch, K1 = min(K0 + a.kch, nz);
   if (MODE == 1 || (a.lean && MODE == 0)) {
     if (cd3d_is_lean<T, TY>(a, s, bx, by, K0, K1)) {
-      if (MODE < 2) cd3d_lean_column(a, s, nts, i, j, K0, K1);
+      if (MODE < 2) {
+        if (a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim) cd3d_lean_column<T, true>(a, s, nts, i, j, K0, K1);
+        else cd3d_lean_column<T, false>(a, s, nts, i, j, K0, K1);
+      }
       return;
     }
     if (MODE == 1) return;
