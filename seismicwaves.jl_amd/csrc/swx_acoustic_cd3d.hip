@@ -96,10 +96,23 @@ __device__ __forceinline__ bool cd3d_is_lean(const Cd3dArgs<T> &a, int s, int bx
   return !__syncthreads_or(hit);
 }
 
-// CORR: the launch correlates this shot's adjoint field (a compile-time
-// flag, so that every load of the march is unconditional).
-template <typename T, bool CORR>
-__device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, int nts, int i, int j, int K0,
+// Replay-history slot this launch stores for shot s (or -1).
+template <typename T>
+__device__ __forceinline__ int cd3d_dslot(const Cd3dArgs<T> &a, int nts) {
+  if (a.dmode == 1) {
+    const int seg0 = ((nts - 1) / a.S) * a.S;
+    return a.it > seg0 ? a.it - seg0 - 1 : -1;
+  }
+  return a.dmode == 2 ? a.it - a.cbase - 1 : -1;
+}
+
+// Compile-time flags, so that the march's loads and stores are
+// unconditional where they can be: CORR, the launch correlates this shot's
+// adjoint field; DEEP, every plane of the chunk is inside the z-interior (no
+// z C-PML memory variable, no edge plane), so a plane is one straight-line
+// update; DST (with DEEP), the launch stores the replay history (slot dslot).
+template <typename T, bool CORR, bool DEEP, bool DST>
+__device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, int dslot, int i, int j, int K0,
                                                  int K1) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], h = a.h;
   const int64_t so = (int64_t)s * a.N;
@@ -109,13 +122,6 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
   T *__restrict__ pn_ = a.pnew + so;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cz = axis_coef(cb, h, 2);
-  int dslot = -1;
-  if (a.dmode == 1) {
-    const int seg0 = ((nts - 1) / a.S) * a.S;
-    if (a.it > seg0) dslot = a.it - seg0 - 1;
-  } else if (a.dmode == 2) {
-    dslot = a.it - a.cbase - 1;
-  }
   const bool corr = CORR;
   const int cslot = a.it - a.cbase;
   const int cxy = j * nx + i;
@@ -142,6 +148,24 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
   auto step = [&](int k, const Plane &cur) {
     const int64_t c = (int64_t)k * sxy + cxy;
     const T pp = cur.pp;
+    if (DEEP) {
+      // the interior branch below with no z C-PML term
+      const T d2x = (cur.pl - T(2) * pc + cur.pr) * a.id2[0];
+      const T d2y = (cur.pd - T(2) * pc + cur.pu) * a.id2[1];
+      const T d2z = (pm - T(2) * pc + pp) * a.id2[2];
+      const T lx = d2x, ly = d2y, lz = d2z;
+      const T flap = cur.fa * (lx + ly + lz);
+      const T pnv = T(2) * pc - cur.po + flap;
+      pn_[c] = pnv;
+      if (DST) {
+        const T inj = 0;
+        a.dbuf[(int64_t)dslot * a.dslot + so + c] = flap + inj;
+      }
+      if (CORR) a.grad[so + c] = cur.g + pc * cur.d;
+      pm = pc;
+      pc = pp;
+      return;
+    }
     T psz = 0;
     const int kz = psi_owner(k, nz, h);
     if (kz >= 0) {
@@ -186,17 +210,21 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
     pm = pc;
     pc = pp;
   };
-  // Two plane buffers used in turn (the loop is unrolled by two): the loads
-  // of plane k+1 stay in flight while plane k is stepped.  With one buffer
-  // copied forward each plane, the copy of registers whose loads are still
-  // pending makes the wave wait for them at the end of every plane.
-  Plane A = load_plane(K0);
-  for (int k = K0; k < K1; k += 2) {
-    const Plane B = load_plane(min(k + 1, K1 - 1));
+  // Three plane buffers used in turn (the loop is unrolled by three): the
+  // loads of planes k+1 and k+2 stay in flight while plane k is stepped.
+  // With one buffer copied forward each plane, the copy of registers whose
+  // loads are still pending makes the wave wait for them at the end of
+  // every plane.
+  Plane A = load_plane(K0), B = load_plane(min(K0 + 1, K1 - 1));
+  for (int k = K0; k < K1; k += 3) {
+    const Plane C = load_plane(min(k + 2, K1 - 1));
     step(k, A);
     if (k + 1 >= K1) break;
-    A = load_plane(min(k + 2, K1 - 1));
+    A = load_plane(min(k + 3, K1 - 1));
     step(k + 1, B);
+    if (k + 2 >= K1) break;
+    B = load_plane(min(k + 4, K1 - 1));
+    step(k + 2, C);
   }
 }
 
@@ -220,8 +248,21 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   if (MODE == 1 || (a.lean && MODE == 0)) {
     if (cd3d_is_lean<T, TY>(a, s, bx, by, K0, K1)) {
       if (MODE < 2) {
-        if (a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim) cd3d_lean_column<T, true>(a, s, nts, i, j, K0, K1);
-        else cd3d_lean_column<T, false>(a, s, nts, i, j, K0, K1);
+        const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
+        const int dslot = cd3d_dslot(a, nts);
+        const bool deep = K0 >= max(h, 1) && K1 - 1 <= nz - h - 2;
+        if (deep) {
+          if (corr) {
+            if (dslot >= 0) cd3d_lean_column<T, true, true, true>(a, s, dslot, i, j, K0, K1);
+            else cd3d_lean_column<T, true, true, false>(a, s, dslot, i, j, K0, K1);
+          } else {
+            if (dslot >= 0) cd3d_lean_column<T, false, true, true>(a, s, dslot, i, j, K0, K1);
+            else cd3d_lean_column<T, false, true, false>(a, s, dslot, i, j, K0, K1);
+          }
+        } else {
+          if (corr) cd3d_lean_column<T, true, false, false>(a, s, dslot, i, j, K0, K1);
+          else cd3d_lean_column<T, false, false, false>(a, s, dslot, i, j, K0, K1);
+        }
       }
       return;
     }
