@@ -497,11 +497,11 @@ class Cd3dEngine final : public Engine {
       nck_ = std::max(K_ - 2, 0);
       ck_.alloc(std::max<int64_t>(nck_ * ck_one_, 1));
     }
-    // 64 x 4 tiles in one launch, 16 planes per chunk: measured best of
-    // 64 x {8, 4, 2} tiles, one or two launches and 16 / 32 / 64 planes on
+    // 64 x 4 tiles in one launch, 24 planes per chunk: measured best of
+    // 64 x {8, 4, 2} tiles, one or two launches and 8 to 64 planes on
     // 600 x 600 x 400 (the smaller blocks let more of them share a CU than
     // the general path's register count would with 64 x 8)
-    kch_ = 16;
+    kch_ = 24;
     if (const char *v = std::getenv("SWX_CD3D_ZCHUNK")) kch_ = std::max(1, std::atoi(v));
     if (const char *v = std::getenv("SWX_CD3D_LEAN")) lean_ = std::atoi(v) != 0;
     // launch shape: 8 / 4 / 2 = one launch with 64 x 8 / 64 x 4 / 64 x 2
@@ -709,7 +709,7 @@ class Cd3dEngine final : public Engine {
   PlanCore &c_;
   int n_[3], h_, ns_;
   int64_t N_, sps_[3], sxi_[3];
-  int cstride_, kch_ = 16;
+  int cstride_, kch_ = 24;
   bool lean_ = true;
   int variant_ = 4;
   DevBuf fact_, p_, psi_[3], xi_[3], cpml_;
