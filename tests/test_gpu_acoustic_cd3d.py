@@ -131,10 +131,13 @@ def _interior_problem(dtype, nt=150):
     return dict(params=params, vtrue=vtrue, v0=v0, shots=shots, osh=osh, h=h, dt=dt, halo=halo, nt=nt)
 
 
-def test_interior_tiles_against_oracle(oracle_mod):
+@pytest.mark.parametrize("chunk", ["24", "7"])
+def test_interior_tiles_against_oracle(oracle_mod, monkeypatch, chunk):
     """FP64 seismograms and gradient on a grid whose interior tiles take the
-    test-free path."""
-    print("[cd3d interior tiles float64]", flush=True)
+    test-free path; with 7 planes per block some chunks lie wholly inside the
+    z-interior and take the straight-line plane update."""
+    print(f"[cd3d interior tiles float64, z chunk {chunk}]", flush=True)
+    monkeypatch.setenv("SWX_CD3D_ZCHUNK", chunk)
     p = _interior_problem(np.float64)
     obs = oracle_mod.acoustic_cd3d_forward(p["vtrue"], p["h"], p["dt"], p["nt"], p["halo"], 1e-4, True, p["osh"])
     sw.swforward(p["params"], sw.VpAcousticCDMaterialProperties(p["vtrue"]), p["shots"], runparams=sw.RunParameters())
