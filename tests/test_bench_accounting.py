@@ -1,0 +1,71 @@
+"""bench.py's algorithmic-byte model (the numerator of every roofline figure)
+against the per-cell counts of the step kernels: bytes per cell-step, e bytes
+per value, halo 0 (the C-PML memory variables then reduce to the 2h+2 = 2
+edge cells of each xi array per axis face, counted per launch as `strip`).
+
+VD: forward 9e (r p, vx, vz, m0, m1x, m1z; w p, vx, vz), 10e with the
+pressure history stored; adjoint 17e (9e + two history levels + three
+gradient fields read and written).  CD: forward 4e (r p^n, p^n-1, fact;
+w p^n+1), 5e with the replay history stored; adjoint 7e (+ r history, grad
+read and written), 4e for the first step of each segment, whose correlation
+runs in a kernel of its own.  Elastic: stress 8e + displacement 11e = 19e per
+step, + 2e history; adjoint 19e + 16e correlation."""
+import bench
+
+E = 4
+N = 1000 * 100
+NS = (1000, 100)
+STRIP = bench._strip_cells(NS, 0) * 2 * E   # read and written once per launch
+
+
+def test_vd_forward_and_adjoint_bytes():
+    # nt a multiple of S: no final segment stored during the forward, every
+    # segment replayed with S - 1 steps (each storing history)
+    nt, S = 500, 50
+    b = bench.alg_bytes("vd2d", NS, 0, E, [nt], S)
+    rep = (nt // S) * (S - 1)
+    assert b[0] == (nt * 9 * E * N + rep * 10 * E * N + (nt + rep) * STRIP, nt + rep)
+    assert b[1] == (nt * 17 * E * N + nt * STRIP, nt)
+    # a final partial segment is stored during the forward (10e per step)
+    nt = 520
+    b = bench.alg_bytes("vd2d", NS, 0, E, [nt], S)
+    last, rep = 20, 10 * (S - 1)
+    assert b[0] == ((nt - last) * 9 * E * N + last * 10 * E * N + rep * 10 * E * N + (nt + rep) * STRIP, nt + rep)
+
+
+def test_cd_forward_and_adjoint_bytes():
+    nt, S = 501, 50
+    b = bench.alg_bytes("cd2d", NS, 0, E, [nt], S)
+    kl = (nt - 1) // S
+    last = nt - kl * S
+    rep = nt - last
+    assert b[0] == ((nt - last) * 4 * E * N + last * 5 * E * N + rep * 5 * E * N + (nt + rep) * STRIP, nt + rep)
+    nseg = kl + 1
+    assert b[1] == ((nt - nseg) * 7 * E * N + nseg * 4 * E * N + nt * STRIP, nt)
+
+
+def test_elastic_bytes_per_step():
+    nt, S = 500, 50
+    b = bench.alg_bytes("el2d", NS, 0, E, [nt], S, variant=(1, 1))
+    rep = (nt // S) * (S - 1)
+    # two launches per step; replayed steps also store the history (2e); the
+    # elastic model counts the full-grid fields only
+    assert b[0] == (nt * 19 * E * N + rep * 21 * E * N, 2 * (nt + rep))
+    assert b[1] == (nt * 35 * E * N, 2 * nt)
+
+
+def test_shots_add_up():
+    one = bench.alg_bytes("vd2d", NS, 20, E, [300], 50)
+    two = bench.alg_bytes("vd2d", NS, 20, E, [300, 300], 50)
+    assert two[0] == (2 * one[0][0], 2 * one[0][1])
+    assert two[1] == (2 * one[1][0], 2 * one[1][1])
+
+
+def test_strips_counted_once_per_launch():
+    # the C-PML strips add the same bytes to every launch of a family
+    nt, S = 100, 50
+    b0 = bench.alg_bytes("cd2d", NS, 0, E, [nt], S)
+    b1 = bench.alg_bytes("cd2d", NS, 10, E, [nt], S)
+    extra = (bench._strip_cells(NS, 10) - bench._strip_cells(NS, 0)) * 2 * E
+    assert b1[0][0] - b0[0][0] == b0[0][1] * extra
+    assert b1[1][0] - b0[1][0] == b0[1][1] * extra
