@@ -1291,9 +1291,9 @@ class El2dEngine final : public Engine {
     if (mt) {
       for (int s = 0; s < nshots; s++) {
         std::vector<int64_t> cells;
-        const swx_taps &f = shots[s].src[0];
-        for (int k = 0; k < f.ptr[nsrc[s]]; k++) {
-          const int64_t i = f.ij[2 * (int64_t)k], j = f.ij[2 * (int64_t)k + 1];
+        const swx_taps &t0 = shots[s].src[0];
+        for (int k = 0; k < t0.ptr[nsrc[s]]; k++) {
+          const int64_t i = t0.ij[2 * (int64_t)k], j = t0.ij[2 * (int64_t)k + 1];
           cells.push_back((i - 1) + (j - 1) * (int64_t)nx_);
           const bool nrm = i >= 2 && i <= nx_ - 1 && j >= (c_.d.freetop ? 1 : 2) && j <= nz_ - 1;
           edge_taps_ |= !nrm;
