@@ -187,12 +187,14 @@ int swx_set_model(swx_plan *plan, int nfields, const void *const *fields);
 
 /* Replaces the per-shot uploads of swforward_1shot!/swgradient_1shot!
  * (acou_forward.jl:38-41) for a batch of shots.  Validates positions.
- * Acoustic plans only. */
+ * Acoustic plans only.  A rejected batch leaves the plan without one:
+ * swx_forward and swx_adjoint then return SWX_ESTATE until a batch is set. */
 int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots);
 
 /* Elastic counterpart of swx_set_shots: the per-shot uploads of the elastic
  * swforward_1shot!/swgradient_1shot! (ela_forward.jl:21-43,
- * ela_gradient.jl:24-46).  All shots of a batch share one source kind. */
+ * ela_gradient.jl:24-46).  All shots of a batch share one source kind.
+ * A rejected batch leaves the plan without one, as for swx_set_shots. */
 int swx_set_shots_elastic(swx_plan *plan, int nshots, const swx_elastic_shot_desc *shots);
 
 /* Forward time loop of every shot (acou_forward.jl:49-58 per shot).

@@ -127,6 +127,7 @@ int SFX(oracle_cd3d_forward)(const oracle_geom *g, const real *fact, const SFX(c
   for (int it = 1; it <= nt; it++) {
     SFX(cd3d_onestep)(g, fact, &s, c, nsrc, src, srctf, nt, it, nrec, rec, traces);
     SFX(cd3d_rotate)(&s);
+    oracle_progress("cd3d forward", it, nt);
   }
   SFX(cd3d_state_free)(&s);
   return 0;
@@ -178,6 +179,7 @@ int SFX(oracle_cd3d_gradient)(const oracle_geom *g, const real *fact, const SFX(
         memcpy(ckpsi[it / cf][3 + d], s.xi[d], nxi[d] * sizeof(real));
       }
     if (it >= last) memcpy(buf[it - last], s.p[1], N * sizeof(real));
+    oracle_progress("cd3d gradient, forward", it, nt);
   }
   real *res = (real *)malloc((size_t)nt * nrec * sizeof(real));
   for (int r = 0; r < nrec; r++) {
@@ -190,6 +192,7 @@ int SFX(oracle_cd3d_gradient)(const oracle_geom *g, const real *fact, const SFX(
   for (int it = nt; it >= 1; it--) {
     SFX(cd3d_onestep)(g, fact, &a, c, nrec, rec, res, nt, it, 0, NULL, NULL);
     SFX(cd3d_rotate)(&a);
+    oracle_progress("cd3d gradient, adjoint", nt - it + 1, nt);
     int t = it - 2;
     if (!((curr <= t && t <= curr + cf) || HAS_P(t))) {
       int old = curr;

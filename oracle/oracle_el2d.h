@@ -331,7 +331,10 @@ int SFX(oracle_el2d_forward)(const oracle_geom *og, const real *lam, const real 
   SFX(el_model) m = {cx, cz, lam, mu, rho_ih, rho_jh, mu_ihjh, (real)og->dt};
   SFX(el_state) s;
   SFX(el_state_alloc)(&g, &s);
-  for (int it = 1; it <= nt; it++) SFX(el_fwd_step)(&g, &m, &s, kind, s1, s2, M, tf, nt, it, rx, rz, traces);
+  for (int it = 1; it <= nt; it++) {
+    SFX(el_fwd_step)(&g, &m, &s, kind, s1, s2, M, tf, nt, it, rx, rz, traces);
+    oracle_progress("elastic forward", it, nt);
+  }
   SFX(el_state_free)(&s);
   return 0;
 }
@@ -428,6 +431,7 @@ int SFX(oracle_el2d_gradient)(const oracle_geom *og, const real *lam, const real
       memcpy(bx[it - last], s.ux[1], Nx * sizeof(real));
       memcpy(bz[it - last], s.uz[1], Nz * sizeof(real));
     }
+    oracle_progress("elastic gradient, forward", it, nt);
   }
   memset(g_rih, 0, Nx * sizeof(real));
   memset(g_rjh, 0, Nz * sizeof(real));
@@ -442,6 +446,7 @@ int SFX(oracle_el2d_gradient)(const oracle_geom *og, const real *lam, const real
     SFX(el_update_disp)(&g, &m, &a);
     SFX(el_inject_force)(&g, &m, &a, rx, rz, adjsrc, nt, it);
     SFX(el_rotate)(&a);
+    oracle_progress("elastic gradient, adjoint", nt - it + 1, nt);
     int t = it - 2;
     if (!((curr <= t && t <= curr + cf) || HAS_U(t))) {
       int old = curr;

@@ -163,6 +163,7 @@ int SFX(oracle_cd2d_forward)(const oracle_geom *g, const real *fact,
     SFX(cd2d_onestep)(g, fact, s.p[0], s.p[1], s.p[2], s.psi_x, s.psi_z, s.xi_x, s.xi_z,
                       cx, cz, nsrc, src, srctf, nt, it, nrec, rec, traces);
     SFX(cd2d_rotate)(&s);
+    oracle_progress("cd2d forward", it, nt);
   }
   SFX(cd2d_state_free)(&s);
   return 0;
@@ -282,6 +283,7 @@ int SFX(oracle_cd2d_gradient)(const oracle_geom *g, const real *fact,
                       cx, cz, nsrc, src, srctf, nt, it, nrec, rec, own_traces);
     SFX(cd2d_rotate)(&s);
     SFX(cd2d_ckpt_save)(&ck, &s, it);
+    oracle_progress("cd2d gradient, forward", it, nt);
   }
   /* prescale_residuals! (acoustic2D_xPU.jl:64-76) */
   real *res = (real *)malloc((size_t)nt * nrec * sizeof(real));
@@ -296,6 +298,7 @@ int SFX(oracle_cd2d_gradient)(const oracle_geom *g, const real *fact,
     SFX(cd2d_onestep)(g, fact, a.p[0], a.p[1], a.p[2], a.psi_x, a.psi_z, a.xi_x, a.xi_z,
                       cx, cz, nrec, rec, res, nt, it, 0, NULL, NULL);
     SFX(cd2d_rotate)(&a);
+    oracle_progress("cd2d gradient, adjoint", nt - it + 1, nt);
     if (!SFX(cd2d_issaved)(&ck, it - 2)) {
       /* initrecover! (checkpointers.jl:87-94) */
       int old = ck.curr;
@@ -504,8 +507,10 @@ int SFX(oracle_vd2d_forward)(const oracle_geom *g, const real *m0, const real *m
                              const real *srctf, int nrec, const int64_t *rec, real *traces) {
   SFX(vd2d_state) s;
   SFX(vd2d_state_alloc)(g, &s);
-  for (int it = 1; it <= nt; it++)
+  for (int it = 1; it <= nt; it++) {
     SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, it, nrec, rec, traces);
+    oracle_progress("vd2d forward", it, nt);
+  }
   SFX(vd2d_state_free)(&s);
   return 0;
 }
@@ -536,6 +541,7 @@ int SFX(oracle_vd2d_gradient)(const oracle_geom *g, const real *m0, const real *
     SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, it, nrec, rec, own_tr);
     if (it % cf == 0) SFX(vd2d_state_copy)(g, &ck[it / cf], &s);
     if (it >= last) memcpy(buf[it - last], s.p, N * sizeof(real));
+    oracle_progress("vd2d gradient, forward", it, nt);
   }
   real *res = (real *)malloc((size_t)nt * nrec * sizeof(real));
   for (int r = 0; r < nrec; r++) {
@@ -550,6 +556,7 @@ int SFX(oracle_vd2d_gradient)(const oracle_geom *g, const real *m0, const real *
 #define VD_SAVED(t) (((t) % cf == 0) ? ck[(t) / cf].p : buf[(t) - curr])
   for (int it = nt; it >= 1; it--) {
     SFX(vd2d_adj_step)(g, m0, m1x, m1z, &a, cx, cz, nrec, rec, res, nt, it);
+    oracle_progress("vd2d gradient, adjoint", nt - it + 1, nt);
     int t = it - 1;
     int saved = (curr <= t && t <= curr + cf) || (t % cf == 0);
     if (!saved) {

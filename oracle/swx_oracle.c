@@ -5,9 +5,23 @@
  * product library.  Build: oracle/Makefile (gcc -O3 -fopenmp).
  */
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+
+/* About ten progress lines per time loop on stderr when the environment
+ * sets ORACLE_PROGRESS (long oracle runs at full benchmark sizes). */
+static void oracle_progress(const char *phase, int done, int total) {
+  static int on = -1;
+  if (on < 0) on = getenv("ORACLE_PROGRESS") != NULL;
+  if (!on || total <= 0) return;
+  const int every = total / 10 > 0 ? total / 10 : 1;
+  if (done % every == 0 || done == total) {
+    fprintf(stderr, "[oracle] %s %d/%d\n", phase, done, total);
+    fflush(stderr);
+  }
+}
 
 typedef struct {
   int64_t n[3];   /* grid size, x (fast axis) first                         */

@@ -214,6 +214,16 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
 
 using namespace swx;
 
+namespace {
+// After a rejected swx_set_shots / swx_set_shots_elastic the plan holds no
+// batch: the engine's tables may be half rebuilt, so swx_forward and
+// swx_adjoint must refuse to run (SWX_ESTATE) until a batch is set again.
+void forget_shots(PlanCore &c) {
+  c.pts.nshots = 0;
+  c.forward_done = false;
+}
+}  // namespace
+
 extern "C" {
 
 int swx_abi_version(void) { return SWX_ABI_VERSION; }
@@ -297,9 +307,16 @@ int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots) {
                   "shot nt must be in 1..nt_max");
     SWX_REQUIRE(d.physics != SWX_ELASTIC_ISO, SWX_EINVAL, "elastic plans take swx_set_shots_elastic");
     SWX_HIP(hipSetDevice(d.device));
-    build_point_tables(plan->core.pts, d.ndim, d.n, plan->core.esize, nshots, shots, plan->core.stream);
     plan->core.forward_done = false;
-    plan->engine->on_shots_set(nshots, shots);
+    try {
+      PointTables tables;
+      build_point_tables(tables, d.ndim, d.n, plan->core.esize, nshots, shots, plan->core.stream);
+      plan->core.pts = std::move(tables);
+      plan->engine->on_shots_set(nshots, shots);
+    } catch (...) {
+      forget_shots(plan->core);
+      throw;
+    }
   });
 }
 
@@ -313,7 +330,12 @@ int swx_set_shots_elastic(swx_plan *plan, int nshots, const swx_elastic_shot_des
       SWX_REQUIRE(shots[s].nt >= 1 && shots[s].nt <= d.nt_max, SWX_EINVAL, "shot nt must be in 1..nt_max");
     SWX_HIP(hipSetDevice(d.device));
     plan->core.forward_done = false;
-    plan->engine->on_elastic_shots(nshots, shots);
+    try {
+      plan->engine->on_elastic_shots(nshots, shots);
+    } catch (...) {
+      forget_shots(plan->core);
+      throw;
+    }
   });
 }
 

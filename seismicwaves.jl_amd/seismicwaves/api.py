@@ -144,6 +144,7 @@ class _AcousticCPMLWaveSimulation:
         self.matprop = None
         self._plan = None
         self._plan_shots = 0
+        self._fit_batch = 0
 
     # -- checks (acou_models.jl:5-38) --
     def check_courant_condition(self, vp):
@@ -232,8 +233,31 @@ class _AcousticCPMLWaveSimulation:
         plan.set_shots(descs)
 
     def batches(self, nshots):
+        """Yields (shot indices, plan) for consecutive batches of the call's
+        shots.  A batch holds RunParameters.max_batch shots (all of them if
+        unset).  When a plan for that many shots does not fit the HBM budget
+        (the library reports SWX_ENOMEM at plan creation, state and
+        checkpoints included), the batch is halved until it fits, and the
+        size that fitted is kept for the next calls on this wave simulation.
+        Shots are independent, so the results do not depend on the split."""
         b = self.runparams.max_batch or nshots
-        return [list(range(k, min(k + b, nshots))) for k in range(0, nshots, b)]
+        if self._fit_batch:
+            b = min(b, self._fit_batch)
+        k = 0
+        while k < nshots:
+            size = min(b, nshots - k)
+            try:
+                plan = self._ensure_plan(size)
+            except MemoryError:
+                if size == 1:
+                    raise
+                b = (size + 1) // 2
+                self._fit_batch = b
+                self.runparams.logger.info("batch of %d shots does not fit the device memory budget; "
+                                           "trying %d", size, b)
+                continue
+            yield list(range(k, k + size)), plan
+            k += size
 
 
 class AcousticCDCPMLWaveSimulation(_AcousticCPMLWaveSimulation):
@@ -525,9 +549,8 @@ def run_swforward(wavesim, matprop, shots):
     """run_swforward! (forward.jl:161-199), batched."""
     check_sim_consistency(wavesim, matprop, shots)
     wavesim.set_matprop(matprop)
-    for batch in wavesim.batches(len(shots)):
+    for batch, plan in wavesim.batches(len(shots)):
         descs = [wavesim.shot_desc(shots[s]) for s in batch]
-        plan = wavesim._ensure_plan(len(batch))
         wavesim.upload_shots(plan, descs)
         traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=wavesim.dtype)
         plan.forward(traces, save_state=False)
@@ -562,22 +585,27 @@ def swmisfit(params_or_wavesim, matprop, shots, misfit, *, runparams: RunParamet
     return sum(m.calcmisfit(sh.recs) for sh, m in zip(shots, misfit))
 
 
-def run_swgradient(wavesim, matprop, shots, misfit):
+def run_swgradient(wavesim, matprop, shots, misfit, *, per_shot_gradients: bool = False):
     """run_swgradient! (gradient.jl:93-136), batched: forward with device
-    checkpoints, host adjoint source, adjoint with replay, correlation."""
+    checkpoints, host adjoint source, adjoint with replay, correlation.
+
+    With per_shot_gradients the result is the list of each shot's gradient
+    (muted and chain-ruled as the reference does per shot, gradient.jl:
+    117-128) instead of their sum; the library then returns one raw
+    gradient per shot (swx_adjoint with per_shot = 1)."""
     check_sim_consistency(wavesim, matprop, shots)
     assert len(misfit) == len(shots)
     wavesim.set_matprop(matprop)
     gp = wavesim.gradparams
-    per_shot = gp.mute_radius_src > 0 or gp.mute_radius_rec > 0
+    per_shot = per_shot_gradients or gp.mute_radius_src > 0 or gp.mute_radius_rec > 0
     totgrad = None
+    shot_grads = []
     totmisfit = 0.0
     T = wavesim.dtype
     shapes = wavesim.grad_fields_shapes()
-    for batch in wavesim.batches(len(shots)):
+    for batch, plan in wavesim.batches(len(shots)):
         bshots = [shots[s] for s in batch]
         descs = [wavesim.shot_desc(sh) for sh in bshots]
-        plan = wavesim._ensure_plan(len(batch))
         wavesim.upload_shots(plan, descs)
         traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=T)
         plan.forward(traces, save_state=True)
@@ -598,26 +626,31 @@ def run_swgradient(wavesim, matprop, shots, misfit):
                 fields.append(rawflat[off:off + n].reshape(s, order="F").copy())
                 off += n
             raw.append(fields)
-        g = wavesim.finish_gradient(raw if per_shot else raw[0], bshots, per_shot)
-        if totgrad is None:
-            totgrad = g
+        if per_shot_gradients:
+            shot_grads += [wavesim.finish_gradient([raw[k]], [bshots[k]], True) for k in range(len(batch))]
         else:
-            for key in totgrad:
-                totgrad[key] += g[key]
+            g = wavesim.finish_gradient(raw if per_shot else raw[0], bshots, per_shot)
+            if totgrad is None:
+                totgrad = g
+            else:
+                for key in totgrad:
+                    totgrad[key] += g[key]
         if gp.compute_misfit:
             totmisfit += sum(misfit[s].calcmisfit(shots[s].recs) for s in batch)
-    return (totgrad, totmisfit) if gp.compute_misfit else totgrad
+    out = shot_grads if per_shot_gradients else totgrad
+    return (out, totmisfit) if gp.compute_misfit else out
 
 
 def swgradient(params_or_wavesim, matprop, shots, misfit, *, runparams: RunParameters | None = None,
-               gradparams: GradParameters | None = None):
-    """swgradient! (gradient.jl:31-88)."""
+               gradparams: GradParameters | None = None, per_shot_gradients: bool = False):
+    """swgradient! (gradient.jl:31-88).  per_shot_gradients: return the
+    list of per-shot gradients instead of their sum (an extension)."""
     if isinstance(params_or_wavesim, (InputParametersAcoustic, InputParametersElastic)):
         ws = build_wavesim(params_or_wavesim, matprop, runparams=runparams or RunParameters(),
                            gradparams=gradparams or GradParameters(), gradient=True)
     else:
         ws = params_or_wavesim
-    return run_swgradient(ws, matprop, shots, misfit)
+    return run_swgradient(ws, matprop, shots, misfit, per_shot_gradients=per_shot_gradients)
 
 
 __all__ = [

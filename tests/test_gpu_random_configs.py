@@ -2,7 +2,9 @@
 physics (grid sizes around the width of a few interior tiles, C-PML halo,
 free surface on or off, several shots of uneven length, sources and
 receivers anywhere in the grid, replay interval), FP64 seismograms and
-gradients against the FP64 oracle at relative L2 <= 1e-10.
+gradients against the FP64 oracle at relative L2 <= 1e-10.  The shots of a
+batch carry different dominant frequencies, so each has its own C-PML
+coefficients (alpha = pi f0 (1 - x), acou_init_bc.jl:19-30).
 
 The sizes are chosen so that each grid has tiles that take the test-free
 interior paths as well as C-PML, edge and point tiles; the positions put
@@ -48,18 +50,23 @@ def _config(rng, nlo, nhi, halos):
     return n, halo, freetop, nts, cf
 
 
+def _domfreq(k):
+    """Dominant frequency of shot k of a batch (C-PML alpha per shot)."""
+    return F0 * (1.0 + 0.45 * k)
+
+
 def _scalar_shots(rng, n, nts, dtype):
     ntmax = max(nts)
     tf = syn.ricker_tf(ntmax, DT, F0, 1.2 / F0)
     shots, osh = [], []
-    for nt in nts:
+    for k, nt in enumerate(nts):
         nsrc = int(rng.integers(1, 3))
         src = _points(rng, nsrc, n, 1.0)
         stf = np.stack([tf[:nt] * (1.0 + 0.3 * k) for k in range(nsrc)], 1)
         rec = _points(rng, int(rng.integers(3, 12)), n, 0.0)
-        shots.append(sw.ScalarShot(sw.ScalarSources(src, stf.astype(dtype), F0),
+        shots.append(sw.ScalarShot(sw.ScalarSources(src, stf.astype(dtype), _domfreq(k)),
                                    sw.ScalarReceivers(rec, nt, dtype=dtype)))
-        osh.append(dict(src_pos=src, srctf=stf, f0=F0, rec_pos=rec, nt=nt))
+        osh.append(dict(src_pos=src, srctf=stf, f0=_domfreq(k), rec_pos=rec, nt=nt))
     return shots, osh
 
 
@@ -163,22 +170,23 @@ def test_elastic_random(oracle_mod, seed):
     ntmax = max(nts)
     tf = syn.ricker_tf(ntmax, DT, 6.0, 1.2 / 6.0)
     shots, osh = [], []
-    for nt in nts:
+    for k, nt in enumerate(nts):
+        f0 = 6.0 * (1.0 + 0.45 * k)
         src = _points(rng, 1, n, 2.0)
         rec = _points(rng, int(rng.integers(3, 10)), n, 0.0)
         if kind == "mt":
             mom = rng.uniform(-1.0, 1.0, 3) * 1e3
             shots.append(sw.MomentTensorShot(
-                sw.MomentTensorSources(src, tf[:nt, None], [sw.MomentTensor2D(*mom)], 6.0),
+                sw.MomentTensorSources(src, tf[:nt, None], [sw.MomentTensor2D(*mom)], f0),
                 sw.VectorReceivers(rec, nt)))
-            osh.append(dict(src_pos=src, srctf=tf[:nt, None] * 1.0, moments=mom[None, :], f0=6.0, rec_pos=rec,
+            osh.append(dict(src_pos=src, srctf=tf[:nt, None] * 1.0, moments=mom[None, :], f0=f0, rec_pos=rec,
                             nt=nt))
         else:
             stf = np.zeros((nt, 2, 1))
             stf[:, 0, 0] = rng.uniform(-1, 1) * 1e3 * tf[:nt]
             stf[:, 1, 0] = rng.uniform(-1, 1) * 1e3 * tf[:nt]
-            shots.append(sw.ExternalForceShot(sw.ExternalForceSources(src, stf, 6.0), sw.VectorReceivers(rec, nt)))
-            osh.append(dict(src_pos=src, srctf=stf, f0=6.0, rec_pos=rec, nt=nt))
+            shots.append(sw.ExternalForceShot(sw.ExternalForceSources(src, stf, f0), sw.VectorReceivers(rec, nt)))
+            osh.append(dict(src_pos=src, srctf=stf, f0=f0, rec_pos=rec, nt=nt))
     params = sw.InputParametersElastic(ntmax, DT, n, (H, H),
                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=freetop))
     args = ((H, H), DT, ntmax, halo, 1e-4, freetop)
