@@ -264,6 +264,13 @@ def _strip_cells(n, halo):
     return sum((2 * halo + 2 * halo + 2) * (N // nd) for nd in n)
 
 
+def _el_strip_cells(n, halo):
+    # the eight elastic C-PML memory variables: per axis two arrays of 2h
+    # and two of 2h+2 rows (ela_models.jl:276-299)
+    nx, nz = n
+    return (8 * halo + 4) * nz + (8 * halo + 4) * nx
+
+
 def _launches_two_step(first, nt, ntmax, S, checkpoints):
     """Launches of the two-step forward schedule that touch a shot of nt
     steps: pairs (it, it+1) from `first` on, except a single step where it
@@ -295,19 +302,20 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             last = nt - kl * S              # steps of the last segment: stored during the forward
             rep = nt - last                 # steps replayed from checkpoints
         if physics == "el2d":
-            # two-launch step: stress r ux, uz, lambda, mu, mu_ihjh, w 3 sigma (8e) +
-            # displacement r 3 sigma, uold, ucur (2 each), dt2/rho x2, w 2 u (11e) = 19e.
-            # z-marching step (one launch, stresses stay in LDS): r ucur, uold (2 each),
-            # lambda, mu, mu_ihjh, dt2/rho x2, w 2 u = 11e.  + w u history (2e).
-            # Adjoint: the same + correlation r 3 history levels x 2 + 5 grad RMW (16e)
+            # The ideal elastic step keeps the stresses on chip: forward r ux, uz
+            # at two levels (4e), lambda, mu, mu_ihjh, dt2/rho_x, dt2/rho_z (5e),
+            # w ux, uz (2e) = 11e; + w the u history (13e) on stored steps.
+            # Adjoint 11e + r 3 history levels x 2 components (6e) + 5 gradient
+            # fields read and written (10e) = 27e.  C-PML: the 8 psi arrays read
+            # and written once per step.  A variant with two launches per step
+            # (stress, displacement) is priced at the same bytes per step.
             march_f, march_a = variant[0] > 100, variant[1] > 100
-            sf = 11 if march_f else 19
-            sa = 11 if march_a else 19
             last = nt - (nt // S) * S
             rep = (nt // S) * (S - 1)
-            fwd_b += nt * sf * e * N + last * 2 * e * N + rep * (sf + 2) * e * N
+            est = _el_strip_cells(n, halo) * 2 * e
+            fwd_b += (nt - last) * 11 * e * N + (last + rep) * 13 * e * N + (nt + rep) * est
             fwd_n += (1 if march_f else 2) * (nt + rep)
-            adj_b += nt * (sa + 16) * e * N
+            adj_b += nt * 27 * e * N + nt * est
             adj_n += (1 if march_a else 2) * nt
             continue
         if physics == "vd2d" and 200 <= variant[0] < 300:

@@ -203,7 +203,7 @@ def l2_dchi_du(syn, obs, invcov=None, windows=()):
 
 
 def mute_around_points(grad, points, spacing, radius):
-    """mutearoundmultiplepoints!, src/utils/mute_grad.jl:49-126."""
+    """mutearoundmultiplepoints!, src/utils/mute_grad.jl:3-80."""
     if radius == 0:
         return grad
     grad = grad.copy()
@@ -293,7 +293,7 @@ def cd2d_gradient(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, ad
 # Whole-API restatements (numpy inputs) used by the parity tests
 # ---------------------------------------------------------------------------
 def acoustic_cd_forward(vp, spacing, dt, nt, halo, rcoef, freetop, shots, dtype=np.float64):
-    """run_swforward! (src/apis/forward.jl:161-199) for CD 2D.
+    """run_swforward! (src/apis/forward.jl:71-115) for CD 2D.
 
     shots: list of dicts {src_pos (nsrc,2), srctf (nt,nsrc), f0, rec_pos (nrec,2)}.
     Returns a list of seismograms (nt, nrec)."""
@@ -459,7 +459,7 @@ def acoustic_vd_forward(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, m
 
 def acoustic_vd_gradient(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1,
                          mute_radius_src=0, mute_radius_rec=0, method="arithmetic", dtype=np.float64):
-    """run_swgradient! for VD 2D (gradient.jl:93-136, acou_gradient.jl:222-327).
+    """run_swgradient! for VD 2D (gradient.jl:93-136, acou_gradient.jl:97-203).
     Returns ({"vp", "rho"}, misfit, seismograms)."""
     T = dtype
     vp = np.asarray(vp, dtype=T)

@@ -515,7 +515,7 @@ int SFX(oracle_vd2d_forward)(const oracle_geom *g, const real *m0, const real *m
   return 0;
 }
 
-/* swgradient_1shot! for VD (acou_gradient.jl:222-300) up to the raw
+/* swgradient_1shot! for VD (acou_gradient.jl:97-184) up to the raw
  * grad_m0 and grad_m1_stag fields, with the LinearCheckpointer of
  * acou_models.jl:409-423 (pcur width 1, buffered pcur; pcur, vcur, psi, xi
  * checkpointed every cf steps). */

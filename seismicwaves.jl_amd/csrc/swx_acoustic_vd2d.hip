@@ -6,7 +6,7 @@
 // Reference semantics (paths relative to the reference root):
 //   step       src/models/acoustic/backends/shared/acoustic2D_VD_xPU.jl:1-178
 //   stencils   src/utils/fdgen.jl:65-161 (order 4, zero outside the array)
-//   driver     src/models/acoustic/acou_gradient.jl:222-327
+//   driver     src/models/acoustic/acou_gradient.jl:97-203
 //   correlate  correlate_gradient_xPU.jl:12-21, acoustic2D_VD_xPU.jl:180-198
 //
 // Fusion.  The forward step is p <- p - m0 div~ v, inject, v <- v - m1 grad~ p.

@@ -33,6 +33,26 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::swx::Error((code), (msg));  \
   } while (0)
 
+// Message returned by swx_last_error() on the calling thread.
+void set_last_error(const char *msg);
+
+// Runs f at the C boundary: SWX_OK, or the status of the error it threw
+// with its message kept for swx_last_error().
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return SWX_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return SWX_EINVAL;
+  }
+}
+
 // Owning device allocation (hipMalloc / hipFree).
 class DevBuf {
  public:
@@ -121,6 +141,22 @@ inline PointArgsView view(const PointTables &t) {
                        t.d_urec_ptr.as<int>(), t.d_urec_list.as<int>(), t.d_rec_off.as<int>(),
                        t.d_rec_cell.as<int64_t>(), t.d_srow.as<int>(), t.d_rrow.as<int>(), t.nrows};
 }
+
+// Checks of swx_plan_create on a plan description (throws SWX_EINVAL).
+void validate_plan_desc(const swx_plan_desc &d);
+
+// Host half of PointTables: the same tables in host vectors.
+struct HostPointTables {
+  int nshots = 0;
+  std::vector<int> nt, nsrc, nrec;
+  std::vector<int64_t> off_tf, off_tr;
+  int64_t total_tf = 0, total_tr = 0, nrows = 0;
+  std::vector<int> usrc_off, usrc_ptr, usrc_list, urec_off, urec_ptr, urec_list, rec_off, srow, rrow;
+  std::vector<int64_t> usrc_cell, urec_cell, rec_cell;
+};
+
+// Validates the ABI shot descriptors and builds the host tables.
+void host_point_tables(HostPointTables &t, int ndim, const int64_t *grid, int nshots, const swx_shot_desc *shots);
 
 // Builds PointTables from the ABI shot descriptors.  `grid` are the point
 // counts per axis of the field sources and receivers live on; indices are

@@ -15,24 +15,11 @@ struct swx_plan {
 
 namespace {
 thread_local std::string g_last_error;
-
-template <typename F>
-int guarded(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return SWX_OK;
-  } catch (const swx::Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return SWX_EINVAL;
-  }
-}
 }  // namespace
 
 namespace swx {
+
+void set_last_error(const char *msg) { g_last_error = msg; }
 
 KernelTimer::~KernelTimer() {
   for (auto e : ev_) (void)hipEventDestroy(e);
@@ -100,18 +87,41 @@ int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_
   return (int)best;
 }
 
-void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize, int nshots,
-                        const swx_shot_desc *shots, hipStream_t stream) {
+void validate_plan_desc(const swx_plan_desc &d) {
+  SWX_REQUIRE(d.dtype == SWX_F32 || d.dtype == SWX_F64, SWX_EINVAL, "dtype must be SWX_F32 or SWX_F64");
+  SWX_REQUIRE(d.ndim == 2 || d.ndim == 3, SWX_EINVAL, "Dimensionality must be 2 or 3!");
+  for (int k = 0; k < d.ndim; k++) {
+    SWX_REQUIRE(d.n[k] > 0, SWX_EINVAL, "All numbers of grid points must be positive!");
+    SWX_REQUIRE(d.h[k] > 0, SWX_EINVAL, "All grid spacings must be positive!");
+  }
+  SWX_REQUIRE(d.dt > 0, SWX_EINVAL, "Timestep size must be positive!");
+  SWX_REQUIRE(d.nt_max > 0, SWX_EINVAL, "Number of timesteps must be positive!");
+  SWX_REQUIRE(d.halo >= 0, SWX_EINVAL, "CPML halo size must be non-negative!");
+  SWX_REQUIRE(d.max_shots > 0, SWX_EINVAL, "max_shots must be positive");
+  SWX_REQUIRE(d.check_freq >= 1, SWX_EINVAL, "check_freq must be positive");
+  for (int k = 0; k < d.ndim; k++)
+    SWX_REQUIRE(d.n[k] >= 2 * d.halo + 3, SWX_EINVAL,
+                "Number grid points in the dimensions with C-PML boundaries must be at least 2*halo+3 = " +
+                    std::to_string(2 * d.halo + 3) + "!");
+  int ndev = 0;
+  SWX_HIP(hipGetDeviceCount(&ndev));
+  SWX_REQUIRE(d.device >= 0 && d.device < ndev, SWX_EINVAL, "invalid device ordinal");
+}
+
+void host_point_tables(HostPointTables &t, int ndim, const int64_t *grid, int nshots, const swx_shot_desc *shots) {
   t.nshots = nshots;
   t.nt.assign(nshots, 0);
   t.nsrc.assign(nshots, 0);
   t.nrec.assign(nshots, 0);
   t.off_tf.assign(nshots, 0);
   t.off_tr.assign(nshots, 0);
-  std::vector<int> usrc_off(1, 0), usrc_ptr(1, 0), usrc_list, urec_off(1, 0), urec_ptr(1, 0), urec_list,
-      rec_off(1, 0);
-  std::vector<int64_t> usrc_cell, urec_cell, rec_cell;
-  std::vector<int> srow, rrow;
+  t.usrc_off.assign(1, 0);
+  t.usrc_ptr.assign(1, 0);
+  t.urec_off.assign(1, 0);
+  t.urec_ptr.assign(1, 0);
+  t.rec_off.assign(1, 0);
+  t.usrc_list.clear(); t.urec_list.clear(); t.srow.clear(); t.rrow.clear();
+  t.usrc_cell.clear(); t.urec_cell.clear(); t.rec_cell.clear();
   int64_t rows = 1;
   for (int d = 1; d < ndim; d++) rows *= grid[d];
   t.nrows = rows;
@@ -147,20 +157,20 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
     std::map<int64_t, std::vector<int>> g;
     for (int k = 0; k < sh.nsrc; k++) g[linear(sh.src_idx, sh.nsrc, k, "Source", s)].push_back(k);
     for (auto &kv : g) {
-      usrc_cell.push_back(kv.first);
-      for (int k : kv.second) usrc_list.push_back(k);
-      usrc_ptr.push_back((int)usrc_list.size());
+      t.usrc_cell.push_back(kv.first);
+      for (int k : kv.second) t.usrc_list.push_back(k);
+      t.usrc_ptr.push_back((int)t.usrc_list.size());
     }
     std::map<int64_t, std::vector<int>> gr;
     for (int k = 0; k < sh.nrec; k++) {
       int64_t c = linear(sh.rec_idx, sh.nrec, k, "Receiver", s);
-      rec_cell.push_back(c);
+      t.rec_cell.push_back(c);
       gr[c].push_back(k);
     }
     for (auto &kv : gr) {
-      urec_cell.push_back(kv.first);
-      for (int k : kv.second) urec_list.push_back(k);
-      urec_ptr.push_back((int)urec_list.size());
+      t.urec_cell.push_back(kv.first);
+      for (int k : kv.second) t.urec_list.push_back(k);
+      t.urec_ptr.push_back((int)t.urec_list.size());
     }
     auto rowptr = [&](std::vector<int> &dst, const std::vector<int64_t> &cells, int c0) {
       // dst[r] = first unique cell of this shot whose row is >= r
@@ -170,43 +180,58 @@ void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize
         dst.push_back(c);
       }
     };
-    rowptr(srow, usrc_cell, usrc_off.back());
-    rowptr(rrow, urec_cell, urec_off.back());
-    usrc_off.push_back((int)usrc_cell.size());
-    urec_off.push_back((int)urec_cell.size());
-    rec_off.push_back((int)rec_cell.size());
+    rowptr(t.srow, t.usrc_cell, t.usrc_off.back());
+    rowptr(t.rrow, t.urec_cell, t.urec_off.back());
+    t.usrc_off.push_back((int)t.usrc_cell.size());
+    t.urec_off.push_back((int)t.urec_cell.size());
+    t.rec_off.push_back((int)t.rec_cell.size());
   }
   t.total_tf = otf;
   t.total_tr = otr;
-  t.h_rec_cell = rec_cell;
+}
+
+void build_point_tables(PointTables &t, int ndim, const int64_t *grid, int esize, int nshots,
+                        const swx_shot_desc *shots, hipStream_t stream) {
+  HostPointTables h;
+  host_point_tables(h, ndim, grid, nshots, shots);
+  t.nshots = nshots;
+  t.nt = h.nt;
+  t.nsrc = h.nsrc;
+  t.nrec = h.nrec;
+  t.off_tf = h.off_tf;
+  t.off_tr = h.off_tr;
+  t.nrows = h.nrows;
+  t.total_tf = h.total_tf;
+  t.total_tr = h.total_tr;
+  t.h_rec_cell = h.rec_cell;
   auto up = [&](DevBuf &b, const void *src, size_t bytes) {
     b.alloc(std::max<size_t>(bytes, 16));
     if (bytes) SWX_HIP(hipMemcpyAsync(b.get(), src, bytes, hipMemcpyHostToDevice, stream));
   };
-  up(t.d_nt, t.nt.data(), sizeof(int) * nshots);
-  up(t.d_off_tf, t.off_tf.data(), sizeof(int64_t) * nshots);
-  up(t.d_off_tr, t.off_tr.data(), sizeof(int64_t) * nshots);
-  up(t.d_usrc_off, usrc_off.data(), sizeof(int) * usrc_off.size());
-  up(t.d_usrc_cell, usrc_cell.data(), sizeof(int64_t) * usrc_cell.size());
-  up(t.d_usrc_ptr, usrc_ptr.data(), sizeof(int) * usrc_ptr.size());
-  up(t.d_usrc_list, usrc_list.data(), sizeof(int) * usrc_list.size());
-  up(t.d_urec_off, urec_off.data(), sizeof(int) * urec_off.size());
-  up(t.d_urec_cell, urec_cell.data(), sizeof(int64_t) * urec_cell.size());
-  up(t.d_urec_ptr, urec_ptr.data(), sizeof(int) * urec_ptr.size());
-  up(t.d_urec_list, urec_list.data(), sizeof(int) * urec_list.size());
-  up(t.d_rec_off, rec_off.data(), sizeof(int) * rec_off.size());
-  up(t.d_rec_cell, rec_cell.data(), sizeof(int64_t) * rec_cell.size());
-  up(t.d_srow, srow.data(), sizeof(int) * srow.size());
-  up(t.d_rrow, rrow.data(), sizeof(int) * rrow.size());
+  up(t.d_nt, h.nt.data(), sizeof(int) * nshots);
+  up(t.d_off_tf, h.off_tf.data(), sizeof(int64_t) * nshots);
+  up(t.d_off_tr, h.off_tr.data(), sizeof(int64_t) * nshots);
+  up(t.d_usrc_off, h.usrc_off.data(), sizeof(int) * h.usrc_off.size());
+  up(t.d_usrc_cell, h.usrc_cell.data(), sizeof(int64_t) * h.usrc_cell.size());
+  up(t.d_usrc_ptr, h.usrc_ptr.data(), sizeof(int) * h.usrc_ptr.size());
+  up(t.d_usrc_list, h.usrc_list.data(), sizeof(int) * h.usrc_list.size());
+  up(t.d_urec_off, h.urec_off.data(), sizeof(int) * h.urec_off.size());
+  up(t.d_urec_cell, h.urec_cell.data(), sizeof(int64_t) * h.urec_cell.size());
+  up(t.d_urec_ptr, h.urec_ptr.data(), sizeof(int) * h.urec_ptr.size());
+  up(t.d_urec_list, h.urec_list.data(), sizeof(int) * h.urec_list.size());
+  up(t.d_rec_off, h.rec_off.data(), sizeof(int) * h.rec_off.size());
+  up(t.d_rec_cell, h.rec_cell.data(), sizeof(int64_t) * h.rec_cell.size());
+  up(t.d_srow, h.srow.data(), sizeof(int) * h.srow.size());
+  up(t.d_rrow, h.rrow.data(), sizeof(int) * h.rrow.size());
   // source time functions: concatenated per shot
-  std::vector<char> tf((size_t)otf * esize);
+  std::vector<char> tf((size_t)h.total_tf * esize);
   for (int s = 0; s < nshots; s++)
-    std::memcpy(tf.data() + (size_t)t.off_tf[s] * esize, shots[s].srctf,
+    std::memcpy(tf.data() + (size_t)h.off_tf[s] * esize, shots[s].srctf,
                 (size_t)shots[s].nt * shots[s].nsrc * esize);
   up(t.d_srctf, tf.data(), tf.size());
-  t.d_traces.alloc(std::max<size_t>((size_t)otr * esize, 16));
-  t.d_res.alloc(std::max<size_t>((size_t)otr * esize, 16));
-  SWX_HIP(hipMemsetAsync(t.d_traces.get(), 0, (size_t)otr * esize, stream));
+  t.d_traces.alloc(std::max<size_t>((size_t)h.total_tr * esize, 16));
+  t.d_res.alloc(std::max<size_t>((size_t)h.total_tr * esize, 16));
+  SWX_HIP(hipMemsetAsync(t.d_traces.get(), 0, (size_t)h.total_tr * esize, stream));
   SWX_HIP(hipStreamSynchronize(stream));
 }
 
@@ -235,24 +260,7 @@ int swx_plan_create(const swx_plan_desc *desc, swx_plan **out) {
     SWX_REQUIRE(desc && out, SWX_EINVAL, "null argument");
     *out = nullptr;
     const swx_plan_desc &d = *desc;
-    SWX_REQUIRE(d.dtype == SWX_F32 || d.dtype == SWX_F64, SWX_EINVAL, "dtype must be SWX_F32 or SWX_F64");
-    SWX_REQUIRE(d.ndim == 2 || d.ndim == 3, SWX_EINVAL, "Dimensionality must be 2 or 3!");
-    for (int k = 0; k < d.ndim; k++) {
-      SWX_REQUIRE(d.n[k] > 0, SWX_EINVAL, "All numbers of grid points must be positive!");
-      SWX_REQUIRE(d.h[k] > 0, SWX_EINVAL, "All grid spacings must be positive!");
-    }
-    SWX_REQUIRE(d.dt > 0, SWX_EINVAL, "Timestep size must be positive!");
-    SWX_REQUIRE(d.nt_max > 0, SWX_EINVAL, "Number of timesteps must be positive!");
-    SWX_REQUIRE(d.halo >= 0, SWX_EINVAL, "CPML halo size must be non-negative!");
-    SWX_REQUIRE(d.max_shots > 0, SWX_EINVAL, "max_shots must be positive");
-    SWX_REQUIRE(d.check_freq >= 1, SWX_EINVAL, "check_freq must be positive");
-    for (int k = 0; k < d.ndim; k++)
-      SWX_REQUIRE(d.n[k] >= 2 * d.halo + 3, SWX_EINVAL,
-                  "Number grid points in the dimensions with C-PML boundaries must be at least 2*halo+3 = " +
-                      std::to_string(2 * d.halo + 3) + "!");
-    int ndev = 0;
-    SWX_HIP(hipGetDeviceCount(&ndev));
-    SWX_REQUIRE(d.device >= 0 && d.device < ndev, SWX_EINVAL, "invalid device ordinal");
+    validate_plan_desc(d);
     SWX_HIP(hipSetDevice(d.device));
     auto p = std::make_unique<swx_plan>();
     p->core.d = d;

@@ -83,7 +83,7 @@ def back_interp(method, m, g_stag, dim):
 
 
 def mute_around_points(grad, points, spacing, radius):
-    """mutearoundmultiplepoints! (mute_grad.jl:49-126): scale by r/rmax
+    """mutearoundmultiplepoints! (mute_grad.jl:3-80): scale by r/rmax
     inside a ball of `radius` grid points around each point."""
     if radius == 0:
         return grad
@@ -217,7 +217,7 @@ class _AcousticCPMLWaveSimulation:
         return nt_s
 
     def shot_desc(self, shot):
-        """init_shot! + possrcrec_scaletf (shot.jl:248-253, acou_forward.jl:6-20)."""
+        """init_shot! + possrcrec_scaletf (shot.jl:46-51, acou_forward.jl:6-20)."""
         self.check_shot(shot)
         T = self.dtype
         vel_max = self.cpmlparams.vel_max if self.cpmlparams.vel_max is not None else float(np.max(self.matprop.vp))
@@ -348,7 +348,7 @@ class AcousticVDStaggeredCPMLWaveSimulation(_AcousticCPMLWaveSimulation):
         return shapes
 
     def finish_gradient(self, raw, shots_in_batch, per_shot):
-        """acou_gradient.jl:302-327: back-interpolate the staggered m1
+        """acou_gradient.jl:176-202: back-interpolate the staggered m1
         gradients, mute, and apply the chain rule to vp and rho."""
         T = self.dtype
         vp = np.asarray(self.matprop.vp, dtype=T)
@@ -546,7 +546,7 @@ def check_sim_consistency(wavesim, matprop, shots):
 
 
 def run_swforward(wavesim, matprop, shots):
-    """run_swforward! (forward.jl:161-199), batched."""
+    """run_swforward! (forward.jl:71-115), batched."""
     check_sim_consistency(wavesim, matprop, shots)
     wavesim.set_matprop(matprop)
     for batch, plan in wavesim.batches(len(shots)):

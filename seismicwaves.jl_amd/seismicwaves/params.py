@@ -1,7 +1,7 @@
 """Parameter and material-property types, mirroring the reference's structs.
 
 Reference: src/models/acoustic/acou_params.jl:9-35,
-src/models/acoustic/acou_material_properties.jl:10-81,
+src/models/acoustic/acou_material_properties.jl:10-46,
 src/models/elastic/ela_params.jl:8-34, ela_material_properties.jl:9-22,
 src/models/bdc_params.jl:16-27, src/models/genparameters.jl:25-78.
 
@@ -21,7 +21,7 @@ log = logging.getLogger("seismicwaves")
 
 @dataclass
 class CPMLBoundaryConditionParameters:
-    """CPMLBoundaryConditionParameters, bdc_params.jl:158-167."""
+    """CPMLBoundaryConditionParameters, bdc_params.jl:16-25."""
     halo: int = 20
     rcoef: float = 0.0001
     freeboundtop: bool = True
@@ -30,7 +30,7 @@ class CPMLBoundaryConditionParameters:
 
 @dataclass
 class ReflectiveBoundaryConditionParameters:
-    """ReflectiveBoundaryConditionParameters, bdc_params.jl:149 (not on the
+    """ReflectiveBoundaryConditionParameters, bdc_params.jl:7 (not on the
     GPU path: the MI355X backend implements the C-PML models only)."""
 
 
@@ -62,7 +62,7 @@ class InputParametersElastic(InputParametersAcoustic):
 
 
 class VpAcousticCDMaterialProperties:
-    """VpAcousticCDMaterialProperties, acou_material_properties.jl:45-48."""
+    """VpAcousticCDMaterialProperties, acou_material_properties.jl:10-13."""
 
     def __init__(self, vp):
         self.vp = np.asarray(vp)
@@ -73,7 +73,7 @@ HARMONIC = "harmonic"
 
 
 class VpRhoAcousticVDMaterialProperties:
-    """VpRhoAcousticVDMaterialProperties, acou_material_properties.jl:57-81."""
+    """VpRhoAcousticVDMaterialProperties, acou_material_properties.jl:22-46."""
 
     def __init__(self, vp, rho, interp_method: str = ARITHMETIC):
         self.vp = np.asarray(vp)
@@ -84,7 +84,7 @@ class VpRhoAcousticVDMaterialProperties:
 
 @dataclass
 class RunParameters:
-    """RunParameters, genparameters.jl:119-140.
+    """RunParameters, genparameters.jl:25-46.
 
     `parall` selects the backend; this package provides "mi355x" (the
     reference's symbols :serial/:threads/:CUDA/:AMDGPU/:Metal live in the
@@ -108,7 +108,7 @@ class RunParameters:
 
 @dataclass
 class GradParameters:
-    """GradParameters, genparameters.jl:157-172."""
+    """GradParameters, genparameters.jl:63-78."""
     mute_radius_src: int = 0
     mute_radius_rec: int = 0
     compute_misfit: bool = False

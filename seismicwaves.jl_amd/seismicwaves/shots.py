@@ -1,7 +1,7 @@
 """Sources, receivers, shots and source time functions.
 
-Reference: src/shots/sources.jl:8-91, src/shots/receivers.jl:64-145,
-src/shots/shot.jl:9-44 and :211-216, src/utils/utils.jl:6-20.
+Reference: src/shots/sources.jl:8-91, src/shots/receivers.jl:8-110,
+src/shots/shot.jl:9-42, src/utils/utils.jl:6-20.
 """
 from __future__ import annotations
 
@@ -43,7 +43,7 @@ class ScalarSources:
 
 
 class ScalarReceivers:
-    """ScalarReceivers, receivers.jl:100-145: seismograms (nt, nrec) are
+    """ScalarReceivers, receivers.jl:8-53: seismograms (nt, nrec) are
     allocated here and filled in place by swforward/swgradient."""
 
     def __init__(self, positions, nt: int, dtype=np.float64):
@@ -54,7 +54,7 @@ class ScalarReceivers:
 
 
 class ScalarShot:
-    """ScalarShot, shot.jl:211-216."""
+    """ScalarShot, shot.jl:9-14."""
 
     def __init__(self, srcs: ScalarSources, recs: ScalarReceivers):
         self.srcs = srcs

@@ -8,8 +8,11 @@ pressure history stored; adjoint 17e (9e + two history levels + three
 gradient fields read and written).  CD: forward 4e (r p^n, p^n-1, fact;
 w p^n+1), 5e with the replay history stored; adjoint 7e (+ r history, grad
 read and written), 4e for the first step of each segment, whose correlation
-runs in a kernel of its own.  Elastic: stress 8e + displacement 11e = 19e per
-step, + 2e history; adjoint 19e + 16e correlation."""
+runs in a kernel of its own.  Elastic, priced as the step with the stresses
+kept on chip: forward 11e (r ux, uz at two levels, lambda, mu, mu_ihjh and
+the two dt^2/rho fields; w ux, uz), 13e with the history stored; adjoint 27e
+(11e + three history levels of two components + five gradient fields read
+and written); the eight C-PML memory variables read and written per step."""
 import bench
 
 E = 4
@@ -45,13 +48,16 @@ def test_cd_forward_and_adjoint_bytes():
 
 
 def test_elastic_bytes_per_step():
-    nt, S = 500, 50
+    nt, S = 520, 50
     b = bench.alg_bytes("el2d", NS, 0, E, [nt], S, variant=(1, 1))
-    rep = (nt // S) * (S - 1)
-    # two launches per step; replayed steps also store the history (2e); the
-    # elastic model counts the full-grid fields only
-    assert b[0] == (nt * 19 * E * N + rep * 21 * E * N, 2 * (nt + rep))
-    assert b[1] == (nt * 35 * E * N, 2 * nt)
+    last, rep = 20, (nt // S) * (S - 1)
+    est = (4 * NS[1] + 4 * NS[0]) * 2 * E          # halo 0: 4 rows of 2h+2 per axis
+    # two launches per step, stored and replayed steps at 13e
+    assert b[0] == ((nt - last) * 11 * E * N + (last + rep) * 13 * E * N + (nt + rep) * est, 2 * (nt + rep))
+    assert b[1] == (nt * 27 * E * N + nt * est, 2 * nt)
+    # one launch per step (z-marching variant): same bytes, half the launches
+    m = bench.alg_bytes("el2d", NS, 0, E, [nt], S, variant=(104, 104))
+    assert m[0] == (b[0][0], nt + rep) and m[1] == (b[1][0], nt)
 
 
 def test_shots_add_up():
