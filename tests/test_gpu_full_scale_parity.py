@@ -96,13 +96,20 @@ def test_cd2d_1000_nt4000_gradient_vs_central_differences(cd2d):
         return sw.swmisfit(ws_f, sw.VpAcousticCDMaterialProperties(v), [_scalar_shot(sh, F64)], misfit)
 
     # 1-based cells: C-PML is i <= 20, i >= 981, j >= 981 (free top); source
-    # (501, 3), receivers on row 3 from i = 21 to 980
+    # (501, 3), receivers on row 3 from i = 21 to 980.  Within 4000 steps
+    # (2 s) the wave path from the surface and back only reaches about 2 km
+    # down: deeper cells have no sensitivity (the gradient underflows to 0
+    # there and so does the misfit change), so cells are drawn from the
+    # upper 300 rows and a cell is kept if its gradient is at least 1e-4 of
+    # the largest one there.
     rng = np.random.default_rng(20)
     cells = []
     vmax = float(v0.max())
+    gmax = float(np.max(np.abs(g[30:970, 13:300])))
     while len(cells) < 20:
-        i, j = int(rng.integers(31, 971)), int(rng.integers(14, 971))
-        if abs(i - 501) + abs(j - 3) >= 10 and v0[i - 1, j - 1] < vmax * (1 - 2e-3) and (i, j) not in cells:
+        i, j = int(rng.integers(31, 971)), int(rng.integers(14, 301))
+        if abs(i - 501) + abs(j - 3) >= 10 and v0[i - 1, j - 1] < vmax * (1 - 2e-3) and (i, j) not in cells \
+                and abs(g[i - 1, j - 1]) >= 1e-4 * gmax:
             cells.append((i, j))
     errs = []
     for (i, j) in cells:
