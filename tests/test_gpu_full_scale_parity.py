@@ -100,8 +100,9 @@ def test_cd2d_1000_nt4000_gradient_vs_central_differences(cd2d):
     # (2 s) the wave path from the surface and back only reaches about 2 km
     # down: deeper cells have no sensitivity (the gradient underflows to 0
     # there and so does the misfit change), so cells are drawn from the
-    # upper 300 rows and a cell is kept if its gradient is at least 1e-4 of
-    # the largest one there.
+    # upper 300 rows and a cell is kept if its gradient is at least 1e-2 of
+    # the largest one there (below that, the FP64 rounding of the misfit
+    # difference, ~1e-16 J / (g delta), reaches 1e-5 of the difference).
     rng = np.random.default_rng(20)
     cells = []
     vmax = float(v0.max())
@@ -109,7 +110,7 @@ def test_cd2d_1000_nt4000_gradient_vs_central_differences(cd2d):
     while len(cells) < 20:
         i, j = int(rng.integers(31, 971)), int(rng.integers(14, 301))
         if abs(i - 501) + abs(j - 3) >= 10 and v0[i - 1, j - 1] < vmax * (1 - 2e-3) and (i, j) not in cells \
-                and abs(g[i - 1, j - 1]) >= 1e-4 * gmax:
+                and abs(g[i - 1, j - 1]) >= 1e-2 * gmax:
             cells.append((i, j))
     errs = []
     for (i, j) in cells:
