@@ -219,6 +219,28 @@ int swx_forward(swx_plan *plan, void *traces, int save_state);
  *                    grad_mu_ihalf_jhalf (nx-1,nz-1) */
 int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot);
 
+/* Bytes of the static device array that swx_forward_noalloc works in. */
+#define SWX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the forward time loop of ONE 2D acoustic
+ * constant-density shot, the work of swx_plan_create + swx_set_model +
+ * swx_set_shots + swx_forward on a 1-shot plan, with the same kernels and
+ * bitwise the same seismograms, but without any device allocation.  All
+ * device memory (model, time levels, C-PML memory variables and
+ * coefficients, point tables, source time function, traces) is carved out
+ * of a static device array of SWX_NOALLOC_SCRATCH_BYTES that every GPU the
+ * library runs on holds until the process ends; a shot that needs more
+ * returns SWX_ENOMEM.  desc: physics SWX_ACOUSTIC_CD, ndim 2, nt_max >=
+ * shot->nt (max_shots, check_freq, gradient and mem_budget are not used).
+ * fact, shot and traces are host memory as for swx_set_model, swx_set_shots
+ * and swx_forward.  Work runs on the device's null stream and the call
+ * returns when the traces are copied.  All calls on one GPU share its
+ * array, which keeps the last call's contents: calls that overlap on one
+ * GPU (from several host threads) race. */
+int swx_forward_noalloc(const swx_plan_desc *desc, const void *fact, const swx_shot_desc *shot, void *traces);
+
 /* Sizes, segment length and device timings of the last calls. */
 int swx_plan_get_info(swx_plan *plan, swx_plan_info *info);
 

@@ -20,6 +20,7 @@
 //   grad   [ns][nx*nz]     per-shot raw gradient (reduced in shot order)
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 
 #include "swx_device.hpp"
 #include "swx_internal.hpp"
@@ -584,3 +585,194 @@ std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core) {
 }
 
 }  // namespace swx
+
+// ---------------------------------------------------------------------------
+// swx_forward_noalloc: the forward of one shot without any device allocation
+// ---------------------------------------------------------------------------
+// Every byte of device memory the call uses (model, three time levels, C-PML
+// memory variables and coefficients, point tables, source time function and
+// traces) is carved out of this array.  Each GPU the library runs on holds
+// its own copy until the process ends; calls on one GPU share it.
+static __device__ __align__(256) unsigned char scratch[SWX_NOALLOC_SCRATCH_BYTES];
+
+namespace swx {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `dev`, looked up once
+// per device.  The host-side name `scratch` is not a device address.
+void *scratch_on(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  SWX_REQUIRE(dev >= 0 && dev < kMaxDevices, SWX_EINVAL, "invalid device ordinal");
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    SWX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SWX_REQUIRE(ptr != nullptr, SWX_EHIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_dev[dev] = ptr;
+  }
+  return g_scratch_dev[dev];
+}
+
+__global__ void zero_words(uint32_t *p, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    p[q] = 0u;
+}
+
+// Offsets of the pieces in the scratch array, each aligned to 256 bytes.
+struct Carve {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    const size_t o = off;
+    off += bytes;
+    return o;
+  }
+};
+
+template <typename T>
+void forward_noalloc(const swx_plan_desc &d, const T *fact, const swx_shot_desc &shot, T *traces_host) {
+  const int nx = (int)d.n[0], nz = (int)d.n[1], h = d.halo;
+  const int64_t N = (int64_t)nx * nz;
+  const int64_t spx = (int64_t)2 * h * nz, spz = (int64_t)nx * 2 * h;
+  const int64_t sxx = (int64_t)(2 * h + 2) * nz, sxz = (int64_t)nx * (2 * h + 2);
+  const int cstride = 2 * (8 * h + 4);
+  HostPointTables t;
+  host_point_tables(t, 2, d.n, 1, &shot);
+  const std::vector<T> cp = pack_cpml<T>(1, &shot, 2, h);
+  const int nt = shot.nt;
+  // the pieces copied from the host come first, the ones cleared on the
+  // device after them
+  Carve cv;
+  const size_t o_fact = cv.take(N * sizeof(T));
+  const size_t o_cpml = cv.take(cp.size() * sizeof(T));
+  const size_t o_nt = cv.take(sizeof(int));
+  const size_t o_offtf = cv.take(sizeof(int64_t)), o_offtr = cv.take(sizeof(int64_t));
+  const size_t o_usrc_off = cv.take(t.usrc_off.size() * sizeof(int));
+  const size_t o_usrc_cell = cv.take(t.usrc_cell.size() * sizeof(int64_t));
+  const size_t o_usrc_ptr = cv.take(t.usrc_ptr.size() * sizeof(int));
+  const size_t o_usrc_list = cv.take(t.usrc_list.size() * sizeof(int));
+  const size_t o_urec_off = cv.take(t.urec_off.size() * sizeof(int));
+  const size_t o_urec_cell = cv.take(t.urec_cell.size() * sizeof(int64_t));
+  const size_t o_urec_ptr = cv.take(t.urec_ptr.size() * sizeof(int));
+  const size_t o_urec_list = cv.take(t.urec_list.size() * sizeof(int));
+  const size_t o_rec_off = cv.take(t.rec_off.size() * sizeof(int));
+  const size_t o_rec_cell = cv.take(t.rec_cell.size() * sizeof(int64_t));
+  const size_t o_srow = cv.take(t.srow.size() * sizeof(int));
+  const size_t o_rrow = cv.take(t.rrow.size() * sizeof(int));
+  const size_t o_tf = cv.take((size_t)t.total_tf * sizeof(T));
+  const size_t copy_end = cv.take(0);
+  const size_t o_p = cv.take((size_t)3 * N * sizeof(T));
+  const size_t o_psix = cv.take((size_t)2 * spx * sizeof(T)), o_psiz = cv.take((size_t)2 * spz * sizeof(T));
+  const size_t o_xix = cv.take((size_t)sxx * sizeof(T)), o_xiz = cv.take((size_t)sxz * sizeof(T));
+  const size_t o_tr = cv.take((size_t)t.total_tr * sizeof(T));
+  const size_t total = cv.take(0);
+  SWX_REQUIRE(total <= SWX_NOALLOC_SCRATCH_BYTES, SWX_ENOMEM,
+              "swx_forward_noalloc: the shot needs " + std::to_string(total) + " bytes of device memory, more than "
+              "the " + std::to_string((size_t)SWX_NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+
+  std::vector<char> stage(copy_end);
+  auto put = [&](size_t o, const void *src, size_t bytes) {
+    if (bytes) std::memcpy(stage.data() + o, src, bytes);
+  };
+  const int64_t zero64 = 0;
+  put(o_fact, fact, N * sizeof(T));
+  put(o_cpml, cp.data(), cp.size() * sizeof(T));
+  put(o_nt, &nt, sizeof(int));
+  put(o_offtf, &zero64, sizeof(int64_t));
+  put(o_offtr, &zero64, sizeof(int64_t));
+  put(o_usrc_off, t.usrc_off.data(), t.usrc_off.size() * sizeof(int));
+  put(o_usrc_cell, t.usrc_cell.data(), t.usrc_cell.size() * sizeof(int64_t));
+  put(o_usrc_ptr, t.usrc_ptr.data(), t.usrc_ptr.size() * sizeof(int));
+  put(o_usrc_list, t.usrc_list.data(), t.usrc_list.size() * sizeof(int));
+  put(o_urec_off, t.urec_off.data(), t.urec_off.size() * sizeof(int));
+  put(o_urec_cell, t.urec_cell.data(), t.urec_cell.size() * sizeof(int64_t));
+  put(o_urec_ptr, t.urec_ptr.data(), t.urec_ptr.size() * sizeof(int));
+  put(o_urec_list, t.urec_list.data(), t.urec_list.size() * sizeof(int));
+  put(o_rec_off, t.rec_off.data(), t.rec_off.size() * sizeof(int));
+  put(o_rec_cell, t.rec_cell.data(), t.rec_cell.size() * sizeof(int64_t));
+  put(o_srow, t.srow.data(), t.srow.size() * sizeof(int));
+  put(o_rrow, t.rrow.data(), t.rrow.size() * sizeof(int));
+  put(o_tf, shot.srctf, (size_t)t.total_tf * sizeof(T));
+
+  char *base = static_cast<char *>(scratch_on(d.device));
+  const hipStream_t stream = nullptr;
+  SWX_HIP(hipMemcpyAsync(base, stage.data(), copy_end, hipMemcpyHostToDevice, stream));
+  // the time levels, memory variables and traces start at zero
+  const int64_t words = (int64_t)(total - copy_end) / 4;
+  zero_words<<<(unsigned)std::min<int64_t>(ceil_div(words, 256), 2048), 256, 0, stream>>>(
+      reinterpret_cast<uint32_t *>(base + copy_end), words);
+  SWX_HIP(hipGetLastError());
+
+  auto at = [&](size_t o) { return base + o; };
+  PointArgsView pv{reinterpret_cast<const int *>(at(o_nt)), reinterpret_cast<const int64_t *>(at(o_offtf)),
+                   reinterpret_cast<const int64_t *>(at(o_offtr)), reinterpret_cast<const int *>(at(o_usrc_off)),
+                   reinterpret_cast<const int64_t *>(at(o_usrc_cell)), reinterpret_cast<const int *>(at(o_usrc_ptr)),
+                   reinterpret_cast<const int *>(at(o_usrc_list)), reinterpret_cast<const int *>(at(o_urec_off)),
+                   reinterpret_cast<const int64_t *>(at(o_urec_cell)), reinterpret_cast<const int *>(at(o_urec_ptr)),
+                   reinterpret_cast<const int *>(at(o_urec_list)), reinterpret_cast<const int *>(at(o_rec_off)),
+                   reinterpret_cast<const int64_t *>(at(o_rec_cell)), reinterpret_cast<const int *>(at(o_srow)),
+                   reinterpret_cast<const int *>(at(o_rrow)), t.nrows};
+  T *p = reinterpret_cast<T *>(at(o_p));
+  T *psix = reinterpret_cast<T *>(at(o_psix));
+  T *psiz = reinterpret_cast<T *>(at(o_psiz));
+  T *tr = reinterpret_cast<T *>(at(o_tr));
+  constexpr int R = 2;
+  Cd2dStepArgs<T> a{};
+  a.nx = nx; a.nz = nz; a.h = h; a.N = N;
+  a.idx = T(1.0 / d.h[0]);
+  a.idz = T(1.0 / d.h[1]);
+  a.idx2 = a.idx * a.idx;
+  a.idz2 = a.idz * a.idz;
+  a.fact = reinterpret_cast<const T *>(at(o_fact));
+  a.spx = spx; a.spz = spz; a.sxx = sxx; a.sxz = sxz;
+  a.xix = reinterpret_cast<T *>(at(o_xix));
+  a.xiz = reinterpret_cast<T *>(at(o_xiz));
+  a.cpml = reinterpret_cast<const T *>(at(o_cpml)); a.cstride = cstride;
+  a.nt_shot = pv.nt;
+  a.S = 1;
+  a.pt = pv;
+  a.srctf = reinterpret_cast<const T *>(at(o_tf));
+  a.traces = tr;
+  a.lean = 1;
+  a.nbx = (int)ceil_div(nx, TX);
+  a.nby = (int)ceil_div(nz, TZ * R);
+  const unsigned nb = (unsigned)((int64_t)a.nbx * a.nby);
+  for (int it = 1; it <= nt; it++) {
+    a.it = it;
+    a.pold = p + (int64_t)mod3(it - 2) * N;
+    a.pcur = p + (int64_t)mod3(it - 1) * N;
+    a.pnew = p + (int64_t)mod3(it) * N;
+    a.psix_o = psix + (int64_t)((it - 1) & 1) * spx;
+    a.psix_n = psix + (int64_t)(it & 1) * spx;
+    a.psiz_o = psiz + (int64_t)((it - 1) & 1) * spz;
+    a.psiz_n = psiz + (int64_t)(it & 1) * spz;
+    cd2d_step_kernel<T, R><<<nb, dim3(TX, TZ), 0, stream>>>(a);
+    SWX_HIP(hipGetLastError());
+  }
+  SWX_HIP(hipMemcpyAsync(traces_host, tr, (size_t)t.total_tr * sizeof(T), hipMemcpyDeviceToHost, stream));
+  SWX_HIP(hipStreamSynchronize(stream));
+}
+
+}  // namespace
+}  // namespace swx
+
+extern "C" int swx_forward_noalloc(const swx_plan_desc *desc, const void *fact, const swx_shot_desc *shot,
+                                   void *traces) {
+  using namespace swx;
+  return guarded([&] {
+    SWX_REQUIRE(desc && fact && shot && traces, SWX_EINVAL, "null argument");
+    const swx_plan_desc &d = *desc;
+    SWX_REQUIRE(d.physics == SWX_ACOUSTIC_CD && d.ndim == 2, SWX_EUNSUPPORTED,
+                "swx_forward_noalloc runs 2D acoustic constant-density shots");
+    validate_plan_desc(d);
+    SWX_REQUIRE(shot->nt >= 1 && shot->nt <= d.nt_max, SWX_EINVAL, "shot nt must be in 1..nt_max");
+    SWX_HIP(hipSetDevice(d.device));
+    if (d.dtype == SWX_F32)
+      forward_noalloc<float>(d, static_cast<const float *>(fact), *shot, static_cast<float *>(traces));
+    else
+      forward_noalloc<double>(d, static_cast<const double *>(fact), *shot, static_cast<double *>(traces));
+  });
+}

@@ -22,8 +22,10 @@ SWX_SRC_MOMENT_TENSOR, SWX_SRC_EXTERNAL_FORCE = 1, 2
 EXPORTS = (
     "swx_abi_version", "swx_last_error", "swx_plan_create", "swx_plan_destroy",
     "swx_set_model", "swx_set_shots", "swx_set_shots_elastic", "swx_forward", "swx_adjoint",
-    "swx_plan_get_info", "swx_set_kernel_timing",
+    "swx_plan_get_info", "swx_set_kernel_timing", "swx_forward_noalloc",
 )
+# Bytes of the static device array swx_forward_noalloc works in.
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class PlanDesc(ctypes.Structure):
@@ -94,6 +96,7 @@ def load(path: str | None = None):
     lib.swx_adjoint.argtypes = [vp, vp, vp, ctypes.c_int]
     lib.swx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     lib.swx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    lib.swx_forward_noalloc.argtypes = [ctypes.POINTER(PlanDesc), vp, ctypes.POINTER(ShotDesc), vp]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int if name != "swx_last_error" else ctypes.c_char_p
     _lib = lib
@@ -117,28 +120,77 @@ def ptr(a: np.ndarray | None):
     return ctypes.c_void_p(0) if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
+def plan_desc(*, physics, dtype, n, h, dt, halo, freetop, max_shots, nt_max, check_freq=1, gradient=False,
+              device=0, mem_budget=0.0) -> PlanDesc:
+    d = PlanDesc()
+    d.physics = physics
+    d.dtype = SWX_F32 if np.dtype(dtype) == np.float32 else SWX_F64
+    d.ndim = len(n)
+    d.halo = int(halo)
+    for k in range(3):
+        d.n[k] = int(n[k]) if k < len(n) else 1
+        d.h[k] = float(h[k]) if k < len(h) else 1.0
+    d.dt = float(dt)
+    d.freetop = int(bool(freetop))
+    d.max_shots = int(max_shots)
+    d.nt_max = int(nt_max)
+    d.check_freq = int(check_freq)
+    d.gradient = int(bool(gradient))
+    d.device = int(device)
+    d.mem_budget = float(mem_budget)
+    return d
+
+
+def shot_descs(shots, dtype):
+    """ShotDesc array for dicts with nt, src_idx (nsrc, ndim) 1-based, srctf
+    (nt, nsrc), rec_idx (nrec, ndim), cpml [axis] -> dict a, a_h, b, b_h;
+    returns (descs, arrays that must outlive the call)."""
+    keep = []
+    descs = (ShotDesc * len(shots))()
+    for k, sh in enumerate(shots):
+        si = np.asfortranarray(sh["src_idx"], dtype=np.int64)
+        ri = np.asfortranarray(sh["rec_idx"], dtype=np.int64)
+        tf = np.asfortranarray(sh["srctf"], dtype=dtype)
+        keep += [si, ri, tf]
+        dd = descs[k]
+        dd.nt = int(sh["nt"])
+        dd.nsrc = si.shape[0]
+        dd.src_idx = si.ctypes.data
+        dd.srctf = tf.ctypes.data
+        dd.nrec = ri.shape[0]
+        dd.rec_idx = ri.ctypes.data
+        for ax, co in enumerate(sh["cpml"]):
+            for j, key in enumerate(("a", "a_h", "b", "b_h")):
+                arr = np.ascontiguousarray(co[key], dtype=dtype)
+                keep.append(arr)
+                dd.cpml[ax][j] = arr.ctypes.data if arr.size else None
+    return descs, keep
+
+
+def forward_noalloc(*, dtype, n, h, dt, halo, fact, shot, device=0):
+    """swx_forward_noalloc: the forward of one 2D acoustic CD shot (a dict as
+    for Plan.set_shots) on the model factor `fact` (dt^2 vp^2), run in the
+    library's static scratch array; returns the traces (nt, nrec)."""
+    d = plan_desc(physics=SWX_ACOUSTIC_CD, dtype=dtype, n=n, h=h, dt=dt, halo=halo, freetop=False, max_shots=1,
+                  nt_max=int(shot["nt"]), device=device)
+    descs, keep = shot_descs([shot], dtype)
+    f = np.asfortranarray(fact, dtype=dtype)
+    nt, nrec = int(shot["nt"]), np.atleast_2d(shot["rec_idx"]).shape[0]
+    traces = np.zeros((nt, nrec), dtype=dtype, order="F")
+    check(load().swx_forward_noalloc(ctypes.byref(d), ptr(f), descs, ptr(traces)))
+    del keep
+    return np.ascontiguousarray(traces)
+
+
 class Plan:
     """Owner of one swx_plan (device state for a batch of shots)."""
 
     def __init__(self, *, physics, dtype, n, h, dt, halo, freetop, max_shots, nt_max,
                  check_freq=1, gradient=False, device=0, mem_budget=0.0):
         lib = load()
-        d = PlanDesc()
-        d.physics = physics
-        d.dtype = SWX_F32 if np.dtype(dtype) == np.float32 else SWX_F64
-        d.ndim = len(n)
-        d.halo = int(halo)
-        for k in range(3):
-            d.n[k] = int(n[k]) if k < len(n) else 1
-            d.h[k] = float(h[k]) if k < len(h) else 1.0
-        d.dt = float(dt)
-        d.freetop = int(bool(freetop))
-        d.max_shots = int(max_shots)
-        d.nt_max = int(nt_max)
-        d.check_freq = int(check_freq)
-        d.gradient = int(bool(gradient))
-        d.device = int(device)
-        d.mem_budget = float(mem_budget)
+        d = plan_desc(physics=physics, dtype=dtype, n=n, h=h, dt=dt, halo=halo, freetop=freetop,
+                      max_shots=max_shots, nt_max=nt_max, check_freq=check_freq, gradient=gradient, device=device,
+                      mem_budget=mem_budget)
         self.dtype = np.dtype(dtype)
         self.ndim = len(n)
         self._h = ctypes.c_void_p()
@@ -164,25 +216,7 @@ class Plan:
     def set_shots(self, shots):
         """shots: list of dicts with nt, src_idx (nsrc, ndim) 1-based, srctf
         (nt, nsrc), rec_idx (nrec, ndim), cpml [axis] -> dict a, a_h, b, b_h."""
-        keep = []
-        descs = (ShotDesc * len(shots))()
-        for k, sh in enumerate(shots):
-            si = np.asfortranarray(sh["src_idx"], dtype=np.int64)
-            ri = np.asfortranarray(sh["rec_idx"], dtype=np.int64)
-            tf = np.asfortranarray(sh["srctf"], dtype=self.dtype)
-            keep += [si, ri, tf]
-            dd = descs[k]
-            dd.nt = int(sh["nt"])
-            dd.nsrc = si.shape[0]
-            dd.src_idx = si.ctypes.data
-            dd.srctf = tf.ctypes.data
-            dd.nrec = ri.shape[0]
-            dd.rec_idx = ri.ctypes.data
-            for ax, co in enumerate(sh["cpml"]):
-                for j, key in enumerate(("a", "a_h", "b", "b_h")):
-                    arr = np.ascontiguousarray(co[key], dtype=self.dtype)
-                    keep.append(arr)
-                    dd.cpml[ax][j] = arr.ctypes.data if arr.size else None
+        descs, keep = shot_descs(shots, self.dtype)
         check(load().swx_set_shots(self._h, len(shots), descs))
         self._keep = keep
 
