@@ -2,10 +2,11 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one full gradient of the workload's batch of shots: forward with
-device-resident checkpoints, host adjoint source, adjoint with replay and
-gradient correlation; all inputs are uploaded to HBM before the timed
-region.  The JSON line carries
+A step is one full gradient of the workload's shots: forward with
+device-resident checkpoints (traces copied to the host, as swgradient fills
+the seismograms), L2 adjoint source and misfit on the device, adjoint with
+replay and gradient correlation; shots are batched as far as the HBM budget
+allows.  All inputs are uploaded to HBM before the timed region.  The JSON line carries
 
   value        GCell-updates/s = 2 * cells * sum(nt of every shot) / time
                (one forward and one adjoint update per cell and time step;
@@ -106,7 +107,7 @@ def workload_cd2d_grad_1000():
                         "check_freq": 50, "halo": halo})
 
 
-def workload_cd3d_600x600x400(nshots=2, nt=3000):
+def workload_cd3d_600x600x400(nshots=8, nt=3000):
     """3D acoustic constant-density gradient: 600 x 600 x 400 cells of 10 m,
     dt = 1 ms, nt = 3000, C-PML (20 cells) on all faces, 7 Hz Ricker.  vp:
     8 seeded layers (1500-5000 m/s with depth) plus 50 ellipsoidal
@@ -114,8 +115,8 @@ def workload_cd3d_600x600x400(nshots=2, nt=3000):
     over 100 m in depth.  Shots on a 2 x 4 grid at 250 m depth (just below
     the top C-PML), each with a 100 x 100 patch of receivers at 10 m spacing
     centred on it at 240 m depth;
-    checkpoint every 50 steps.  The first `nshots` of the 8 shots are run as
-    one batch (two fit one plan in HBM with their checkpoints)."""
+    checkpoint every 50 steps.  The shots run in batches of the size whose
+    state and checkpoints fit the HBM budget (two of these shots per plan)."""
     import seismicwaves as sw
     from seismicwaves import synthetic as syn
     nx, ny, nz = 600, 600, 400
@@ -318,6 +319,17 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             adj_b += nt * 27 * e * N + nt * est
             adj_n += (1 if march_a else 2) * nt
             continue
+        if physics == "vd2d" and (400 <= variant[0] < 500 or 400 <= variant[1] < 500):
+            # two shots per block: the model factors m0, m1x, m1z (3e) are read
+            # once per shot pair and step (counted below per pair); per shot
+            # forward r/w p, vx, vz (6e) + w p-history, adjoint 14e
+            pf, pa = 400 <= variant[0] < 500, 400 <= variant[1] < 500
+            fwd_b += ((nt + rep) * (6 if pf else 9) + (last + rep)) * e * N
+            fwd_b += (nt + rep) * strip
+            fwd_n += nt + rep
+            adj_b += nt * (14 if pa else 17) * e * N + nt * strip
+            adj_n += nt
+            continue
         if physics == "vd2d" and 200 <= variant[0] < 300:
             # two steps per launch: a launch reads p, vx, vz, m0, m1x, m1z and
             # writes p, vx, vz once (9e) whether it advances the shot one or two
@@ -343,6 +355,15 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             nseg = kl + 1
             adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
             adj_n += nt
+    if physics == "vd2d":
+        # model factors once per shot pair and step the pair steps (its
+        # longer shot's forward and replay steps; adjoint steps)
+        for k in range(0, len(nts), 2):
+            pair = nts[k:k + 2]
+            if 400 <= variant[0] < 500:
+                fwd_b += 3 * e * N * max(nt + (nt // S) * (S - 1) for nt in pair)
+            if 400 <= variant[1] < 500:
+                adj_b += 3 * e * N * max(pair)
     return {0: (fwd_b, fwd_n), 1: (adj_b, adj_n)}
 
 
@@ -351,6 +372,8 @@ def kernel_names(physics, variant):
     picked (variant codes of swx_plan_info.kernel_variant)."""
     if physics == "vd2d":
         def name(kind, v):
+            if v >= 400:   # two shots per block, one launch
+                return f"vd2d_{kind}_pair_kernel<float, {v - 400}>"
             if v >= 300:   # interior and other tiles in one launch
                 return f"vd2d_{kind}_kernel<float, {v - 300}, true>"
             if v >= 200:   # two steps per launch
@@ -371,8 +394,16 @@ def kernel_names(physics, variant):
                     "el_disp_kernel<float, false>" + tail)
         return name(False, variant[0]), name(True, variant[1])
     if physics == "cd3d":
-        return "cd3d_step_kernel<float, 4, 0> (forward)", "cd3d_step_kernel<float, 4, 0> (adjoint)"
-    return "cd2d_step_kernel<float, 2> (forward)", "cd2d_step_kernel<float, 2> (adjoint)"
+        # [launch shape, planes per z chunk]
+        v = variant[0] or 4
+        if v > 100:   # interior tiles and the others in two launches
+            k = f"cd3d_step_kernel<float, {v - 100}, 1> + cd3d_step_kernel<float, {v - 100}, 2>"
+        else:
+            k = f"cd3d_step_kernel<float, {v}, 0>"
+        return f"{k} (forward)", f"{k} (adjoint)"
+    # [rows per thread, 0]
+    k = f"cd2d_step_kernel<float, {variant[0] or 2}>"
+    return f"{k} (forward)", f"{k} (adjoint)"
 
 
 def traffic_from_profiles(workload, kernel):
@@ -393,6 +424,14 @@ def traffic_from_profiles(workload, kernel):
 # Gradient driver on an already-populated plan
 # ---------------------------------------------------------------------------
 class Runner:
+    """One gradient of a group of shots per step, through the same calls as
+    seismicwaves.swgradient: batches as large as the HBM budget allows
+    (one plan, reused), forward with device checkpoints and the traces
+    copied to the host, the L2 adjoint source and misfit formed on the
+    device, adjoint with replay and correlation.  With one batch the shots
+    and observed data stay on the device between steps, as in an inversion
+    loop over a fixed acquisition."""
+
     def __init__(self, wl, shots, max_batch=None):
         import seismicwaves as sw
         p = wl["params"]
@@ -401,40 +440,57 @@ class Runner:
                                    gradparams=sw.GradParameters(check_freq=wl["check_freq"]), gradient=True)
         self.ws.set_matprop(wl["mp0"])
         self.descs = [self.ws.shot_desc(sh) for sh in shots]
-        self.misfits = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in shots]
-        self.plan = self.ws._ensure_plan(len(self.descs))
-        self.ws.upload_shots(self.plan, self.descs)
-        self.ntr = sum(int(np.prod(d["tshape"])) for d in self.descs)
-        self.traces = np.zeros(self.ntr, dtype=np.float32)
+        self.observed = [np.asfortranarray(sh.recs.seismograms, dtype=np.float32).ravel(order="F") for sh in shots]
+        self.batches = [b for b, _ in self.ws.batches(len(self.descs))]
+        self.plan = self.ws._plan
         size = sum(int(np.prod(s)) for s in self.ws.grad_fields_shapes())
         self.grad = np.zeros(size, dtype=np.float32)
+        self.gsum = np.zeros(size, dtype=np.float32)
+        self._bufs = []
+        for b in self.batches:
+            ntr = sum(int(np.prod(self.descs[k]["tshape"])) for k in b)
+            self._bufs.append((np.zeros(ntr, dtype=np.float32), np.concatenate([self.observed[k] for k in b])))
+        self._loaded = -1
+        if len(self.batches) == 1:
+            self._load(0)
+
+    def _load(self, i):
+        if self._loaded == i:
+            return
+        self.ws.upload_shots(self.plan, [self.descs[k] for k in self.batches[i]])
+        self.plan.set_observed(self._bufs[i][1])
+        self._loaded = i
 
     def step(self):
-        self.plan.forward(self.traces, True)
-        off, adj = 0, []
-        for d, m in zip(self.descs, self.misfits):
-            n = int(np.prod(d["tshape"]))
-            syn_ = self.traces[off:off + n].reshape(d["tshape"], order="F")
-            adj.append(np.asfortranarray(-(syn_.astype(np.float64) - m.observed), dtype=np.float32).ravel(order="F"))
-            off += n
-        self.plan.adjoint(np.concatenate(adj), self.grad, False)
+        self.gsum[:] = 0
+        for i, b in enumerate(self.batches):
+            self._load(i)
+            self.plan.forward(self._bufs[i][0], True)
+            self.plan.adjoint_l2(self.grad, False, len(b))
+            self.gsum += self.grad
 
     def timed_kernels(self):
-        """One gradient with HIP events around every step-kernel launch."""
+        """One gradient with HIP events around every step-kernel launch;
+        returns the kernel times and launch counts and the device time."""
         self.plan.set_kernel_timing(True)
         ms, n = [0.0, 0.0], [0, 0]
-        self.plan.forward(self.traces, True)
-        i = self.plan.info()
-        for k in range(2):
-            ms[k] += i.ms_kernel[k]
-            n[k] += i.n_kernel[k]
-        self.plan.adjoint(np.zeros(self.ntr, dtype=np.float32), self.grad, False)
-        i = self.plan.info()
-        for k in range(2):
-            ms[k] += i.ms_kernel[k]
-            n[k] += i.n_kernel[k]
+        dev = 0.0
+        for i, b in enumerate(self.batches):
+            self._load(i)
+            self.plan.forward(self._bufs[i][0], True)
+            inf = self.plan.info()
+            dev += inf.ms_forward
+            for k in range(2):
+                ms[k] += inf.ms_kernel[k]
+                n[k] += inf.n_kernel[k]
+            self.plan.adjoint_l2(self.grad, False, len(b))
+            inf = self.plan.info()
+            dev += inf.ms_adjoint
+            for k in range(2):
+                ms[k] += inf.ms_kernel[k]
+                n[k] += inf.n_kernel[k]
         self.plan.set_kernel_timing(False)
-        return ms, n
+        return ms, n, dev
 
     def close(self):
         self.plan.close()
@@ -488,7 +544,7 @@ def cpu_baseline(wl, nshots_sample=4, nt_sample=1000):
                       f"reference's threads backend (not the Julia binary); {el:.2f} s"}
 
 
-def cpu_baseline_cd3d(wl, cores, nt_sample=40, cf=20):
+def cpu_baseline_cd3d(wl, cores, nt_sample=100, cf=20):
     """The oracle's FP32 OpenMP 3D CD gradient on the workload's first shot,
     nt_sample steps with a checkpoint every cf steps."""
     import oracle
@@ -578,7 +634,9 @@ def main():
         t0 = time.perf_counter()
         for _ in range(args.steps):
             for r in runs:
-                r.plan.forward(r.traces, True)
+                for i in range(len(r.batches)):
+                    r._load(i)
+                    r.plan.forward(r._bufs[i][0], True)
         ms = (time.perf_counter() - t0) / args.steps * 1e3
         for r in runs:
             r.close()
@@ -611,13 +669,13 @@ def main():
     kms, kn = [0.0, 0.0], [0, 0]
     dev_ms = 0.0
     for r in runs:
-        m, n = r.timed_kernels()
-        i = r.plan.info()
-        dev_ms += i.ms_forward + i.ms_adjoint
+        m, n, dv = r.timed_kernels()
+        dev_ms += dv
         for k in range(2):
             kms[k] += m[k]
             kn[k] += n[k]
     info = runs[0].plan.info()
+    batches = [len(b) for r in runs for b in r.batches]
     S = info.segment
     for r in runs:
         r.close()
@@ -639,7 +697,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded layered model, Ricker sources, observed data from a forward run)",
-        "config": {"workload": wl["name"], **wl["config"], "segment_used": S},
+        "config": {"workload": wl["name"], **wl["config"], "segment_used": S, "batches": batches},
         "roofline": {"bound": "hbm", "achieved": fam[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fam[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(wl["name"], kern[dom]),
@@ -651,20 +709,13 @@ def main():
     if not args.no_one_shot and len(wl["shots"]) > 1:
         # the same kernels on a plan holding one shot, shots run in sequence
         sample = wl.get("groups", [wl["shots"]])[0][:args.one_shot_sample]
-        one = Runner(wl, sample[:1])
+        one = Runner(wl, sample, max_batch=1)
         one.step()    # warm-up
-        descs = [one.ws.shot_desc(sh) for sh in sample]
         t0 = time.perf_counter()
-        for d, sh in zip(descs, sample):
-            one.descs = [d]
-            one.misfits = [sw.L2Misfit(sh.recs.seismograms.copy())]
-            one.ws.upload_shots(one.plan, [d])
-            one.ntr = int(np.prod(d["tshape"]))
-            one.traces = np.zeros(one.ntr, dtype=np.float32)
-            one.step()
+        one.step()
         el = time.perf_counter() - t0
         one.close()
-        v1 = 2.0 * cells * sum(d["nt"] for d in descs) / el / 1e9
+        v1 = 2.0 * cells * sum(d["nt"] for d in one.descs) / el / 1e9
         out["one_shot"] = {"value": round(v1, 3), "unit": "GCell-updates/s",
                            "sample": f"{len(sample)} shots run one at a time on a 1-shot plan",
                            "batched_over_one_shot": round(value / v1, 3)}

@@ -71,7 +71,7 @@ typedef struct swx_plan swx_plan;
 typedef struct {
   int32_t physics;       /* swx_physics                                     */
   int32_t dtype;         /* swx_dtype                                       */
-  int32_t ndim;          /* 2 or 3                                          */
+  int32_t ndim;          /* 1, 2 or 3 (1: acoustic only; elastic: 2)        */
   int32_t halo;          /* C-PML thickness in grid points (>= 0)           */
   int64_t n[3];          /* grid points per dimension, x first              */
   double h[3];           /* grid spacing per dimension                      */
@@ -154,9 +154,11 @@ typedef struct {
    * swx_adjoint call. */
   double ms_kernel[2];
   int64_t n_kernel[2];
-  /* Kernel variant the plan's autotuner chose for each family (VD: tile
-   * height 8/16/32 of the 2-D tiled kernels, 102/104 for the z-marching
-   * kernels on 128/256-column strips; 0 where there is one variant). */
+  /* Launch shapes the plan's autotuner chose (or the environment pinned):
+   * VD [forward, adjoint] tile codes (8/16/32 interior + other launches,
+   * 308/316/332 one launch, 102/104 z-marching strips); elastic [forward,
+   * adjoint] variant codes; CD 2D [rows per thread, 0]; CD 3D [launch shape
+   * 8/4/2/108/104/102, planes per z chunk]; 0 where there is one shape. */
   int32_t kernel_variant[2];
 } swx_plan_info;
 
@@ -167,7 +169,11 @@ int swx_abi_version(void);
 const char *swx_last_error(void);
 
 /* Replaces the allocation done by the WaveSimulation constructors
- * (acou_models.jl:90-201, :333-451) for `max_shots` shots at once. */
+ * (acou_models.jl:90-201, :333-451) for `max_shots` shots at once.  The
+ * time stepping of the plan replaces the backends' forward_onestep_CPML! /
+ * adjoint_onestep_CPML!: acoustic1D_xPU.jl, acoustic1D_VD_xPU.jl (ndim 1),
+ * acoustic2D_xPU.jl, acoustic2D_VD_xPU.jl, acoustic3D_xPU.jl and
+ * elastic2D_iso_xPU.jl (src/models/{acoustic,elastic}/backends/shared/). */
 int swx_plan_create(const swx_plan_desc *desc, swx_plan **plan);
 
 int swx_plan_destroy(swx_plan *plan);
@@ -179,6 +185,7 @@ int swx_plan_destroy(swx_plan *plan);
  *   SWX_ACOUSTIC_VD: fields[0] = fact_m0 = vp^2 rho dt       (n)
  *                    fields[1] = fact_m1_stag[1] (x)         (nx-1, nz)
  *                    fields[2] = fact_m1_stag[2] (z)         (nx, nz-1)
+ *                    (1D: fields[0] (nx), fields[1] (nx-1))
  *   SWX_ELASTIC_ISO: fields[0] = lambda, fields[1] = mu      (n)
  *                    fields[2] = rho_ihalf                   (nx-1, nz)
  *                    fields[3] = rho_jhalf                   (nx, nz-1)
@@ -214,6 +221,7 @@ int swx_forward(swx_plan *plan, void *traces, int save_state);
  *   per_shot != 0 (then nshots consecutive copies):
  *   SWX_ACOUSTIC_CD: grad_vp (n)
  *   SWX_ACOUSTIC_VD: grad_m0 (n), grad_m1_stag x (nx-1,nz), z (nx,nz-1)
+ *                    (1D: grad_m0 (nx), grad_m1_stag x (nx-1))
  *   SWX_ELASTIC_ISO: grad_rho_ihalf (nx-1,nz), grad_rho_jhalf (nx,nz-1),
  *                    grad_lambda (n), grad_mu (n),
  *                    grad_mu_ihalf_jhalf (nx-1,nz-1) */
@@ -240,6 +248,41 @@ int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot);
  * array, which keeps the last call's contents: calls that overlap on one
  * GPU (from several host threads) race. */
 int swx_forward_noalloc(const swx_plan_desc *desc, const void *fact, const swx_shot_desc *shot, void *traces);
+
+/* Snapshots of the following swx_forward calls of the current batch
+ * (LinearSnapshotter, src/utils/snapshotter.jl:1-41, runparams.snapevery;
+ * taken right after every snapevery-th step as at acou_forward.jl:55-57 and
+ * :116-120, ela_forward.jl:74-77).  snaps is host memory of `bytes`; for
+ * each shot in batch order, for k = 1 .. floor(nt_s / snapevery), the
+ * fields of step k * snapevery follow each other at their Julia sizes:
+ *   SWX_ACOUSTIC_CD: "pcur" (n)
+ *   SWX_ACOUSTIC_VD: "pcur" (nx, nz), "vcur" vx (nx-1, nz), vz (nx, nz-1)
+ *                    (1D: "pcur" (nx), "vcur" vx (nx-1))
+ *   SWX_ELASTIC_ISO: "ucur" ux (nx-1, nz), uz (nx, nz-1),
+ *                    "σ" σxx (nx, nz), σzz (nx, nz), σxz (nx-1, nz-1)
+ * snapevery = 0 turns snapshots off; swx_set_shots* also does.  snapevery
+ * must be smaller than every shot's nt (snapshotter.jl:12). */
+int swx_set_snapshots(swx_plan *plan, int snapevery, void *snaps, int64_t bytes);
+
+/* Observed data of an L2 misfit for the batch last set (L2Misfit.jl:2-20):
+ * observed is a host T array in the layout of swx_forward's traces (per
+ * shot nt_s x nrec_s, elastic nt_s x 2 x nrec_s, concatenated).  weights is
+ * NULL (identity inverse covariance, no windows) or a host double array of
+ * the same layout holding window mask x diagonal of the inverse covariance
+ * for every sample.  Kept on the device until the next swx_set_shots*. */
+int swx_set_observed(swx_plan *plan, const void *observed, const double *weights);
+
+/* calcmisfit of L2Misfit (L2Misfit.jl:24-62) for every shot of the batch,
+ * from the traces of the last swx_forward and the observed data on the
+ * device: misfit[s] = 1/2 sum w (syn - obs)^2, summed in double. */
+int swx_misfit_l2(swx_plan *plan, double *misfit);
+
+/* swx_adjoint with the adjoint source built on the device: -dchi/du =
+ * -(syn - obs) * w (L2Misfit.jl:64-95, acou_gradient.jl:45-51), formed in
+ * double and rounded to T, so bitwise the source the host would pass to
+ * swx_adjoint; no trace or residual crosses PCIe.  misfit (may be NULL)
+ * receives the per-shot misfits as from swx_misfit_l2. */
+int swx_adjoint_l2(swx_plan *plan, void *grad, int per_shot, double *misfit);
 
 /* Sizes, segment length and device timings of the last calls. */
 int swx_plan_get_info(swx_plan *plan, swx_plan_info *info);

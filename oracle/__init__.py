@@ -45,6 +45,30 @@ def lib():
     return _LIB
 
 
+def with_snapshots(every, nt, shapes, dtype, run):
+    """Runs run() -- one oracle forward loop of nt steps -- with the snapshot
+    sink of swx_oracle.c on (LinearSnapshotter, snapshotter.jl:1-41): after
+    every step it with it % every == 0 the fields of `shapes` (the
+    reference's snapshotted fields in its order) are kept.  Returns
+    {it: [arrays]}."""
+    count = nt // every
+    sizes = [int(np.prod(sh)) for sh in shapes]
+    buf = np.zeros(max(count * sum(sizes), 1), dtype=dtype)
+    lib().oracle_set_snapshots(ctypes.c_int(every), ctypes.c_void_p(buf.ctypes.data))
+    try:
+        run()
+    finally:
+        lib().oracle_set_snapshots(ctypes.c_int(0), ctypes.c_void_p(0))
+    out, off = {}, 0
+    for k in range(1, count + 1):
+        arrs = []
+        for sh, n in zip(shapes, sizes):
+            arrs.append(buf[off:off + n].reshape(sh, order="F").copy())
+            off += n
+        out[k * every] = arrs
+    return out
+
+
 # ---------------------------------------------------------------------------
 # ctypes structures mirroring swx_oracle.c
 # ---------------------------------------------------------------------------
@@ -855,3 +879,179 @@ def elastic_gradient(lam, mu, rho, spacing, dt, nt, halo, rcoef, freetop, shots,
         misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
         seis.append(tr)
     return tot, misfit, seis
+
+
+# ---------------------------------------------------------------------------
+# 1D acoustic, constant and variable density
+# ---------------------------------------------------------------------------
+def cd1d_forward(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, dtype=np.float64):
+    """swforward_1shot! for CD 1D (acoustic1D_xPU.jl) on prepared inputs;
+    src_idx/rec_idx 1-based (n, 1).  Returns traces (nt, nrec)."""
+    fact = _f_order(fact, dtype)
+    g = _geom(fact.shape, spacing, dt, halo)
+    cx, keep = _cpml_struct(cpml[0], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    rc = _fn("oracle_cd1d_forward", dtype)(
+        ctypes.byref(g), _ptr(fact), ctypes.byref(cx), ctypes.c_int(nt), ctypes.c_int(src.shape[0]), _ptr(src),
+        _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(traces))
+    assert rc == 0
+    return np.ascontiguousarray(traces)
+
+
+def cd1d_gradient(fact, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, adjsrc, check_freq, dtype=np.float64):
+    """Raw grad_vp (before muting and 2/vp^3) and traces of swgradient_1shot!
+    for CD 1D."""
+    fact = _f_order(fact, dtype)
+    g = _geom(fact.shape, spacing, dt, halo)
+    cx, keep = _cpml_struct(cpml[0], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    grad = np.zeros(fact.shape, dtype=dtype)
+    rc = _fn("oracle_cd1d_gradient", dtype)(
+        ctypes.byref(g), _ptr(fact), ctypes.byref(cx), ctypes.c_int(nt), ctypes.c_int(src.shape[0]), _ptr(src),
+        _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(adj), ctypes.c_int(check_freq), _ptr(grad),
+        _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return grad, np.ascontiguousarray(traces)
+
+
+def vd1d_forward(m0, m1, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, dtype=np.float64):
+    m0 = _f_order(m0, dtype)
+    m1 = _f_order(m1, dtype)
+    g = _geom(m0.shape, spacing, dt, halo)
+    cx, keep = _cpml_struct(cpml[0], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    rc = _fn("oracle_vd1d_forward", dtype)(
+        ctypes.byref(g), _ptr(m0), _ptr(m1), ctypes.byref(cx), ctypes.c_int(nt), ctypes.c_int(src.shape[0]),
+        _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(traces))
+    assert rc == 0
+    return np.ascontiguousarray(traces)
+
+
+def vd1d_gradient(m0, m1, spacing, dt, halo, cpml, nt, src_idx, srctf, rec_idx, adjsrc, check_freq,
+                  dtype=np.float64):
+    """Raw grad_m0, grad_m1_stag and traces of swgradient_1shot! (VD 1D)."""
+    m0 = _f_order(m0, dtype)
+    m1 = _f_order(m1, dtype)
+    nx = m0.shape[0]
+    g = _geom(m0.shape, spacing, dt, halo)
+    cx, keep = _cpml_struct(cpml[0], dtype)
+    src = np.asfortranarray(src_idx, dtype=np.int64)
+    rec = np.asfortranarray(rec_idx, dtype=np.int64)
+    stf = _f_order(srctf, dtype)
+    adj = _f_order(adjsrc, dtype)
+    traces = np.zeros((nt, rec.shape[0]), dtype=dtype, order="F")
+    g0 = np.zeros(nx, dtype=dtype)
+    g1 = np.zeros(nx - 1, dtype=dtype)
+    rc = _fn("oracle_vd1d_gradient", dtype)(
+        ctypes.byref(g), _ptr(m0), _ptr(m1), ctypes.byref(cx), ctypes.c_int(nt), ctypes.c_int(src.shape[0]),
+        _ptr(src), _ptr(stf), ctypes.c_int(rec.shape[0]), _ptr(rec), _ptr(adj), ctypes.c_int(check_freq),
+        _ptr(g0), _ptr(g1), _ptr(traces))
+    if rc != 0:
+        raise ValueError("Checkpointing frequency must be smaller than the number of timesteps!")
+    return (g0, g1), np.ascontiguousarray(traces)
+
+
+def _shot_1d(vp, spacing, dt, halo, rcoef, freetop, sh, T):
+    cpml = init_bdc(vp, dt, spacing, halo, rcoef, freetop, sh["f0"], T)
+    si = find_nearest_grid_points(sh["src_pos"], spacing)
+    ri = find_nearest_grid_points(sh["rec_pos"], spacing)
+    return cpml, si, ri
+
+
+def acoustic_cd1d_forward(vp, spacing, dt, nt, halo, rcoef, freetop, shots, dtype=np.float64):
+    """run_swforward! for CD 1D; shots as acoustic_cd_forward (positions (n, 1))."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2
+    out = []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml, si, ri = _shot_1d(vp, spacing, dt, halo, rcoef, freetop, sh, T)
+        stf = scale_srctf_cd(sh["srctf"][:nt_s], spacing, dt, vp, si, T)
+        out.append(cd1d_forward(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, T))
+    return out
+
+
+def acoustic_cd1d_gradient(vp, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1,
+                           mute_radius_src=0, mute_radius_rec=0, dtype=np.float64):
+    """run_swgradient! for CD 1D with L2 misfits; returns ({"vp"}, misfit, seismograms)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    fact = (T(dt) ** 2) * vp ** 2
+    tot = np.zeros(vp.shape, dtype=T)
+    misfit, seis = 0.0, []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml, si, ri = _shot_1d(vp, spacing, dt, halo, rcoef, freetop, sh, T)
+        stf = scale_srctf_cd(sh["srctf"][:nt_s], spacing, dt, vp, si, T)
+        syn = cd1d_forward(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
+        g, tr = cd1d_gradient(fact, spacing, dt, halo, cpml, nt_s, si, stf, ri, adj.astype(T),
+                              min(check_freq, nt_s - 1), T)
+        g = mute_around_points(g, sh["src_pos"], spacing, mute_radius_src)
+        g = mute_around_points(g, sh["rec_pos"], spacing, mute_radius_rec)
+        tot += (T(2.0) / vp ** 3) * g
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return {"vp": tot}, misfit, seis
+
+
+def acoustic_vd1d_forward(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, method="arithmetic",
+                          dtype=np.float64):
+    """run_swforward! for VD 1D."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    rho = np.asarray(rho, dtype=T)
+    m0 = vp ** 2 * rho * T(dt)
+    m1 = (interp_stag(T(1) / rho, 1, method) * T(dt)).astype(T)
+    out = []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml, si, ri = _shot_1d(vp, spacing, dt, halo, rcoef, freetop, sh, T)
+        stf = scale_srctf_vd(sh["srctf"][:nt_s], spacing, dt, vp, rho, si, T)
+        out.append(vd1d_forward(m0, m1, spacing, dt, halo, cpml, nt_s, si, stf, ri, T))
+    return out
+
+
+def acoustic_vd1d_gradient(vp, rho, spacing, dt, nt, halo, rcoef, freetop, shots, check_freq=1,
+                           mute_radius_src=0, mute_radius_rec=0, method="arithmetic", dtype=np.float64):
+    """run_swgradient! for VD 1D (acou_gradient.jl:97-203); returns
+    ({"vp", "rho"}, misfit, seismograms)."""
+    T = dtype
+    vp = np.asarray(vp, dtype=T)
+    rho = np.asarray(rho, dtype=T)
+    m0 = vp ** 2 * rho * T(dt)
+    m1 = (interp_stag(T(1) / rho, 1, method) * T(dt)).astype(T)
+    gvp = np.zeros(vp.shape, dtype=T)
+    grho = np.zeros(vp.shape, dtype=T)
+    misfit, seis = 0.0, []
+    for sh in shots:
+        nt_s = sh.get("nt", nt)
+        cpml, si, ri = _shot_1d(vp, spacing, dt, halo, rcoef, freetop, sh, T)
+        stf = scale_srctf_vd(sh["srctf"][:nt_s], spacing, dt, vp, rho, si, T)
+        syn = vd1d_forward(m0, m1, spacing, dt, halo, cpml, nt_s, si, stf, ri, T)
+        adj = -l2_dchi_du(syn, sh["observed"], sh.get("invcov"))
+        (g0, g1), tr = vd1d_gradient(m0, m1, spacing, dt, halo, cpml, nt_s, si, stf, ri, adj.astype(T),
+                                     min(check_freq, nt_s - 1), T)
+        g1 = back_interp(1.0 / rho.astype(np.float64), g1, 1, method)
+        g0 = g0.astype(np.float64)
+        g0 = mute_around_points(mute_around_points(g0, sh["src_pos"], spacing, mute_radius_src),
+                                sh["rec_pos"], spacing, mute_radius_rec)
+        g1 = mute_around_points(mute_around_points(g1, sh["src_pos"], spacing, mute_radius_src),
+                                sh["rec_pos"], spacing, mute_radius_rec)
+        gvp += (-2.0 * g0 / (vp ** 3 * rho)).astype(T)
+        grho += (-g0 / (vp ** 2 * rho ** 2) - g1 / rho).astype(T)
+        misfit += l2_misfit(tr, sh["observed"], sh.get("invcov"))
+        seis.append(tr)
+    return {"vp": gvp, "rho": grho}, misfit, seis

@@ -42,7 +42,10 @@ static void SFX(cd3d_rotate)(SFX(cd3d_state) *s) {
 }
 
 /* forward_onestep_CPML! of acoustic3D_xPU.jl (update_psi_x/y/z!, then
- * update_p_CPML!, inject_sources!, record_receivers!) */
+ * update_p_CPML!, inject_sources!, record_receivers!).  As in 2D, the
+ * reference's rotation (acoustic3D_xPU.jl:155-157) binds pnew to the array of
+ * the new pold, so cells on the six outer faces keep level it-2 plus any
+ * injection: their pold values are copied into pnew before the injection. */
 static void SFX(cd3d_onestep)(const oracle_geom *g, const real *fact, SFX(cd3d_state) *st,
                               const SFX(cpml) *c, int nsrc, const int64_t *src, const real *srctf, int nt,
                               int it, int nrec, const int64_t *rec, real *traces) {
@@ -113,6 +116,13 @@ static void SFX(cd3d_onestep)(const oracle_geom *g, const real *fact, SFX(cd3d_s
         }
         A3(pnew, nx, ny, i, j, k) = 2 * pc - A3(pold, nx, ny, i, j, k) + A3(fact, nx, ny, i, j, k) * lap;
       }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 1; k <= nz; k++)
+    for (int64_t j = 1; j <= ny; j++) {
+      const int face = (k == 1 || k == nz || j == 1 || j == ny);
+      for (int64_t i = 1; i <= nx; i += (face || i == nx) ? 1 : nx - 1)
+        A3(pnew, nx, ny, i, j, k) = A3(pold, nx, ny, i, j, k);
+    }
   for (int s = 0; s < nsrc; s++)
     A3(pnew, nx, ny, src[s], src[s + nsrc], src[s + 2 * nsrc]) += srctf[(it - 1) + (size_t)nt * s];
   if (traces)
@@ -127,6 +137,7 @@ int SFX(oracle_cd3d_forward)(const oracle_geom *g, const real *fact, const SFX(c
   for (int it = 1; it <= nt; it++) {
     SFX(cd3d_onestep)(g, fact, &s, c, nsrc, src, srctf, nt, it, nrec, rec, traces);
     SFX(cd3d_rotate)(&s);
+    if (snap_due(it)) snap_put(s.p[1], (size_t)g->n[0] * g->n[1] * g->n[2] * sizeof(real));   /* "pcur" */
     oracle_progress("cd3d forward", it, nt);
   }
   SFX(cd3d_state_free)(&s);

@@ -22,6 +22,18 @@
 
 #define EL_A(arr, n1, i, j) (arr)[((i) - 1) + (size_t)(n1) * ((j) - 1)]
 
+/* ORACLE_PLANT (tests only, oracle/Makefile target `planted`) builds a copy
+ * of the oracle with ONE free-surface mirror sign flipped, to show that the
+ * half-space check of the free surface fails on such a mistake:
+ *   1: the sigma_zz mirror of d_szz_dz at j = 1
+ *   2: the sigma_xz mirror of d_sxz_dz at j = 2
+ *   3: the ux mirror of d_ux_dz at j = 1
+ *   4: the uz mirror of d_uz_dz at j = 2                                    */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+#define EL_FLIP(k) (ORACLE_PLANT == (k) ? -1 : 1)
+
 static inline real SFX(el_inner)(real f1, real f2, real f3, real f4, real d) {
   /* ∂x4th_inner (freesurface_derivatives_4th_mirror.jl:2-7) */
   return ((real)(1.0 / 24.0) * f1 - (real)(27.0 / 24.0) * f2 + (real)(27.0 / 24.0) * f3 - (real)(1.0 / 24.0) * f4) * d;
@@ -46,7 +58,7 @@ static real SFX(d_szz_dz)(const SFX(el_geom) *g, const real *s, int64_t i, int64
   const int64_t nx = g->nx, nz = g->nz;
 #define S(b) EL_A(s, nx, i, (b))
   if (j == 1) {
-    if (g->freetop) return SFX(el_inner)(-S(j + 1), S(j), S(j + 1), S(j + 2), g->dz_);
+    if (g->freetop) return SFX(el_inner)(-EL_FLIP(1) * S(j + 1), S(j), S(j + 1), S(j + 2), g->dz_);
     return SFX(el_inner)(0, S(j), S(j + 1), S(j + 2), g->dz_);
   }
   if (j == nz - 1) return SFX(el_inner)(S(j - 1), S(j), S(j + 1), 0, g->dz_);
@@ -71,7 +83,7 @@ static real SFX(d_sxz_dz)(const SFX(el_geom) *g, const real *s, int64_t i, int64
     return SFX(el_inner)(0, 0, S(j), S(j + 1), g->dz_);
   }
   if (j == 2) {
-    if (g->freetop) return SFX(el_inner)(-S(j - 1), S(j - 1), S(j), S(j + 1), g->dz_);
+    if (g->freetop) return SFX(el_inner)(-EL_FLIP(2) * S(j - 1), S(j - 1), S(j), S(j + 1), g->dz_);
     return SFX(el_inner)(0, S(j - 1), S(j), S(j + 1), g->dz_);
   }
   if (j == nz - 1) return SFX(el_inner)(S(j - 2), S(j - 1), S(j), 0, g->dz_);
@@ -102,7 +114,7 @@ static real SFX(d_uz_dz)(const SFX(el_geom) *g, const real *ux, const real *uz, 
     return SFX(el_inner)(0, 0, U(j), U(j + 1), g->dz_);
   }
   if (j == 2) {
-    if (g->freetop) return SFX(el_inner)(U(j - 1), U(j - 1), U(j), U(j + 1), g->dz_);
+    if (g->freetop) return SFX(el_inner)(EL_FLIP(4) * U(j - 1), U(j - 1), U(j), U(j + 1), g->dz_);
     return SFX(el_inner)(0, U(j - 1), U(j), U(j + 1), g->dz_);
   }
   if (j == nz - 1) return SFX(el_inner)(U(j - 2), U(j - 1), U(j), 0, g->dz_);
@@ -114,7 +126,7 @@ static real SFX(d_ux_dz)(const SFX(el_geom) *g, const real *u, int64_t i, int64_
   const int64_t nz = g->nz, n1 = g->nx - 1;
 #define U(b) EL_A(u, n1, i, (b))
   if (j == 1) {
-    if (g->freetop) return SFX(el_inner)(U(j + 1), U(j), U(j + 1), U(j + 2), g->dz_);
+    if (g->freetop) return SFX(el_inner)(EL_FLIP(3) * U(j + 1), U(j), U(j + 1), U(j + 2), g->dz_);
     return SFX(el_inner)(0, U(j), U(j + 1), U(j + 2), g->dz_);
   }
   if (j == nz - 1) return SFX(el_inner)(U(j - 1), U(j), U(j + 1), 0, g->dz_);
@@ -333,6 +345,14 @@ int SFX(oracle_el2d_forward)(const oracle_geom *og, const real *lam, const real 
   SFX(el_state_alloc)(&g, &s);
   for (int it = 1; it <= nt; it++) {
     SFX(el_fwd_step)(&g, &m, &s, kind, s1, s2, M, tf, nt, it, rx, rz, traces);
+    if (snap_due(it)) {   /* "ucur" = [ux, uz], "σ" = [σxx, σzz, σxz] (ela_forward.jl:74-77) */
+      const int64_t nx = g.nx, nz = g.nz;
+      snap_put(s.ux[1], (size_t)(nx - 1) * nz * sizeof(real));
+      snap_put(s.uz[1], (size_t)nx * (nz - 1) * sizeof(real));
+      snap_put(s.sxx, (size_t)nx * nz * sizeof(real));
+      snap_put(s.szz, (size_t)nx * nz * sizeof(real));
+      snap_put(s.sxz, (size_t)(nx - 1) * (nz - 1) * sizeof(real));
+    }
     oracle_progress("elastic forward", it, nt);
   }
   SFX(el_state_free)(&s);

@@ -26,7 +26,12 @@
  * semantics of src/utils/fdgen.jl:137-193 (order 2: first derivative
  * (A[i+1]-A[i])/h on the half grid, second derivative (A[i-1]-2A[i]+A[i+1])/h^2;
  * psi <- b_h psi + a_h dA in the strips, xi <- b xi + a (d2A + dpsi)).
- * The caller rotates pold/pcur/pnew (acoustic2D_xPU.jl:122-124).            */
+ * The caller rotates pold/pcur/pnew (acoustic2D_xPU.jl:122-124).  Those three
+ * sequential assignments leave "pnew" bound to the same array as the new
+ * "pold", so the reference runs on two arrays and writes pnew over pold in
+ * place: a cell outside the stencil range (the outermost rows and columns)
+ * keeps the value of level it-2 plus what is injected there.  With three
+ * arrays here, the edge cells of pold are copied into pnew to match.       */
 static void SFX(cd2d_onestep)(const oracle_geom *g, const real *fact,
                               const real *pold, const real *pcur, real *pnew,
                               real *psi_x, real *psi_z, real *xi_x, real *xi_z,
@@ -105,6 +110,15 @@ static void SFX(cd2d_onestep)(const oracle_geom *g, const real *fact,
     }
   }
   (void)nxz;
+  /* edge cells: pnew is pold's array in the reference (see above) */
+  for (int64_t i = 1; i <= nx; i++) {
+    A2(pnew, nx, i, 1) = A2(pold, nx, i, 1);
+    A2(pnew, nx, i, nz) = A2(pold, nx, i, nz);
+  }
+  for (int64_t j = 2; j <= nz - 1; j++) {
+    A2(pnew, nx, 1, j) = A2(pold, nx, 1, j);
+    A2(pnew, nx, nx, j) = A2(pold, nx, nx, j);
+  }
   /* inject_sources! (acoustic2D_xPU.jl:48-54), serial order */
   for (int s = 0; s < nsrc; s++) {
     int64_t is = src[s], js = src[s + nsrc];
@@ -163,6 +177,7 @@ int SFX(oracle_cd2d_forward)(const oracle_geom *g, const real *fact,
     SFX(cd2d_onestep)(g, fact, s.p[0], s.p[1], s.p[2], s.psi_x, s.psi_z, s.xi_x, s.xi_z,
                       cx, cz, nsrc, src, srctf, nt, it, nrec, rec, traces);
     SFX(cd2d_rotate)(&s);
+    if (snap_due(it)) snap_put(s.p[1], (size_t)g->n[0] * g->n[1] * sizeof(real));   /* "pcur" */
     oracle_progress("cd2d forward", it, nt);
   }
   SFX(cd2d_state_free)(&s);
@@ -509,6 +524,12 @@ int SFX(oracle_vd2d_forward)(const oracle_geom *g, const real *m0, const real *m
   SFX(vd2d_state_alloc)(g, &s);
   for (int it = 1; it <= nt; it++) {
     SFX(vd2d_fwd_step)(g, m0, m1x, m1z, &s, cx, cz, nsrc, src, srctf, nt, it, nrec, rec, traces);
+    if (snap_due(it)) {   /* "pcur", then "vcur" = [vx, vz] (acou_forward.jl:116-120) */
+      const int64_t nx = g->n[0], nz = g->n[1];
+      snap_put(s.p, (size_t)nx * nz * sizeof(real));
+      snap_put(s.vx, (size_t)(nx - 1) * nz * sizeof(real));
+      snap_put(s.vz, (size_t)nx * (nz - 1) * sizeof(real));
+    }
     oracle_progress("vd2d forward", it, nt);
   }
   SFX(vd2d_state_free)(&s);

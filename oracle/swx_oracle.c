@@ -23,6 +23,27 @@ static void oracle_progress(const char *phase, int done, int total) {
   }
 }
 
+/* Snapshot sink of the forward loops (LinearSnapshotter, snapshotter.jl:
+ * 1-41): with every > 0, after each step it with it % every == 0 the
+ * snapshotted fields are appended to buf, in the order the reference's
+ * snapshotter holds them.  Set by oracle_set_snapshots; 0 turns it off. */
+static int g_snap_every = 0;
+static char *g_snap_buf = NULL;
+static size_t g_snap_off = 0;
+
+void oracle_set_snapshots(int every, void *buf) {
+  g_snap_every = every;
+  g_snap_buf = (char *)buf;
+  g_snap_off = 0;
+}
+
+static int snap_due(int it) { return g_snap_every > 0 && g_snap_buf && it % g_snap_every == 0; }
+
+static void snap_put(const void *src, size_t bytes) {
+  memcpy(g_snap_buf + g_snap_off, src, bytes);
+  g_snap_off += bytes;
+}
+
 typedef struct {
   int64_t n[3];   /* grid size, x (fast axis) first                         */
   double h[3];    /* grid spacing                                           */
@@ -37,9 +58,11 @@ typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
 #include "oracle_cd3d.h"
 #include "oracle_el2d.h"
+#include "oracle_1d.h"
 #undef real
 #undef SFX
 #undef A3
+#undef A1
 
 #define real float
 #define SFX(x) x##_f32
@@ -47,6 +70,7 @@ typedef struct { const real *a, *a_h, *b, *b_h; } SFX(cpml);
 #include "oracle_impl.h"
 #include "oracle_cd3d.h"
 #include "oracle_el2d.h"
+#include "oracle_1d.h"
 #undef real
 #undef SFX
 

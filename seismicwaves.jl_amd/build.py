@@ -1,6 +1,11 @@
 """Builds libswx.so (HIP, gfx950) in-tree from csrc/*.hip.
 
 Usage: python seismicwaves.jl_amd/build.py [--debug]
+
+--debug builds seismicwaves/libswx_debug.so instead: -O1 -g and SWX_DEBUG,
+which turns on device-side asserts on the indices of the shared accessors
+(point-table rows, C-PML coefficient vectors, the elastic field accessor).
+Load it with SWX_LIBRARY=<path> (seismicwaves/_native.py).
 """
 from __future__ import annotations
 
@@ -13,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "seismicwaves", "libswx.so")
+OUT_DEBUG = os.path.join(HERE, "seismicwaves", "libswx_debug.so")
 ARCH = "gfx950"
 
 # -ffp-contract=on fuses a*b+c into an FMA only inside one source
@@ -28,16 +34,18 @@ HIPFLAGS = [
 ]
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
+def build(verbose: bool = False, jobs: int = 4, debug: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    objdir = os.path.join(HERE, "_build")
+    objdir = os.path.join(HERE, "_build_debug" if debug else "_build")
+    flags = [f if f != "-O3" else "-O1" for f in HIPFLAGS] + ["-g", "-DSWX_DEBUG"] if debug else HIPFLAGS
+    out = OUT_DEBUG if debug else OUT
     os.makedirs(objdir, exist_ok=True)
     procs, objs = [], []
     for src in srcs:
         obj = os.path.join(objdir, os.path.basename(src) + ".o")
         objs.append(obj)
-        cmd = [hipcc, *HIPFLAGS, "-c", src, "-o", obj]
+        cmd = [hipcc, *flags, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -45,13 +53,13 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
             _wait(procs.pop(0))
     for p in procs:
         _wait(p)
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
-        raise RuntimeError("link of libswx.so failed:\n" + r.stdout)
-    os.replace(tmp, OUT)
-    return OUT
+        raise RuntimeError(f"link of {os.path.basename(out)} failed:\n" + r.stdout)
+    os.replace(tmp, out)
+    return out
 
 
 def _wait(item):
@@ -64,4 +72,4 @@ def _wait(item):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, debug="--debug" in sys.argv))

@@ -210,41 +210,37 @@ __device__ __forceinline__ void cd2d_cell(const Cd2dStepArgs<T> &a, int s, int n
     flap = fa * (lx + lz);
     pnv = T(2) * pc - pov + flap;
   }
+  // Edge cells are outside the stencil range.  The reference's rotation
+  // (acoustic2D_xPU.jl:122-124) binds pnew to the array of the new pold, so
+  // an edge cell of level it holds level it-2 plus what is injected there.
+  if (!interior) pnv = pov;
   // Points of this cell: inject_sources! / record_receivers! (forward,
   // acoustic2D_xPU.jl:48-62) or the prescaled residual injection of the
   // adjoint (:64-76, :161-162), in the reference's serial order per cell.
-  // Edge cells are not updated by the stencil: their value is the stale
-  // time level plus any injection, as in the reference's buffer rotation.
-  bool wrote = interior;
   if (a.res == nullptr) {
     const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c);
     if (u >= 0) {
-      if (!interior) pnv = pn_[c];
       const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
       for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) {
         const T v = tf[(int64_t)nts * a.pt.usrc_list[k]];
         pnv += v;
         inj += v;
       }
-      wrote = true;
     }
   } else {
     const int u = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
     if (u >= 0) {
-      if (!interior) pnv = pn_[c];
       const T *rs = a.res + a.pt.off_tr[s] + (a.it - 1);
       const T f = fa;
       for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++) pnv += rs[(int64_t)nts * a.pt.urec_list[k]] * f;
-      wrote = true;
     }
   }
-  if (wrote) pn_[c] = pnv;
+  pn_[c] = pnv;
   if (a.traces) {
     const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
     if (r >= 0) {
-      const T v = wrote ? pnv : pn_[c];
       T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
-      for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = v;
+      for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = pnv;
     }
   }
   if (a.dmode) {
@@ -256,10 +252,9 @@ __device__ __forceinline__ void cd2d_cell(const Cd2dStepArgs<T> &a, int s, int n
       slot = a.it - a.cbase - 1;
     }
     if (slot >= 0) {
-      const T pn = wrote ? pnv : pn_[c];
       // p^{it} - 2 p^{it-1} + p^{it-2}: in the interior exactly the stencil
       // term plus the injection, formed without cancellation
-      a.dbuf[(int64_t)slot * a.dslot + so + c] = interior ? flap + inj : (pov - T(2) * pc) + pn;
+      a.dbuf[(int64_t)slot * a.dslot + so + c] = interior ? flap + inj : (pov - T(2) * pc) + pnv;
     }
   }
   if (corr) {
@@ -318,6 +313,7 @@ class Cd2dEngine final : public Engine {
       const int r = std::atoi(v);
       SWX_REQUIRE(r == 1 || r == 2 || r == 4, SWX_EINVAL, "SWX_CD2D_ROWS must be 1, 2 or 4");
       rows_ = r;
+      tuned_ = true;
     }
     if (const char *v = std::getenv("SWX_CD2D_LEAN")) lean_ = std::atoi(v) != 0;
     const size_t e = sizeof(T);
@@ -386,14 +382,22 @@ class Cd2dEngine final : public Engine {
     SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
+    if (!tuned_) tune_rows(save, tr);
     for (int it = 1; it <= ntmax_; it++) {
       progress("forward step", it, ntmax_);
       step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
+      take_snapshots(c_, *this, it);
       if (save && it % S_ == 0) {
         const int k = it / S_;
         if (k >= 1 && k <= K_ - 2) save_ckpt(k, it);
       }
     }
+  }
+
+  // "pcur" (acou_forward.jl:55-57): level it is in slot it mod 3
+  int snap_fields(int it, int s, SnapPiece *out) const override {
+    out[0] = {P(it) + (int64_t)s * N_, (int64_t)(N_ * sizeof(T)), (int64_t)(N_ * sizeof(T)), 1};
+    return 1;
   }
 
   void adjoint(bool per_shot) override {
@@ -444,6 +448,7 @@ class Cd2dEngine final : public Engine {
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return dbuf_.bytes(); }
   int segment() const override { return S_; }
+  void variants(int32_t v[2]) const override { v[0] = rows_; v[1] = 0; }
 
  private:
   T *P(int k) const { return p_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
@@ -493,7 +498,25 @@ class Cd2dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
 
-  void step_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
+  // Rows per thread (1, 2 or 4) timed on step 1 of the zero initial state,
+  // which every repetition recomputes identically (all stencil inputs are
+  // zero; the new level is the injection alone).  Every choice gives
+  // bitwise the same results.
+  void tune_rows(bool save, T *tr) {
+    static const int rows[3] = {2, 1, 4};
+    Cd2dStepArgs<T> a = fwd_args(1, 0, save ? 1 : 0, 0, tr);
+    a.lean = lean_ ? 1 : 0;
+    const int k = fastest_candidate(c_.stream, 3, [&](int c) {
+      if (rows[c] == 4) launch_rows<4>(a);
+      else if (rows[c] == 1) launch_rows<1>(a);
+      else launch_rows<2>(a);
+      SWX_HIP(hipGetLastError());
+    });
+    rows_ = rows[k];
+    tuned_ = true;
+  }
+
+  Cd2dStepArgs<T> fwd_args(int it, int act_min, int dmode, int cbase, T *traces) const {
     Cd2dStepArgs<T> a = base_args(it);
     a.pold = P(it - 2); a.pcur = P(it - 1); a.pnew = P(it);
     a.psix_o = psix_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spx_;
@@ -506,7 +529,11 @@ class Cd2dEngine final : public Engine {
     a.cbase = cbase;
     a.corr = 0;
     a.traces = traces;
-    launch(a);
+    return a;
+  }
+
+  void step_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
+    launch(fwd_args(it, act_min, dmode, cbase, traces));
   }
 
   void step_adj(int it, int cbase) {
@@ -566,6 +593,7 @@ class Cd2dEngine final : public Engine {
   int64_t N_, spx_, spz_, sxx_, sxz_;
   int cstride_;
   int rows_ = 2;
+  bool tuned_ = false;   // rows_ chosen (by SWX_CD2D_ROWS or tune_rows)
   bool lean_ = true;
   DevBuf fact_, p_, psix_, psiz_, xix_, xiz_, cpml_;
   DevBuf a_, apsix_, apsiz_, axix_, axiz_, grad_, gout_;

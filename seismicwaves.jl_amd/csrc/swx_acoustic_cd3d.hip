@@ -198,12 +198,14 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
       }
       flap = cur.fa * (lx + ly + lz);
       pnv = T(2) * pc - cur.po + flap;
-      pn_[c] = pnv;
+    } else {
+      // outer face: pnew is pold's array in the reference
+      pnv = cur.po;
     }
+    pn_[c] = pnv;
     if (dslot >= 0) {
       const T inj = 0;
-      const T pn = interior ? pnv : pn_[c];
-      a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pn;
+      a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pnv;
     }
     if (corr) a.grad[so + c] = cur.g + pc * cur.d;
     psz_prev = psz;
@@ -400,12 +402,14 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
         flap = cur.fa * (lx + ly + lz);
         pnv = T(2) * pc - cur.po + flap;
       }
-      bool wrote = interior;
+      // Cells on the outer faces are outside the stencil range.  The
+      // reference's rotation (acoustic3D_xPU.jl:155-157) binds pnew to the
+      // array of the new pold, so they hold level it-2 plus any injection.
+      if (!interior) pnv = cur.po;
       if (any_pt) {
         const int64_t row = j + (int64_t)ny * k;
         const int u = find_point(rowp, rcell, a.pt.nrows, s, row, c);
         if (u >= 0) {
-          if (!interior) pnv = pn_[c];
           if (a.res == nullptr) {
             const T *tf = a.srctf + a.pt.off_tf[s] + (a.it - 1);
             for (int q = a.pt.usrc_ptr[u]; q < a.pt.usrc_ptr[u + 1]; q++) {
@@ -418,23 +422,20 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
             const T f = cur.fa;
             for (int q = a.pt.urec_ptr[u]; q < a.pt.urec_ptr[u + 1]; q++) pnv += rs[(int64_t)nts * a.pt.urec_list[q]] * f;
           }
-          wrote = true;
         }
       }
-      if (wrote) pn_[c] = pnv;
+      pn_[c] = pnv;
       if (any_rec) {
         const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j + (int64_t)ny * k, c);
         if (r >= 0) {
-          const T v = wrote ? pnv : pn_[c];
           T *tr = a.traces + a.pt.off_tr[s] + (a.it - 1);
-          for (int q = a.pt.urec_ptr[r]; q < a.pt.urec_ptr[r + 1]; q++) tr[(int64_t)nts * a.pt.urec_list[q]] = v;
+          for (int q = a.pt.urec_ptr[r]; q < a.pt.urec_ptr[r + 1]; q++) tr[(int64_t)nts * a.pt.urec_list[q]] = pnv;
         }
       }
       if (dslot >= 0) {
         // p^{it} - 2 p^{it-1} + p^{it-2}: in the interior this is exactly the
         // stencil term plus the injection, formed without cancellation
-        const T pn = wrote ? pnv : pn_[c];
-        a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pn;
+        a.dbuf[(int64_t)dslot * a.dslot + so + c] = interior ? flap + inj : (cur.po - T(2) * pc) + pnv;
       }
       if (corr) a.grad[so + c] = cur.g + pc * cur.d;
       psz_prev = psz;
@@ -502,7 +503,10 @@ class Cd3dEngine final : public Engine {
     // 600 x 600 x 400 (the smaller blocks let more of them share a CU than
     // the general path's register count would with 64 x 8)
     kch_ = 24;
-    if (const char *v = std::getenv("SWX_CD3D_ZCHUNK")) kch_ = std::max(1, std::atoi(v));
+    if (const char *v = std::getenv("SWX_CD3D_ZCHUNK")) {
+      kch_ = std::max(1, std::atoi(v));
+      kch_env_ = true;
+    }
     if (const char *v = std::getenv("SWX_CD3D_LEAN")) lean_ = std::atoi(v) != 0;
     // launch shape: 8 / 4 / 2 = one launch with 64 x 8 / 64 x 4 / 64 x 2
     // tiles, 108 / 104 / 102 = interior tiles and the others in two launches
@@ -512,6 +516,7 @@ class Cd3dEngine final : public Engine {
                   "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
       SWX_REQUIRE(c < 200, SWX_EINVAL, "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
       variant_ = c;
+      variant_env_ = true;
     }
     zero_forward_state();
   }
@@ -536,14 +541,22 @@ class Cd3dEngine final : public Engine {
     SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
+    if (!tuned_) tune_shape(save, tr);
     for (int it = 1; it <= ntmax_; it++) {
       progress("forward step", it, ntmax_);
       step_fwd(it, 0, save ? 1 : 0, 0, tr);
+      take_snapshots(c_, *this, it);
       if (save && it % S_ == 0) {
         const int k = it / S_;
         if (k >= 1 && k <= K_ - 2) ckpt_copy(k, it, true);
       }
     }
+  }
+
+  // "pcur" (acou_forward.jl:55-57): level it is in slot it mod 3
+  int snap_fields(int it, int s, SnapPiece *out) const override {
+    out[0] = {P(it) + (int64_t)s * N_, (int64_t)(N_ * sizeof(T)), (int64_t)(N_ * sizeof(T)), 1};
+    return 1;
   }
 
   void adjoint(bool per_shot) override {
@@ -586,6 +599,7 @@ class Cd3dEngine final : public Engine {
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return dbuf_.bytes(); }
   int segment() const override { return S_; }
+  void variants(int32_t v[2]) const override { v[0] = variant_; v[1] = kch_; }
 
  private:
   int64_t strip(int ax, int w) const {
@@ -655,7 +669,35 @@ class Cd3dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
 
-  void step_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
+  // Launch shape (tile height 8 / 4 / 2, one or two launches) and then the
+  // planes per chunk, timed on step 1 of the zero initial state, which every
+  // repetition recomputes identically.  Every choice gives bitwise the same
+  // results; SWX_CD3D_VARIANT / SWX_CD3D_ZCHUNK pin them instead.
+  void tune_shape(bool save, T *tr) {
+    auto run = [&](int variant, int kch) {
+      variant_ = variant;
+      kch_ = kch;
+      Cd3dArgs<T> a = fwd_args(1, 0, save ? 1 : 0, 0, tr);
+      const int ty = variant % 100;
+      if (ty == 4) launch_shape<4>(a, variant > 100);
+      else if (ty == 2) launch_shape<2>(a, variant > 100);
+      else launch_shape<8>(a, variant > 100);
+      SWX_HIP(hipGetLastError());
+    };
+    if (!variant_env_) {
+      static const int vars[6] = {4, 8, 2, 104, 108, 102};
+      const int kch = kch_;
+      variant_ = vars[fastest_candidate(c_.stream, 6, [&](int c) { run(vars[c], kch); })];
+    }
+    if (!kch_env_) {
+      static const int chunks[3] = {24, 12, 48};
+      const int var = variant_;
+      kch_ = chunks[fastest_candidate(c_.stream, 3, [&](int c) { run(var, chunks[c]); })];
+    }
+    tuned_ = true;
+  }
+
+  Cd3dArgs<T> fwd_args(int it, int act_min, int dmode, int cbase, T *traces) const {
     Cd3dArgs<T> a = base_args(it);
     a.pold = P(it - 2); a.pcur = P(it - 1); a.pnew = P(it);
     for (int ax = 0; ax < 3; ax++) {
@@ -667,7 +709,11 @@ class Cd3dEngine final : public Engine {
     a.dmode = c_.d.gradient ? dmode : 0;
     a.cbase = cbase;
     a.traces = traces;
-    launch(a, 0);
+    return a;
+  }
+
+  void step_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
+    launch(fwd_args(it, act_min, dmode, cbase, traces), 0);
   }
 
   void step_adj(int it, int cbase) {
@@ -712,6 +758,7 @@ class Cd3dEngine final : public Engine {
   int cstride_, kch_ = 24;
   bool lean_ = true;
   int variant_ = 4;
+  bool variant_env_ = false, kch_env_ = false, tuned_ = false;
   DevBuf fact_, p_, psi_[3], xi_[3], cpml_;
   DevBuf a_, apsi_[3], axi_[3], grad_, gout_;
   DevBuf dbuf_, ck_;

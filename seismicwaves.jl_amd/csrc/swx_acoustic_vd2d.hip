@@ -71,6 +71,7 @@ struct Vd2dArgs {
   T *traces;                     // forward recording (nullable)
   const T *res;                  // adjoint residuals
   const int *nt_shot;
+  int nsh;                       // shots of the batch (pair kernels)
   int it, act_min;
   // pressure history of the segment: slot k holds p^{base+k}
   T *pbuf; int64_t pslot; int smode; int S; int cbase;
@@ -123,10 +124,57 @@ __device__ __forceinline__ bool vd2d_fwd_is_lean(const Vd2dArgs<T> &a, int s, in
 // Forward / replay step of an interior tile (vd2d_fwd_is_lean): the updates
 // of vd2d_fwd_kernel, in the same operand order, without per-cell tests, on
 // 32-bit offsets from per-shot bases.
+// Cross-shaped halo of the new pressure of a forward tile: columns I0-1,
+// I0+TX, I0+TX+1 on the tile rows, rows J0-1, J0+TZ, J0+TZ+1 on the tile
+// columns; entry k of thread tid, (lx, lz) in s_p coordinates or -1.
+template <int TZ>
+__device__ __forceinline__ void fwd_halo_cell(int q, int &lx, int &lz) {
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX;
+  lx = -1;
+  lz = -1;
+  if (q < HX) {
+    const int col = q / TZ;
+    lz = 1 + q % TZ;
+    lx = col == 0 ? 0 : TX + col;
+  } else if (q < H) {
+    const int row = (q - HX) / TX;
+    lx = 1 + (q - HX) % TX;
+    lz = row == 0 ? 0 : TZ + row;
+  }
+}
+
+// Model factors an interior forward tile reads (the same for every shot):
+// m0, m1x, m1z on the thread's R tile cells, m0 on its halo cells.
+template <typename T, int TZ>
+struct FwdTileModel {
+  static constexpr int R = TZ / BY, HR = (3 * TZ + 3 * TX + NT - 1) / NT;
+  T m0[R], m1x[R], m1z[R], hm0[HR];
+};
+
+template <typename T, int TZ>
+__device__ __forceinline__ void vd2d_fwd_lean_model(const Vd2dArgs<T> &a, int I0, int J0, FwdTileModel<T, TZ> &m) {
+  constexpr int R = TZ / BY, HR = FwdTileModel<T, TZ>::HR;
+  const int nx = a.nx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + ty + BY * r) * nx + I0 + tx;
+    m.m0[r] = a.m0[c];
+    m.m1x[r] = a.m1x[c];
+    m.m1z[r] = a.m1z[c];
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    int lx, lz;
+    fwd_halo_cell<TZ>(tid + k * NT, lx, lz);
+    m.hm0[k] = lx >= 0 ? a.m0[(J0 - 1 + lz) * nx + I0 - 1 + lx] : T(0);
+  }
+}
+
 template <typename T, int TZ>
 __device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, int nts, int I0, int J0,
                                                    T (&s_vx)[TZ + 3][TX + 6], T (&s_vz)[TZ + 6][TX + 3],
-                                                   T (&s_p)[TZ + 3][TX + 3]) {
+                                                   T (&s_p)[TZ + 3][TX + 3], const FwdTileModel<T, TZ> &m) {
   constexpr int R = TZ / BY;
   const int nx = a.nx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
@@ -151,44 +199,21 @@ __device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, 
     lz0[k] = lz < TZ + 6 ? vzo[c] : T(0);
     lz1[k] = lz < TZ + 6 && tx < 3 ? vzo[c + TX] : T(0);
   }
-  T rp[R], rm0[R], rm1x[R], rm1z[R];
+  T rp[R];
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int c = (J0 + ty + BY * r) * nx + I0 + tx;
-    rp[r] = po[c];
-    rm0[r] = a.m0[c];
-    rm1x[r] = a.m1x[c];
-    rm1z[r] = a.m1z[c];
-  }
-  // cross-shaped halo of the new pressure: columns I0-1, I0+TX, I0+TX+1
-  // on the tile rows, rows J0-1, J0+TZ, J0+TZ+1 on the tile columns
-  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
-  T hp[HR], hm0[HR];
+  for (int r = 0; r < R; r++) rp[r] = po[(J0 + ty + BY * r) * nx + I0 + tx];
+  constexpr int HR = FwdTileModel<T, TZ>::HR;
+  T hp[HR];
   int hlx[HR], hlz[HR];
 #pragma unroll
   for (int k = 0; k < HR; k++) {
-    const int q = tid + k * NT;
-    int lx = -1, lz = -1;
-    if (q < HX) {
-      const int col = q / TZ;
-      lz = 1 + q % TZ;
-      lx = col == 0 ? 0 : TX + col;
-    } else if (q < H) {
-      const int row = (q - HX) / TX;
-      lx = 1 + (q - HX) % TX;
-      lz = row == 0 ? 0 : TZ + row;
-    }
-    T vp = 0, vm = 0;
-    if (lx >= 0) {
-      const int c = (J0 - 1 + lz) * nx + I0 - 1 + lx;
-      vp = po[c];
-      vm = a.m0[c];
-    }
-    hlx[k] = lx;
-    hlz[k] = lz;
-    hp[k] = vp;
-    hm0[k] = vm;
+    fwd_halo_cell<TZ>(tid + k * NT, hlx[k], hlz[k]);
+    hp[k] = hlx[k] >= 0 ? po[(J0 - 1 + hlz[k]) * nx + I0 - 1 + hlx[k]] : T(0);
   }
+  const T(&rm0)[R] = m.m0;
+  const T(&rm1x)[R] = m.m1x;
+  const T(&rm1z)[R] = m.m1z;
+  const T(&hm0)[HR] = m.hm0;
 #pragma unroll
   for (int k = 0; k < KX; k++) {
     const int lz = ty + k * BY;
@@ -257,34 +282,24 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
   __shared__ T s_vx[TZ + 3][TX + 6];
   __shared__ T s_vz[TZ + 6][TX + 3];
   __shared__ T s_p[TZ + 3][TX + 3];
-  vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p);
+  FwdTileModel<T, TZ> m;
+  vd2d_fwd_lean_model<T, TZ>(a, I0, J0, m);
+  vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p, m);
 }
 
-// Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.  Interior
-// tiles are stepped inline (FOLD, one launch per step) or left to
-// vd2d_fwd_lean.
-template <typename T, int TZ, bool FOLD>
-__global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
+// Forward / replay step of a tile that is not interior (C-PML strips, grid
+// edges, points): update_p_CPML!, injection, recording, then the velocity
+// updates with their C-PML memory variables, with per-cell tests.
+template <typename T, int TZ>
+__device__ __forceinline__ void vd2d_fwd_general_tile(const Vd2dArgs<T> &a, int s, int nts, int I0, int J0,
+                                                      T (&s_vx)[TZ + 3][TX + 6], T (&s_vz)[TZ + 6][TX + 3],
+                                                      T (&s_p)[TZ + 3][TX + 3], int (&s_pts)[2]) {
   constexpr int R = TZ / BY;
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts || nts <= a.act_min) return;
   const int nx = a.nx, nz = a.nz, h = a.h;
-  const int I0 = bx * TX, J0 = bz * TZ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
-
-  __shared__ T s_vx[TZ + 3][TX + 6];   // vx^{it-1}: x in [I0-3, I0+TX+3), z in [J0-1, J0+TZ+2)
-  __shared__ T s_vz[TZ + 6][TX + 3];   // vz^{it-1}: x in [I0-1, I0+TX+2), z in [J0-3, J0+TZ+3)
-  __shared__ T s_p[TZ + 3][TX + 3];    // p^{it}:    x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2)
-
-  if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) {
-    if (FOLD) vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p);
-    return;
-  }
 
   // ---- phase 0: every load that does not depend on LDS, all issued before
   // the first LDS store -------------------------------------------------------
@@ -325,7 +340,6 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
   constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
   // does any source lie in the rows whose new pressure this block computes,
   // or any receiver in its tile rows?  (most blocks: no point lookups)
-  __shared__ int s_pts[2];
   if (tid == 0) {
     const int *sr = a.pt.srow + (int64_t)s * (a.pt.nrows + 1);
     const int *rr = a.pt.rrow + (int64_t)s * (a.pt.nrows + 1);
@@ -460,6 +474,65 @@ __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
         d = d + ps;
       }
       a.vz_n[so + c] = s_vz[lz + 3][tx + 1] - rm1z[r] * d;
+    }
+  }
+}
+
+// Forward / replay step: (p, vx, vz)^{it-1} -> (p, vx, vz)^{it}.  Interior
+// tiles are stepped inline (FOLD, one launch per step) or left to
+// vd2d_fwd_lean.
+template <typename T, int TZ, bool FOLD>
+__global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  __shared__ T s_vx[TZ + 3][TX + 6];   // vx^{it-1}: x in [I0-3, I0+TX+3), z in [J0-1, J0+TZ+2)
+  __shared__ T s_vz[TZ + 6][TX + 3];   // vz^{it-1}: x in [I0-1, I0+TX+2), z in [J0-3, J0+TZ+3)
+  __shared__ T s_p[TZ + 3][TX + 3];    // p^{it}:    x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2)
+  __shared__ int s_pts[2];
+  if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) {
+    if (FOLD) {
+      FwdTileModel<T, TZ> m;
+      vd2d_fwd_lean_model<T, TZ>(a, I0, J0, m);
+      vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p, m);
+    }
+    return;
+  }
+  vd2d_fwd_general_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p, s_pts);
+}
+
+// Forward / replay step of a pair of shots (2k, 2k+1) per block: the model
+// factors of an interior tile are read once for both shots (m0, m1x, m1z
+// are 3 of the 9 values a cell step reads and writes).  The shots are
+// stepped one after the other with the expressions of vd2d_fwd_kernel, so
+// the results are bitwise the same.
+template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_fwd_pair_kernel(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, pr = bzs / a.nbz;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  __shared__ T s_vx[TZ + 3][TX + 6];
+  __shared__ T s_vz[TZ + 6][TX + 3];
+  __shared__ T s_p[TZ + 3][TX + 3];
+  __shared__ int s_pts[2];
+  FwdTileModel<T, TZ> m;
+  bool have_m = false;
+  for (int k = 0; k < 2; k++) {
+    const int s = 2 * pr + k;
+    if (s >= a.nsh) break;
+    const int nts = a.nt_shot[s];
+    if (a.it > nts || nts <= a.act_min) continue;
+    if (k == 1) __syncthreads();   // the LDS tiles of the first shot are consumed
+    if (vd2d_fwd_is_lean<TZ>(a, s, I0, J0)) {
+      if (!have_m) {
+        vd2d_fwd_lean_model<T, TZ>(a, I0, J0, m);
+        have_m = true;
+      }
+      vd2d_fwd_lean_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p, m);
+    } else {
+      vd2d_fwd_general_tile<T, TZ>(a, s, nts, I0, J0, s_vx, s_vz, s_p, s_pts);
     }
   }
 }
@@ -941,9 +1014,46 @@ __device__ __forceinline__ bool vd2d_adj_is_lean(const Vd2dArgs<T> &a, int s, in
 // correlations of vd2d_adj_kernel, in the same operand order, without
 // per-cell tests.
 template <typename T, int TZ>
+struct AdjTileModel {
+  static constexpr int R = TZ / BY, HR = (3 * TZ + 3 * TX + NT - 1) / NT;
+  T m1x[R], m1z[R], m0[R], hm[HR];
+};
+
+// Model factors an interior adjoint tile reads (the same for every shot):
+// m1x, m1z, m0 on the thread's tile cells; m1x on the v~x halo columns
+// I0-2, I0-1, I0+TX and m1z on the v~z halo rows J0-2, J0-1, J0+TZ.
+template <typename T, int TZ>
+__device__ __forceinline__ void vd2d_adj_lean_model(const Vd2dArgs<T> &a, int I0, int J0, AdjTileModel<T, TZ> &m) {
+  constexpr int R = TZ / BY, HX = 3 * TZ, H = HX + 3 * TX, HR = AdjTileModel<T, TZ>::HR;
+  const int nx = a.nx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int c = (J0 + ty + BY * r) * nx + I0 + tx;
+    m.m1x[r] = a.m1x[c];
+    m.m1z[r] = a.m1z[c];
+    m.m0[r] = a.m0[c];
+  }
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int q = tid + k * NT;
+    T v = 0;
+    if (q < HX) {
+      const int col = q / TZ, lz = q % TZ;
+      v = a.m1x[(J0 + lz) * nx + I0 - 2 + (col < 2 ? col : TX + 2)];
+    } else if (q < H) {
+      const int row = (q - HX) / TX, lx = (q - HX) % TX;
+      v = a.m1z[(J0 - 2 + (row < 2 ? row : TZ + 2)) * nx + I0 + lx];
+    }
+    m.hm[k] = v;
+  }
+}
+
+template <typename T, int TZ>
 __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, int I0, int J0,
                                                    T (&s_p)[TZ + 6][TX + 6], T (&s_pc)[TZ + 3][TX + 3],
-                                                   T (&s_vx)[TZ][TX + 3], T (&s_vz)[TZ + 3][TX]) {
+                                                   T (&s_vx)[TZ][TX + 3], T (&s_vz)[TZ + 3][TX],
+                                                   const AdjTileModel<T, TZ> &m) {
   constexpr int R = TZ / BY;
   const int nx = a.nx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
@@ -976,15 +1086,12 @@ __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, 
     lc0[k] = lz < TZ + 3 ? pc_[c + (arm ? 0 : 1) + tx] : T(0);
     lc1[k] = arm && tx < 3 ? pc_[c + TX + tx] : T(0);
   }
-  T rvx[R], rvz[R], rm1x[R], rm1z[R], rm0[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
+  T rvx[R], rvz[R], rpo[R], rg0[R], rg1x[R], rg1z[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int c = (J0 + ty + BY * r) * nx + I0 + tx;
     rvx[r] = vxo[c];
     rvz[r] = vzo[c];
-    rm1x[r] = a.m1x[c];
-    rm1z[r] = a.m1z[c];
-    rm0[r] = a.m0[c];
     rpo[r] = po_[c];
     rg0[r] = a.g0[so + c];
     rg1x[r] = a.g1x[so + c];
@@ -992,26 +1099,25 @@ __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, 
   }
   // velocity halo: v~x columns I0-2, I0-1, I0+TX on tile rows; v~z rows
   // J0-2, J0-1, J0+TZ on tile columns
-  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = (H + NT - 1) / NT;
-  T hv[HR], hm[HR];
+  constexpr int HX = 3 * TZ, H = HX + 3 * TX, HR = AdjTileModel<T, TZ>::HR;
+  T hv[HR];
 #pragma unroll
   for (int k = 0; k < HR; k++) {
     const int q = tid + k * NT;
-    T v = 0, m = 0;
+    T v = 0;
     if (q < HX) {
       const int col = q / TZ, lz = q % TZ;
-      const int c = (J0 + lz) * nx + I0 - 2 + (col < 2 ? col : TX + 2);
-      v = vxo[c];
-      m = a.m1x[c];
+      v = vxo[(J0 + lz) * nx + I0 - 2 + (col < 2 ? col : TX + 2)];
     } else if (q < H) {
       const int row = (q - HX) / TX, lx = (q - HX) % TX;
-      const int c = (J0 - 2 + (row < 2 ? row : TZ + 2)) * nx + I0 + lx;
-      v = vzo[c];
-      m = a.m1z[c];
+      v = vzo[(J0 - 2 + (row < 2 ? row : TZ + 2)) * nx + I0 + lx];
     }
     hv[k] = v;
-    hm[k] = m;
   }
+  const T(&rm1x)[R] = m.m1x;
+  const T(&rm1z)[R] = m.m1z;
+  const T(&rm0)[R] = m.m0;
+  const T(&hm)[HR] = m.hm;
 #pragma unroll
   for (int k = 0; k < KP; k++) {
     const int lz = ty + k * BY;
@@ -1104,33 +1210,23 @@ __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
   __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
   __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
   __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
-  vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz);
+  AdjTileModel<T, TZ> m;
+  vd2d_adj_lean_model<T, TZ>(a, I0, J0, m);
+  vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz, m);
 }
 
-// Adjoint step it: (p~, v~)^{it+1} -> (p~, v~)^{it}, then the correlation
-// with p^{it}, p^{it-1} of the replayed segment.  Interior tiles are stepped
-// inline (FOLD, one launch per step) or left to vd2d_adj_lean.
-template <typename T, int TZ, bool FOLD>
-__global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
+// Adjoint step of a tile that is not interior: the general path with
+// per-cell tests (C-PML memory variables, edges, residual injection).
+template <typename T, int TZ>
+__device__ __forceinline__ void vd2d_adj_general_tile(const Vd2dArgs<T> &a, int s, int nts, int I0, int J0,
+                                                      T (&s_p)[TZ + 6][TX + 6], T (&s_pc)[TZ + 3][TX + 3],
+                                                      T (&s_vx)[TZ][TX + 3], T (&s_vz)[TZ + 3][TX]) {
   constexpr int R = TZ / BY;
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
-  const int nts = a.nt_shot[s];
-  if (a.it > nts) return;
   const int nx = a.nx, nz = a.nz, h = a.h;
-  const int I0 = bx * TX, J0 = bz * TZ;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int64_t so = (int64_t)s * a.N;
   const T *cb = a.cpml + (int64_t)s * a.cstride;
   const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
-  __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
-  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
-  __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
-  __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
-  if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) {
-    if (FOLD) vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz);
-    return;
-  }
   const int slot = a.it - a.cbase;
   const T *pc_ = a.pbuf + (int64_t)slot * a.pslot + so;        // p^{it}
   const T *po_ = a.pbuf + (int64_t)(slot - 1) * a.pslot + so;  // p^{it-1}
@@ -1328,6 +1424,64 @@ __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
     }
   }
 }
+
+// Adjoint step it: (p~, v~)^{it+1} -> (p~, v~)^{it}, then the correlation
+// with p^{it}, p^{it-1} of the replayed segment.  Interior tiles are stepped
+// inline (FOLD, one launch per step) or left to vd2d_adj_lean.
+template <typename T, int TZ, bool FOLD>
+__global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts) return;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  __shared__ T s_p[TZ + 6][TX + 6];   // p~^{old}: x in [I0-3, I0+TX+3), z in [J0-3, J0+TZ+3) (cross)
+  __shared__ T s_pc[TZ + 3][TX + 3];  // p^{it}:   x in [I0-1, I0+TX+2), z in [J0-1, J0+TZ+2) (cross)
+  __shared__ T s_vx[TZ][TX + 3];      // v~x^{new}: x in [I0-2, I0+TX+1), z in [J0, J0+TZ)
+  __shared__ T s_vz[TZ + 3][TX];      // v~z^{new}: x in [I0, I0+TX), z in [J0-2, J0+TZ+1)
+  if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) {
+    if (FOLD) {
+      AdjTileModel<T, TZ> m;
+      vd2d_adj_lean_model<T, TZ>(a, I0, J0, m);
+      vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz, m);
+    }
+    return;
+  }
+  vd2d_adj_general_tile<T, TZ>(a, s, nts, I0, J0, s_p, s_pc, s_vx, s_vz);
+}
+
+// Adjoint step of a pair of shots (2k, 2k+1) per block: the model factors
+// of an interior tile are read once for both shots; each shot is stepped
+// with the expressions of vd2d_adj_kernel (bitwise the same results).
+template <typename T, int TZ>
+__global__ __launch_bounds__(NT) void vd2d_adj_pair_kernel(Vd2dArgs<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, pr = bzs / a.nbz;
+  const int I0 = bx * TX, J0 = bz * TZ;
+  __shared__ T s_p[TZ + 6][TX + 6];
+  __shared__ T s_pc[TZ + 3][TX + 3];
+  __shared__ T s_vx[TZ][TX + 3];
+  __shared__ T s_vz[TZ + 3][TX];
+  AdjTileModel<T, TZ> m;
+  bool have_m = false;
+  for (int k = 0; k < 2; k++) {
+    const int s = 2 * pr + k;
+    if (s >= a.nsh) break;
+    const int nts = a.nt_shot[s];
+    if (a.it > nts) continue;
+    if (k == 1) __syncthreads();   // the LDS tiles of the first shot are consumed
+    if (vd2d_adj_is_lean<TZ>(a, s, I0, J0)) {
+      if (!have_m) {
+        vd2d_adj_lean_model<T, TZ>(a, I0, J0, m);
+        have_m = true;
+      }
+      vd2d_adj_lean_tile<T, TZ>(a, s, I0, J0, s_p, s_pc, s_vx, s_vz, m);
+    } else {
+      vd2d_adj_general_tile<T, TZ>(a, s, nts, I0, J0, s_p, s_pc, s_vx, s_vz);
+    }
+  }
+}
+
 
 // Adjoint step marching down z (same strip/chunk decomposition as
 // vd2d_fwd_march).  At row j: the new adjoint velocities of row j (vx also
@@ -1585,14 +1739,17 @@ class Vd2dEngine final : public Engine {
     zero(st_);
     if (!tz_fwd_) {
       const Vd2dArgs<T> a1 = fwd_args(1, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
-      tz_fwd_ = tune("SWX_VD2D_TILE_FWD", candidates({8, 16, Tiles<T>::fwd_max, 308, 316, 300 + Tiles<T>::fwd_max}),
+      tz_fwd_ = tune("SWX_VD2D_TILE_FWD",
+                     candidates({8, 16, Tiles<T>::fwd_max, 308, 316, 300 + Tiles<T>::fwd_max}),
                      [&](int tz) { launch_fwd(a1, tz); });
     }
     for (int it = 1; it <= ntmax_;) {
       progress("forward step", it, ntmax_);
-      // a two-step launch never steps over a checkpoint level
-      const int n = steps_per_launch(tz_fwd_) == 2 && it % S_ != 0 && it + 1 <= ntmax_ ? 2 : 1;
+      // a two-step launch never steps over a checkpoint or snapshot level
+      const int n = steps_per_launch(tz_fwd_) == 2 && it % S_ != 0 && it + 1 <= ntmax_ && c_.snap_every == 0 ? 2
+                                                                                                             : 1;
       fwd_step(it, n, 0, save && c_.d.gradient ? 1 : 0, 0, c_.pts.d_traces.as<T>());
+      if (n == 1) take_snapshots(c_, *this, it);
       it += n;
       const int last = it - 1;
       if (save && last % S_ == 0) {
@@ -1675,6 +1832,16 @@ class Vd2dEngine final : public Engine {
   int segment() const override { return S_; }
   void variants(int32_t v[2]) const override { v[0] = tz_fwd_; v[1] = tz_adj_; }
 
+  // "pcur" and "vcur" = [vx, vz] (acou_forward.jl:116-120): level it is in
+  // parity slot(it) right after its step
+  int snap_fields(int it, int s, SnapPiece *out) const override {
+    const int64_t e = sizeof(T), so = (int64_t)s * N_;
+    out[0] = {F3(st_, slot(it), 0) + so, nx_ * e, nx_ * e, nz_};
+    out[1] = {F3(st_, slot(it), 1) + so, (nx_ - 1) * e, nx_ * e, nz_};
+    out[2] = {F3(st_, slot(it), 2) + so, nx_ * e, nx_ * e, nz_ - 1};
+    return 3;
+  }
+
  private:
   T *M(int f) const { return m_.as<T>() + (int64_t)f * N_; }
   // field f (0 p, 1 vx, 2 vz) of parity par in state block b
@@ -1712,6 +1879,7 @@ class Vd2dEngine final : public Engine {
     a.srctf = c_.pts.d_srctf.as<T>();
     a.res = c_.pts.d_res.as<T>();
     a.nt_shot = c_.pts.d_nt.as<int>();
+    a.nsh = nsh_;
     a.it = it;
     a.pbuf = pbuf_.as<T>(); a.pslot = (int64_t)ns_ * N_;
     a.S = store_all_ ? (1 << 30) : S_;   // store-all: the final segment is the whole run
@@ -1757,7 +1925,16 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipGetLastError());
       return;
     }
-    // interior tiles, then the others (each kernel skips the other's tiles)
+    if (tz == 408 || tz == 416) {
+      // two shots per block, both kinds of tiles in one launch
+      a.nbx = (int)ceil_div(nx_, TX);
+      a.nbz = (int)ceil_div(nz_, tz - 400);
+      const unsigned g = (unsigned)((int64_t)a.nbx * a.nbz * ceil_div(nsh_, 2));
+      if (tz == 408) vd2d_fwd_pair_kernel<T, 8><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      else vd2d_fwd_pair_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      SWX_HIP(hipGetLastError());
+      return;
+    }
     // 8/16/32: interior tiles, then the others (each kernel skips the other's
     // tiles); 308/316/332: one launch stepping both kinds of tiles
     const unsigned g = grid(a, tz % 100);
@@ -1780,7 +1957,15 @@ class Vd2dEngine final : public Engine {
       SWX_HIP(hipGetLastError());
       return;
     }
-    // interior tiles, then the others (each kernel skips the other's tiles)
+    if (tz == 416) {
+      // two shots per block, both kinds of tiles in one launch
+      a.nbx = (int)ceil_div(nx_, TX);
+      a.nbz = (int)ceil_div(nz_, 16);
+      const unsigned g = (unsigned)((int64_t)a.nbx * a.nbz * ceil_div(nsh_, 2));
+      vd2d_adj_pair_kernel<T, 16><<<g, dim3(TX, BY), 0, c_.stream>>>(a);
+      SWX_HIP(hipGetLastError());
+      return;
+    }
     // 8/16: interior tiles, then the others; 308/316: one launch
     const unsigned g = grid(a, tz % 100);
     const dim3 blk(TX, BY);
@@ -1851,32 +2036,42 @@ class Vd2dEngine final : public Engine {
   // Times each candidate tile height on a real launch (a step depends only
   // on the previous parity, so repeating it is harmless; the adjoint tuning
   // runs before the adjoint state and gradients are cleared) and returns the
-  // fastest.  SWX_VD2D_TILE_FWD / _ADJ pin a value instead.
+  // fastest.  The candidates are timed in three interleaved rounds of five
+  // launches and each keeps its best round, so a clock ramp or a busy
+  // moment during one round does not decide.  SWX_VD2D_TILE_FWD / _ADJ pin
+  // a value instead (also the variants the tuner does not time: the
+  // z-marching strips 102/104, two steps per launch 208/216/232 and two
+  // shots per block 408/416, all measured slower on MI355X).
   template <typename F>
   int tune(const char *env, const std::vector<int> &cands, F launch) {
     if (const char *v = std::getenv(env)) {
       const int t = std::atoi(v);
       for (int c : cands)
         if (c == t) return t;
-      if (t == 102 || t == 104 || t == 208 || t == 216 || t == 232) return t;
+      if (t == 102 || t == 104 || t == 208 || t == 216 || t == 232 || t == 408 || t == 416) return t;
     }
     if (cands.size() == 1) return cands.front();
     hipEvent_t e0, e1;
     SWX_HIP(hipEventCreate(&e0));
     SWX_HIP(hipEventCreate(&e1));
+    std::vector<float> best_of(cands.size(), 1e30f);
+    for (size_t c = 0; c < cands.size(); c++) launch(cands[c]);
+    for (int round = 0; round < 3; round++) {
+      for (size_t c = 0; c < cands.size(); c++) {
+        SWX_HIP(hipEventRecord(e0, c_.stream));
+        for (int r = 0; r < 5; r++) launch(cands[c]);
+        SWX_HIP(hipEventRecord(e1, c_.stream));
+        SWX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)steps_per_launch(cands[c]);   // per time step
+        best_of[c] = std::min(best_of[c], ms);
+      }
+    }
     int best = cands.front();
     float bestms = 1e30f;
-    for (int c : cands) {
-      launch(c);
-      SWX_HIP(hipEventRecord(e0, c_.stream));
-      for (int r = 0; r < 5; r++) launch(c);
-      SWX_HIP(hipEventRecord(e1, c_.stream));
-      SWX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ms /= (float)steps_per_launch(c);   // per time step
-      if (ms < bestms) { bestms = ms; best = c; }
-    }
+    for (size_t c = 0; c < cands.size(); c++)
+      if (best_of[c] < bestms) { bestms = best_of[c]; best = cands[c]; }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return best;

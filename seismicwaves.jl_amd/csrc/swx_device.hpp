@@ -13,6 +13,7 @@ namespace swx {
 // shot s, or -1.  rowptr/cells are a PointTables row index (sorted cells).
 __device__ __forceinline__ int find_point(const int *rowptr, const int64_t *cells, int64_t nrows, int s,
                                           int64_t row, int64_t cell) {
+  SWX_DASSERT(row >= 0 && row < nrows && s >= 0);
   const int *rp = rowptr + (int64_t)s * (nrows + 1) + row;
   int lo = rp[0], hi = rp[1];
   while (lo < hi) {
@@ -32,10 +33,10 @@ template <typename T>
 struct AxisCoef {
   const T *p;
   int h;
-  __device__ __forceinline__ T a(int k) const { return p[k]; }
-  __device__ __forceinline__ T ah(int k) const { return p[2 * h + 2 + k]; }
-  __device__ __forceinline__ T b(int k) const { return p[4 * h + 2 + k]; }
-  __device__ __forceinline__ T bh(int k) const { return p[6 * h + 4 + k]; }
+  __device__ __forceinline__ T a(int k) const { SWX_DASSERT(k >= 0 && k < 2 * h + 2); return p[k]; }
+  __device__ __forceinline__ T ah(int k) const { SWX_DASSERT(k >= 0 && k < 2 * h); return p[2 * h + 2 + k]; }
+  __device__ __forceinline__ T b(int k) const { SWX_DASSERT(k >= 0 && k < 2 * h + 2); return p[4 * h + 2 + k]; }
+  __device__ __forceinline__ T bh(int k) const { SWX_DASSERT(k >= 0 && k < 2 * h); return p[6 * h + 4 + k]; }
 };
 template <typename T>
 __device__ __forceinline__ AxisCoef<T> axis_coef(const T *base, int h, int axis) {

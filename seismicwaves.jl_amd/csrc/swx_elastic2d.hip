@@ -143,7 +143,10 @@ __device__ __forceinline__ T d4(T f1, T f2, T f3, T f4, T inv) {
 template <typename T>
 struct F2 {
   const T *p; int nx;
-  __device__ __forceinline__ T operator()(int i, int j) const { return p[(i - 1) + (int64_t)(j - 1) * nx]; }
+  __device__ __forceinline__ T operator()(int i, int j) const {
+    SWX_DASSERT(i >= 1 && i <= nx && j >= 1);
+    return p[(i - 1) + (int64_t)(j - 1) * nx];
+  }
 };
 
 // Derivative wrappers with the boundary rules of
@@ -1348,11 +1351,28 @@ class El2dEngine final : public Engine {
     for (int it = 1; it <= ntmax_; it++) {
       progress("forward step", it, ntmax_);
       step_fwd(it, 0, save ? 1 : 0, 0, tr);
+      take_snapshots(c_, *this, it);
       if (save && it % S_ == 0) {
         const int k = it / S_;
         if (k >= 1 && k <= nck_) ckpt_copy(k, true);
       }
     }
+  }
+
+  // "ucur" = [ux, uz] and "σ" = [σxx, σzz, σxz] (ela_forward.jl:74-77)
+  // right after step it; the stresses are in HBM for the variants with a
+  // stress launch (the z-marching ones keep them on chip, so a forward with
+  // snapshots runs variant 1)
+  int snap_fields(int it, int s, SnapPiece *out) const override {
+    const int64_t e = sizeof(T), nx = nx_, nz = nz_;
+    const T *u = U(u_, it) + (int64_t)s * 2 * N_;
+    const T *sg = sig_.as<T>() + (int64_t)s * 3 * N_;
+    out[0] = {u, (nx - 1) * e, nx * e, nz};
+    out[1] = {u + N_, nx * e, nx * e, nz - 1};
+    out[2] = {sg, nx * e, nx * e, nz};
+    out[3] = {sg + N_, nx * e, nx * e, nz};
+    out[4] = {sg + 2 * N_, (nx - 1) * e, nx * e, nz - 1};
+    return 5;
   }
 
   void adjoint(bool per_shot) override {
@@ -1528,7 +1548,8 @@ class El2dEngine final : public Engine {
   }
 
   void step_fwd(int it, int act_min, int hmode, int cbase, T *traces) {
-    step_fwd_v(var_fwd_, it, act_min, hmode, cbase, traces);
+    // with snapshots the stresses must be in HBM: no z-marching variant
+    step_fwd_v(c_.snap_every > 0 && var_fwd_ > 100 ? 1 : var_fwd_, it, act_min, hmode, cbase, traces);
   }
   void step_adj(int it, int cbase) { step_adj_v(var_adj_, it, cbase); }
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -12,6 +13,16 @@
 #include <vector>
 
 #include "swx.h"
+
+// Debug build (build.py --debug, -DSWX_DEBUG): device-side asserts on the
+// indices of the shared accessors (point-table rows, C-PML coefficient
+// vectors, elastic field accessor); they compile to nothing otherwise.
+#ifdef SWX_DEBUG
+#include <cassert>
+#define SWX_DASSERT(cond) assert(cond)
+#else
+#define SWX_DASSERT(cond) ((void)0)
+#endif
 
 namespace swx {
 
@@ -182,6 +193,13 @@ class KernelTimer {
   size_t used_ = 0;
 };
 
+// One field of a snapshot: `rows` rows of `width` bytes, `pitch` bytes apart
+// on the device (staggered fields live in nx-pitched slabs).
+struct SnapPiece {
+  const void *dev;
+  int64_t width, pitch, rows;
+};
+
 // Engine interface: one implementation per physics x dimension x dtype.
 class Engine {
  public:
@@ -206,6 +224,13 @@ class Engine {
   virtual int segment() const = 0;
   // Kernel variants chosen by the autotuner (0 if a family has one).
   virtual void variants(int32_t v[2]) const { v[0] = v[1] = 0; }
+  // The snapshotted fields of shot s at forward level it (valid right after
+  // that forward step), in the reference snapshotter's order; returns their
+  // number (0: snapshots not provided).
+  virtual int snap_fields(int it, int s, SnapPiece *out) const {
+    (void)it; (void)s; (void)out;
+    return 0;
+  }
 };
 
 struct PlanCore {
@@ -219,12 +244,35 @@ struct PlanCore {
   double ms_kernel[2] = {0, 0};
   int64_t n_kernel[2] = {0, 0};
   size_t budget = 0;  // bytes
+  // observed data of an L2 misfit for the current batch (swx_set_observed):
+  // T [total_tr], optional double weights [total_tr], per-shot sample
+  // offsets [ns+1] and the partial sums of the misfit reduction
+  bool obs_set = false, obs_weighted = false;
+  bool traces_valid = false;   // d_traces holds the last forward's traces
+  // snapshots of the forward runs (swx_set_snapshots): every snap_every
+  // steps the engine's snap_fields go to snap_host + snap_off[s] + k * snap_one
+  int snap_every = 0;
+  char *snap_host = nullptr;
+  int64_t snap_one = 0;
+  std::vector<int64_t> snap_off;
+  DevBuf d_obs, d_obs_w, d_obs_off, d_obs_part;
 };
+
+// Copies the snapshot of forward level it of every shot still running (a
+// no-op unless snapshots are set and it is a multiple of snap_every).
+void take_snapshots(PlanCore &c, const Engine &e, int it);
+
+// Uploads the observed data (and weights) of the current batch.
+void set_observed(PlanCore &c, const void *observed, const double *weights);
+// d_res = -(traces - observed) * weights on the device; with misfit_host,
+// the per-shot misfits 1/2 sum w r^2 (copied asynchronously on c.stream).
+void l2_adjoint_source(PlanCore &c, double *misfit_host);
 
 std::unique_ptr<Engine> make_acoustic_cd2d(PlanCore &core);
 std::unique_ptr<Engine> make_acoustic_vd2d(PlanCore &core);
 std::unique_ptr<Engine> make_acoustic_cd3d(PlanCore &core);
 std::unique_ptr<Engine> make_elastic2d(PlanCore &core);
+std::unique_ptr<Engine> make_acoustic_1d(PlanCore &core);
 
 // Helpers
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -238,5 +286,38 @@ void progress(const char *phase, int done, int total);
 // bytes per replay slot (all shots) and per checkpoint (all shots), the
 // user check_freq (1 = keep every step) and the memory still available.
 int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail);
+
+// Launch-shape tuner: times launch(k) for the candidates k = 0 .. n-1 on
+// `stream`, after one warm-up launch each, in three interleaved rounds of
+// five launches (a clock ramp or a busy moment then cannot decide), keeps
+// each candidate's best round and returns the index of the fastest.  The
+// launches must be repeatable (the engines time a step that reads only
+// state it does not write, or the zero initial state).
+template <typename F>
+int fastest_candidate(hipStream_t stream, int n, F launch) {
+  if (n <= 1) return 0;
+  hipEvent_t e0, e1;
+  SWX_HIP(hipEventCreate(&e0));
+  SWX_HIP(hipEventCreate(&e1));
+  std::vector<float> best(n, 1e30f);
+  for (int k = 0; k < n; k++) launch(k);
+  for (int round = 0; round < 3; round++) {
+    for (int k = 0; k < n; k++) {
+      SWX_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < 5; r++) launch(k);
+      SWX_HIP(hipEventRecord(e1, stream));
+      SWX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      best[k] = std::min(best[k], ms);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int k_best = 0;
+  for (int k = 1; k < n; k++)
+    if (best[k] < best[k_best]) k_best = k;
+  return k_best;
+}
 
 }  // namespace swx

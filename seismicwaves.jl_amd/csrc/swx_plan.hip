@@ -21,6 +21,21 @@ namespace swx {
 
 void set_last_error(const char *msg) { g_last_error = msg; }
 
+void take_snapshots(PlanCore &c, const Engine &e, int it) {
+  if (c.snap_every <= 0 || it % c.snap_every != 0) return;
+  SnapPiece pc[8];
+  for (int s = 0; s < c.pts.nshots; s++) {
+    if (it > c.pts.nt[s]) continue;
+    char *dst = c.snap_host + c.snap_off[s] + (int64_t)(it / c.snap_every - 1) * c.snap_one;
+    const int n = e.snap_fields(it, s, pc);
+    for (int k = 0; k < n; k++) {
+      SWX_HIP(hipMemcpy2DAsync(dst, pc[k].width, pc[k].dev, pc[k].pitch, pc[k].width, pc[k].rows,
+                               hipMemcpyDeviceToHost, c.stream));
+      dst += pc[k].width * pc[k].rows;
+    }
+  }
+}
+
 KernelTimer::~KernelTimer() {
   for (auto e : ev_) (void)hipEventDestroy(e);
 }
@@ -89,7 +104,7 @@ int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_
 
 void validate_plan_desc(const swx_plan_desc &d) {
   SWX_REQUIRE(d.dtype == SWX_F32 || d.dtype == SWX_F64, SWX_EINVAL, "dtype must be SWX_F32 or SWX_F64");
-  SWX_REQUIRE(d.ndim == 2 || d.ndim == 3, SWX_EINVAL, "Dimensionality must be 2 or 3!");
+  SWX_REQUIRE(d.ndim >= 1 && d.ndim <= 3, SWX_EINVAL, "Dimensionality must be less than or equal to 3!");
   for (int k = 0; k < d.ndim; k++) {
     SWX_REQUIRE(d.n[k] > 0, SWX_EINVAL, "All numbers of grid points must be positive!");
     SWX_REQUIRE(d.h[k] > 0, SWX_EINVAL, "All grid spacings must be positive!");
@@ -246,6 +261,31 @@ namespace {
 void forget_shots(PlanCore &c) {
   c.pts.nshots = 0;
   c.forward_done = false;
+  c.traces_valid = false;
+  c.obs_set = false;
+  c.snap_every = 0;
+  c.snap_host = nullptr;
+}
+
+// Adjoint loop of the engine (residuals already in pts.d_res), timed, then
+// the gradient copied to the host.
+void run_adjoint(swx_plan *plan, void *grad, int per_shot) {
+  PlanCore &c = plan->core;
+  hipEvent_t e0, e1;
+  SWX_HIP(hipEventCreate(&e0));
+  SWX_HIP(hipEventCreate(&e1));
+  c.ktimer.reset();
+  SWX_HIP(hipEventRecord(e0, c.stream));
+  plan->engine->adjoint(per_shot != 0);
+  SWX_HIP(hipEventRecord(e1, c.stream));
+  SWX_HIP(hipStreamSynchronize(c.stream));
+  float ms = 0;
+  SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.ms_adjoint = ms;
+  if (c.ktimer.enabled) c.ktimer.collect(c.ms_kernel, c.n_kernel);
+  plan->engine->copy_grad(grad, per_shot != 0);
 }
 }  // namespace
 
@@ -269,7 +309,9 @@ int swx_plan_create(const swx_plan_desc *desc, swx_plan **out) {
     size_t fr = 0, tot = 0;
     SWX_HIP(hipMemGetInfo(&fr, &tot));
     p->core.budget = d.mem_budget > 0 ? (size_t)d.mem_budget : (size_t)(0.9 * (double)fr);
-    if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 2) {
+    if ((d.physics == SWX_ACOUSTIC_CD || d.physics == SWX_ACOUSTIC_VD) && d.ndim == 1) {
+      p->engine = make_acoustic_1d(p->core);
+    } else if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 2) {
       p->engine = make_acoustic_cd2d(p->core);
     } else if (d.physics == SWX_ACOUSTIC_CD && d.ndim == 3) {
       p->engine = make_acoustic_cd3d(p->core);
@@ -315,7 +357,7 @@ int swx_set_shots(swx_plan *plan, int nshots, const swx_shot_desc *shots) {
                   "shot nt must be in 1..nt_max");
     SWX_REQUIRE(d.physics != SWX_ELASTIC_ISO, SWX_EINVAL, "elastic plans take swx_set_shots_elastic");
     SWX_HIP(hipSetDevice(d.device));
-    plan->core.forward_done = false;
+    forget_shots(plan->core);
     try {
       PointTables tables;
       build_point_tables(tables, d.ndim, d.n, plan->core.esize, nshots, shots, plan->core.stream);
@@ -337,7 +379,7 @@ int swx_set_shots_elastic(swx_plan *plan, int nshots, const swx_elastic_shot_des
     for (int s = 0; s < nshots; s++)
       SWX_REQUIRE(shots[s].nt >= 1 && shots[s].nt <= d.nt_max, SWX_EINVAL, "shot nt must be in 1..nt_max");
     SWX_HIP(hipSetDevice(d.device));
-    plan->core.forward_done = false;
+    forget_shots(plan->core);
     try {
       plan->engine->on_elastic_shots(nshots, shots);
     } catch (...) {
@@ -371,6 +413,7 @@ int swx_forward(swx_plan *plan, void *traces, int save_state) {
     if (traces)
       SWX_HIP(hipMemcpy(traces, c.pts.d_traces.get(), (size_t)c.pts.total_tr * c.esize, hipMemcpyDeviceToHost));
     c.forward_done = save_state != 0;
+    c.traces_valid = true;
   });
 }
 
@@ -383,21 +426,73 @@ int swx_adjoint(swx_plan *plan, const void *adjsrc, void *grad, int per_shot) {
     SWX_HIP(hipSetDevice(c.d.device));
     SWX_HIP(hipMemcpyAsync(c.pts.d_res.get(), adjsrc, (size_t)c.pts.total_tr * c.esize, hipMemcpyHostToDevice,
                            c.stream));
-    hipEvent_t e0, e1;
-    SWX_HIP(hipEventCreate(&e0));
-    SWX_HIP(hipEventCreate(&e1));
-    c.ktimer.reset();
-    SWX_HIP(hipEventRecord(e0, c.stream));
-    plan->engine->adjoint(per_shot != 0);
-    SWX_HIP(hipEventRecord(e1, c.stream));
+    run_adjoint(plan, grad, per_shot);
+  });
+}
+
+int swx_set_snapshots(swx_plan *plan, int snapevery, void *snaps, int64_t bytes) {
+  return guarded([&] {
+    SWX_REQUIRE(plan, SWX_EINVAL, "null plan");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(snapevery >= 0, SWX_EINVAL, "snapevery must be non-negative");
+    c.snap_every = 0;
+    c.snap_host = nullptr;
+    if (snapevery == 0) return;
+    SWX_REQUIRE(c.pts.nshots > 0, SWX_ESTATE, "swx_set_shots must be called before swx_set_snapshots");
+    SWX_REQUIRE(snaps, SWX_EINVAL, "null snapshot buffer");
+    SnapPiece pc[8];
+    const int n = plan->engine->snap_fields(1, 0, pc);
+    SWX_REQUIRE(n > 0, SWX_EUNSUPPORTED, "snapshots are not provided for this physics");
+    int64_t one = 0;
+    for (int k = 0; k < n; k++) one += pc[k].width * pc[k].rows;
+    std::vector<int64_t> off(c.pts.nshots);
+    int64_t tot = 0;
+    for (int s = 0; s < c.pts.nshots; s++) {
+      SWX_REQUIRE(snapevery < c.pts.nt[s], SWX_EINVAL,
+                  "Checkpointing frequency must be smaller than the number of timesteps!");
+      off[s] = tot;
+      tot += (int64_t)(c.pts.nt[s] / snapevery) * one;
+    }
+    SWX_REQUIRE(bytes >= tot, SWX_EINVAL, "snapshot buffer too small: need " + std::to_string(tot) + " bytes");
+    c.snap_one = one;
+    c.snap_off = off;
+    c.snap_host = static_cast<char *>(snaps);
+    c.snap_every = snapevery;
+  });
+}
+
+int swx_set_observed(swx_plan *plan, const void *observed, const double *weights) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && observed, SWX_EINVAL, "null argument");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(c.pts.nshots > 0, SWX_ESTATE, "swx_set_shots must be called before swx_set_observed");
+    SWX_HIP(hipSetDevice(c.d.device));
+    set_observed(c, observed, weights);
+  });
+}
+
+int swx_misfit_l2(swx_plan *plan, double *misfit) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && misfit, SWX_EINVAL, "null argument");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(c.pts.nshots > 0 && c.traces_valid, SWX_ESTATE, "swx_forward must precede swx_misfit_l2");
+    SWX_REQUIRE(c.obs_set, SWX_ESTATE, "swx_set_observed must precede swx_misfit_l2");
+    SWX_HIP(hipSetDevice(c.d.device));
+    l2_adjoint_source(c, misfit);
     SWX_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0;
-    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    c.ms_adjoint = ms;
-    if (c.ktimer.enabled) c.ktimer.collect(c.ms_kernel, c.n_kernel);
-    plan->engine->copy_grad(grad, per_shot != 0);
+  });
+}
+
+int swx_adjoint_l2(swx_plan *plan, void *grad, int per_shot, double *misfit) {
+  return guarded([&] {
+    SWX_REQUIRE(plan && grad, SWX_EINVAL, "null argument");
+    PlanCore &c = plan->core;
+    SWX_REQUIRE(c.d.gradient, SWX_ESTATE, "plan was not created for gradients");
+    SWX_REQUIRE(c.forward_done, SWX_ESTATE, "swx_forward(save_state=1) must precede swx_adjoint_l2");
+    SWX_REQUIRE(c.obs_set, SWX_ESTATE, "swx_set_observed must precede swx_adjoint_l2");
+    SWX_HIP(hipSetDevice(c.d.device));
+    l2_adjoint_source(c, misfit);
+    run_adjoint(plan, grad, per_shot);
   });
 }
 

@@ -16,7 +16,7 @@ from .params import (ARITHMETIC, HARMONIC, CPMLBoundaryConditionParameters, Elas
                      VpRhoAcousticVDMaterialProperties)
 from .shots import (ExternalForceShot, ExternalForceSources, MomentTensor2D, MomentTensorShot, MomentTensorSources,
                     ScalarReceivers, ScalarShot, ScalarSources, VectorReceivers, gaussderivstf, gaussstf, rickerstf)
-from .misfits import L2Misfit
+from .misfits import CCTSMisfit, L2Misfit
 from .points import coeffsinc1d, spread_positions
 from .api import (AcousticCDCPMLWaveSimulation, AcousticVDStaggeredCPMLWaveSimulation, ElasticIsoCPMLWaveSimulation,
                   back_interp, build_wavesim, find_nearest_grid_points, interp_stag, mute_around_points,
@@ -27,7 +27,7 @@ __all__ = [
     "ARITHMETIC", "HARMONIC", "CPMLBoundaryConditionParameters", "GradParameters", "InputParametersAcoustic",
     "ReflectiveBoundaryConditionParameters", "RunParameters", "VpAcousticCDMaterialProperties",
     "VpRhoAcousticVDMaterialProperties", "ScalarReceivers", "ScalarShot", "ScalarSources", "gaussderivstf",
-    "gaussstf", "rickerstf", "L2Misfit", "AcousticCDCPMLWaveSimulation", "AcousticVDStaggeredCPMLWaveSimulation",
+    "gaussstf", "rickerstf", "L2Misfit", "CCTSMisfit", "AcousticCDCPMLWaveSimulation", "AcousticVDStaggeredCPMLWaveSimulation",
     "back_interp", "build_wavesim", "find_nearest_grid_points", "interp_stag", "mute_around_points",
     "run_swforward", "run_swgradient", "swforward", "swgradient", "swmisfit",
     "InputParametersElastic", "ElasticIsoMaterialProperties", "MomentTensor2D", "MomentTensorSources",

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "libswx.so")
+# SWX_LIBRARY selects another build of the library (e.g. libswx_debug.so)
+LIBPATH = os.environ.get("SWX_LIBRARY") or os.path.join(_HERE, "libswx.so")
 
 SWX_OK, SWX_EINVAL, SWX_ENOMEM, SWX_EHIP, SWX_ESTATE, SWX_EUNSUPPORTED = range(6)
 SWX_ACOUSTIC_CD, SWX_ACOUSTIC_VD, SWX_ELASTIC_ISO = 1, 2, 3
@@ -23,6 +24,7 @@ EXPORTS = (
     "swx_abi_version", "swx_last_error", "swx_plan_create", "swx_plan_destroy",
     "swx_set_model", "swx_set_shots", "swx_set_shots_elastic", "swx_forward", "swx_adjoint",
     "swx_plan_get_info", "swx_set_kernel_timing", "swx_forward_noalloc",
+    "swx_set_observed", "swx_misfit_l2", "swx_adjoint_l2", "swx_set_snapshots",
 )
 # Bytes of the static device array swx_forward_noalloc works in.
 NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -97,6 +99,10 @@ def load(path: str | None = None):
     lib.swx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     lib.swx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     lib.swx_forward_noalloc.argtypes = [ctypes.POINTER(PlanDesc), vp, ctypes.POINTER(ShotDesc), vp]
+    lib.swx_set_observed.argtypes = [vp, vp, vp]
+    lib.swx_misfit_l2.argtypes = [vp, vp]
+    lib.swx_adjoint_l2.argtypes = [vp, vp, ctypes.c_int, vp]
+    lib.swx_set_snapshots.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int64]
     for name in EXPORTS:
         getattr(lib, name).restype = ctypes.c_int if name != "swx_last_error" else ctypes.c_char_p
     _lib = lib
@@ -268,6 +274,37 @@ class Plan:
 
     def adjoint(self, adjsrc: np.ndarray, grad: np.ndarray, per_shot: bool):
         check(load().swx_adjoint(self._h, ptr(adjsrc), ptr(grad), int(per_shot)))
+
+    def set_snapshots(self, snapevery: int, buf: np.ndarray | None):
+        """Snapshots of the next forward runs into buf (host, the layout of
+        swx_set_snapshots); snapevery = 0 turns them off."""
+        if snapevery == 0:
+            check(load().swx_set_snapshots(self._h, 0, ctypes.c_void_p(0), 0))
+            return
+        self._snapbuf = buf
+        check(load().swx_set_snapshots(self._h, int(snapevery), ptr(buf), int(buf.nbytes)))
+
+    def set_observed(self, observed: np.ndarray, weights: np.ndarray | None = None):
+        """Observed data of an L2 misfit (traces layout), optional float64
+        weights of the same layout (window mask x diagonal inverse
+        covariance); stays on the device for this batch."""
+        obs = np.ascontiguousarray(observed, dtype=self.dtype)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None:
+            assert w.size == obs.size
+        check(load().swx_set_observed(self._h, ptr(obs), ptr(w)))
+
+    def misfit_l2(self, nshots: int) -> np.ndarray:
+        out = np.zeros(nshots, dtype=np.float64)
+        check(load().swx_misfit_l2(self._h, ptr(out)))
+        return out
+
+    def adjoint_l2(self, grad: np.ndarray, per_shot: bool, nshots: int) -> np.ndarray:
+        """Adjoint with the L2 adjoint source formed on the device; returns
+        the per-shot misfits."""
+        mis = np.zeros(nshots, dtype=np.float64)
+        check(load().swx_adjoint_l2(self._h, ptr(grad), int(per_shot), ptr(mis)))
+        return mis
 
     def info(self) -> PlanInfo:
         inf = PlanInfo()

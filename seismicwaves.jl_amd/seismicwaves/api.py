@@ -127,7 +127,9 @@ class _AcousticCPMLWaveSimulation:
             raise ValueError(f"No backend found for model of type {type(self).__name__} and `parall =` "
                              f"{runparams.parall}. Argument `parall` must be one of the following: {BACKENDS}.")
         if runparams.snapevery is not None:
-            raise NotImplementedError("snapshots (snapevery) are not provided by the mi355x backend")
+            # LinearSnapshotter (snapshotter.jl:12)
+            assert runparams.snapevery < params.ntimesteps, \
+                "Checkpointing frequency must be smaller than the number of timesteps!"
         if runparams.infoevery is not None:
             assert 1 <= runparams.infoevery <= params.ntimesteps, \
                 "Infoevery parameter must be positive and less then nt!"
@@ -145,6 +147,9 @@ class _AcousticCPMLWaveSimulation:
         self._plan = None
         self._plan_shots = 0
         self._fit_batch = 0
+        # L2 misfits (identity or diagonal inverse covariance) are evaluated
+        # on the device; False takes the host path (calcmisfit / ∂χ_∂u)
+        self.device_misfit = True
 
     # -- checks (acou_models.jl:5-38) --
     def check_courant_condition(self, vp):
@@ -231,6 +236,15 @@ class _AcousticCPMLWaveSimulation:
 
     def upload_shots(self, plan, descs):
         plan.set_shots(descs)
+
+    def snapshot_layout(self):
+        """The reference snapshotter's fields (acou_models.jl:177, :427):
+        [(name, [shapes], several)]."""
+        if isinstance(self, AcousticVDStaggeredCPMLWaveSimulation):
+            n = self.gridsize
+            stag = [tuple(n[k] - (1 if k == d else 0) for k in range(len(n))) for d in range(len(n))]
+            return [("pcur", [n], False), ("vcur", stag, True)]
+        return [("pcur", [self.gridsize], False)]
 
     def batches(self, nshots):
         """Yields (shot indices, plan) for consecutive batches of the call's
@@ -472,6 +486,11 @@ class ElasticIsoCPMLWaveSimulation(_AcousticCPMLWaveSimulation):
     def upload_shots(self, plan, descs):
         plan.set_shots_elastic(descs)
 
+    def snapshot_layout(self):
+        """ela_models.jl:390-410: "ucur" = [ux, uz], "σ" = [σxx, σzz, σxz]."""
+        nx, nz = self.gridsize
+        return [("ucur", [(nx - 1, nz), (nx, nz - 1)], True), ("σ", [(nx, nz), (nx, nz), (nx - 1, nz - 1)], True)]
+
     def grad_fields_shapes(self):
         nx, nz = self.gridsize
         return [(nx - 1, nz), (nx, nz - 1), (nx, nz), (nx, nz), (nx - 1, nz - 1)]
@@ -546,16 +565,51 @@ def check_sim_consistency(wavesim, matprop, shots):
 
 
 def run_swforward(wavesim, matprop, shots):
-    """run_swforward! (forward.jl:71-115), batched."""
+    """run_swforward! (forward.jl:71-115), batched.  With
+    RunParameters.snapevery set, returns the snapshots of every shot as the
+    reference does: a list (one entry per shot) of {it: {name: field}},
+    a field being an array or, for several arrays ("vcur", "ucur", "σ"), a
+    list of arrays."""
     check_sim_consistency(wavesim, matprop, shots)
     wavesim.set_matprop(matprop)
+    every = wavesim.runparams.snapevery
+    snaps = []
     for batch, plan in wavesim.batches(len(shots)):
         descs = [wavesim.shot_desc(shots[s]) for s in batch]
         wavesim.upload_shots(plan, descs)
         traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=wavesim.dtype)
-        plan.forward(traces, save_state=False)
+        if every:
+            layout = wavesim.snapshot_layout()
+            one = sum(int(np.prod(sh)) for _, shapes, _ in layout for sh in shapes)
+            buf = np.zeros(sum(d["nt"] // every for d in descs) * one, dtype=wavesim.dtype)
+            plan.set_snapshots(every, buf)
+            try:
+                plan.forward(traces, save_state=False)
+            finally:
+                plan.set_snapshots(0, None)
+            snaps += _split_snapshots(buf, descs, every, layout)
+        else:
+            plan.forward(traces, save_state=False)
         _scatter_traces(traces, descs, [shots[s] for s in batch])
-    return None
+    return snaps if every else None
+
+
+def _split_snapshots(buf, descs, every, layout):
+    out, off = [], 0
+    for d in descs:
+        shot = {}
+        for k in range(1, d["nt"] // every + 1):
+            fields = {}
+            for name, shapes, several in layout:
+                arrs = []
+                for sh in shapes:
+                    n = int(np.prod(sh))
+                    arrs.append(buf[off:off + n].reshape(sh, order="F").copy())
+                    off += n
+                fields[name] = arrs if several else arrs[0]
+            shot[k * every] = fields
+        out.append(shot)
+    return out
 
 
 def _scatter_traces(traces, descs, shots):
@@ -568,7 +622,8 @@ def _scatter_traces(traces, descs, shots):
 
 
 def swforward(params_or_wavesim, matprop, shots, *, runparams: RunParameters | None = None):
-    """swforward! (forward.jl:114-156): fills shot.recs.seismograms."""
+    """swforward! (forward.jl:24-66): fills shot.recs.seismograms; returns
+    the snapshots when RunParameters.snapevery is set."""
     if isinstance(params_or_wavesim, (InputParametersAcoustic, InputParametersElastic)):
         assert runparams is not None
         ws = build_wavesim(params_or_wavesim, matprop, runparams=runparams, gradient=False)
@@ -585,9 +640,41 @@ def swmisfit(params_or_wavesim, matprop, shots, misfit, *, runparams: RunParamet
     return sum(m.calcmisfit(sh.recs) for sh, m in zip(shots, misfit))
 
 
+def _device_l2(m):
+    """An L2 misfit the device can evaluate: identity or diagonal inverse
+    covariance, any windows."""
+    return isinstance(m, L2Misfit) and (m.invcov is None or np.asarray(m.invcov).ndim == 1)
+
+
+def _l2_observed(misfits, descs, T):
+    """Observed data of the batch in the traces layout and, when a misfit
+    has windows or an inverse covariance, the float64 weights mask x C^-1
+    per sample (None when every weight is 1)."""
+    obs, ws = [], []
+    weighted = any(m.windows or m.invcov is not None for m in misfits)
+    for m, d in zip(misfits, descs):
+        o = np.asarray(m.observed)
+        assert o.shape == tuple(d["tshape"]), "observed data and seismograms must have the same shape"
+        obs.append(np.asfortranarray(o, dtype=T).ravel(order="F"))
+        if weighted:
+            nt = o.shape[0]
+            w = np.ones(nt)
+            if m.windows:
+                w = np.zeros(nt)
+                for a, b in m.windows:
+                    w[a - 1:b] = 1.0
+            if m.invcov is not None:
+                w = w * np.asarray(m.invcov, dtype=np.float64)
+            full = np.broadcast_to(w.reshape((nt,) + (1,) * (o.ndim - 1)), o.shape)
+            ws.append(np.asfortranarray(full).ravel(order="F"))
+    return np.concatenate(obs), (np.concatenate(ws) if weighted else None)
+
+
 def run_swgradient(wavesim, matprop, shots, misfit, *, per_shot_gradients: bool = False):
     """run_swgradient! (gradient.jl:93-136), batched: forward with device
-    checkpoints, host adjoint source, adjoint with replay, correlation.
+    checkpoints, adjoint source (on the device for L2 misfits with an
+    identity or diagonal inverse covariance, on the host otherwise), adjoint
+    with replay, correlation.
 
     With per_shot_gradients the result is the list of each shot's gradient
     (muted and chain-ruled as the reference does per shot, gradient.jl:
@@ -610,14 +697,23 @@ def run_swgradient(wavesim, matprop, shots, misfit, *, per_shot_gradients: bool 
         traces = np.zeros(sum(int(np.prod(d["tshape"])) for d in descs), dtype=T)
         plan.forward(traces, save_state=True)
         _scatter_traces(traces, descs, bshots)
-        adj = []
-        for sh, d in zip(bshots, [misfit[s] for s in batch]):
-            adj.append(np.asfortranarray(-d.dchi_du(sh.recs), dtype=T).ravel(order="F"))  # acou_gradient.jl:48
-        adjsrc = np.concatenate(adj).astype(T)
+        bmis = [misfit[s] for s in batch]
         ncopy = len(batch) if per_shot else 1
         size1 = sum(int(np.prod(s)) for s in shapes)
         rawflat = np.zeros(ncopy * size1, dtype=T)
-        plan.adjoint(adjsrc, rawflat, per_shot)
+        if wavesim.device_misfit and all(_device_l2(m) for m in bmis):
+            # L2 adjoint source and misfit formed on the device from the
+            # traces still there (acou_gradient.jl:45-51, L2Misfit.jl:24-95)
+            obs, w = _l2_observed(bmis, descs, T)
+            plan.set_observed(obs, w)
+            shot_mis = plan.adjoint_l2(rawflat, per_shot, len(batch))
+        else:
+            adj = []
+            for sh, d in zip(bshots, bmis):
+                adj.append(np.asfortranarray(-d.dchi_du(sh.recs), dtype=T).ravel(order="F"))  # acou_gradient.jl:48
+            adjsrc = np.concatenate(adj).astype(T)
+            plan.adjoint(adjsrc, rawflat, per_shot)
+            shot_mis = None
         raw = []
         for k in range(ncopy):
             fields, off = [], k * size1
@@ -636,7 +732,10 @@ def run_swgradient(wavesim, matprop, shots, misfit, *, per_shot_gradients: bool 
                 for key in totgrad:
                     totgrad[key] += g[key]
         if gp.compute_misfit:
-            totmisfit += sum(misfit[s].calcmisfit(shots[s].recs) for s in batch)
+            if shot_mis is not None:
+                totmisfit += float(np.sum(shot_mis))
+            else:
+                totmisfit += sum(misfit[s].calcmisfit(shots[s].recs) for s in batch)
     out = shot_grads if per_shot_gradients else totgrad
     return (out, totmisfit) if gp.compute_misfit else out
 

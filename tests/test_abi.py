@@ -14,7 +14,7 @@ HEADER = os.path.join(REPO, "include", "swx.h")
 def declared_symbols():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(swx_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(swx_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_entry_points():

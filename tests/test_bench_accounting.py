@@ -75,3 +75,17 @@ def test_strips_counted_once_per_launch():
     extra = (bench._strip_cells(NS, 10) - bench._strip_cells(NS, 0)) * 2 * E
     assert b1[0][0] - b0[0][0] == b0[0][1] * extra
     assert b1[1][0] - b0[1][0] == b0[1][1] * extra
+
+
+def test_vd_pair_variant_reads_model_once_per_pair():
+    # two shots per block (variant 416): m0, m1x, m1z (3e) are read once per
+    # shot pair and step (the pair steps as long as its longer shot)
+    nts, S = [500, 420, 333], 50
+    one = bench.alg_bytes("vd2d", NS, 0, E, nts, S, variant=(316, 316))
+    rep = [(nt // S) * (S - 1) for nt in nts]
+    for v, fam, saved in (((416, 316), 0, 3 * E * N * (sum(nt + r for nt, r in zip(nts, rep))
+                                                         - max(500 + rep[0], 420 + rep[1]) - (333 + rep[2]))),
+                          ((316, 416), 1, 3 * E * N * (sum(nts) - 500 - 333))):
+        pair = bench.alg_bytes("vd2d", NS, 0, E, nts, S, variant=v)
+        assert one[fam][0] - pair[fam][0] == saved
+        assert one[1 - fam][0] == pair[1 - fam][0]

@@ -120,7 +120,8 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
     """Every forward and adjoint kernel variant the autotuner may pick (2-D
     tiles of height 8/16/32 as interior + other-tile launches or as one
     launch, two steps per launch on tiles of height 8/16/32, z-marching
-    strips of 128/256 columns) gives the same seismograms
+    strips of 128/256 columns, two shots per block on tiles of height 8/16)
+    gives the same seismograms
     and gradients bit for bit, and matches the oracle.  The uneven shots
     (one ends on an odd step) and the odd replay length (39) make the
     two-step launches end with single steps; check_freq = 1 stores every
@@ -132,7 +133,8 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch):
     outs = []
     for v, w, cf in (("8", "8", 40), ("16", "16", 40), ("32", "102", 40), ("102", "104", 40), ("104", "8", 40),
                      ("208", "16", 40), ("216", "8", 40), ("232", "16", 40), ("216", "16", 1), ("208", "8", 7),
-                     ("316", "316", 40), ("308", "308", 9), ("332", "16", 40)):
+                     ("316", "316", 40), ("308", "308", 9), ("416", "16", 40), ("408", "316", 7),
+                     ("332", "16", 40)):
         print(f"[vd2d kernel variants forward {v} adjoint {w} check_freq {cf}]", flush=True)
         monkeypatch.setenv("SWX_VD2D_TILE_FWD", v)
         monkeypatch.setenv("SWX_VD2D_TILE_ADJ", w)
@@ -178,3 +180,23 @@ def test_point_edge_cases(oracle_mod, case):
     assert rel_l2(shots[0].recs.seismograms, sref[0]) <= 1e-10
     for key in ("vp", "rho"):
         assert rel_l2(g[key], gref[key]) <= 1e-10
+
+
+def test_pair_kernel_odd_shot_count(monkeypatch):
+    """Two shots per block with an odd number of shots (the last block has
+    one) and uneven lengths: bitwise the seismograms and gradients of the
+    one-shot-per-block kernel."""
+    p = _problem(np.float64, nts=(300, 257, 281), n=(301, 150))
+    sw.swforward(p["params"], sw.VpRhoAcousticVDMaterialProperties(p["vtrue"], p["rtrue"]), p["shots"],
+                 runparams=sw.RunParameters())
+    mis = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in p["shots"]]
+    mp = sw.VpRhoAcousticVDMaterialProperties(p["v0"], p["r0"])
+    outs = []
+    for v in ("316", "416"):
+        monkeypatch.setenv("SWX_VD2D_TILE_FWD", v)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=40))
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
+    assert np.array_equal(outs[0][0]["vp"], outs[1][0]["vp"])
+    assert np.array_equal(outs[0][0]["rho"], outs[1][0]["rho"])
+    assert all(np.array_equal(a, b) for a, b in zip(outs[0][1], outs[1][1]))

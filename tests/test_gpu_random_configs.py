@@ -8,9 +8,10 @@ coefficients (alpha = pi f0 (1 - x), acou_init_bc.jl:19-30).
 
 The sizes are chosen so that each grid has tiles that take the test-free
 interior paths as well as C-PML, edge and point tiles; the positions put
-some points inside interior tiles.  Sources of the acoustic cases stay one
-cell clear of the outermost rows and columns (with checkpoint replay those
-are where the reference keeps a stale buffer)."""
+some points inside interior tiles.  Acoustic sources and receivers are put
+on the outermost rows and columns half of the time (cells the stencil does
+not update: the CD time step keeps level it-2 there, acoustic2D_xPU.jl:
+122-124)."""
 import numpy as np
 import pytest
 
@@ -35,9 +36,16 @@ def _field(rng, shape, lo, hi, sigma=2.0):
     return lo + (m - m.min()) / max(m.max() - m.min(), 1e-12) * (hi - lo)
 
 
-def _points(rng, k, n, margin):
-    """k positions (m) with every coordinate at least `margin` cells inside."""
-    return np.stack([rng.uniform(margin * H, (ni - 1 - margin) * H, k) for ni in n], 1)
+def _points(rng, k, n, margin, edge=0.0):
+    """k positions (m) with every coordinate at least `margin` cells inside;
+    with probability `edge` a point has one coordinate on the grid's first or
+    last row/column instead."""
+    pts = np.stack([rng.uniform(margin * H, (ni - 1 - margin) * H, k) for ni in n], 1)
+    for q in range(k):
+        if rng.random() < edge:
+            ax = int(rng.integers(len(n)))
+            pts[q, ax] = 0.0 if rng.random() < 0.5 else (n[ax] - 1) * H
+    return pts
 
 
 def _config(rng, nlo, nhi, halos):
@@ -61,9 +69,11 @@ def _scalar_shots(rng, n, nts, dtype):
     shots, osh = [], []
     for k, nt in enumerate(nts):
         nsrc = int(rng.integers(1, 3))
-        src = _points(rng, nsrc, n, 1.0)
+        src = _points(rng, nsrc, n, 0.0, edge=0.5)
         stf = np.stack([tf[:nt] * (1.0 + 0.3 * k) for k in range(nsrc)], 1)
-        rec = _points(rng, int(rng.integers(3, 12)), n, 0.0)
+        # one receiver on the first source's cell: every shot records a wave
+        # (a short shot's wave may not reach any other receiver)
+        rec = np.vstack([_points(rng, int(rng.integers(3, 12)), n, 0.0, edge=0.5), src[:1]])
         shots.append(sw.ScalarShot(sw.ScalarSources(src, stf.astype(dtype), _domfreq(k)),
                                    sw.ScalarReceivers(rec, nt, dtype=dtype)))
         osh.append(dict(src_pos=src, srctf=stf, f0=_domfreq(k), rec_pos=rec, nt=nt))
