@@ -310,7 +310,9 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             # fields read and written (10e) = 27e.  C-PML: the 8 psi arrays read
             # and written once per step.  A variant with two launches per step
             # (stress, displacement) is priced at the same bytes per step.
-            march_f, march_a = variant[0] > 100, variant[1] > 100
+            # (one launch per step: the z-marching variants > 100 and the
+            # fused tile step 2)
+            march_f, march_a = variant[0] > 100 or variant[0] == 2, variant[1] > 100 or variant[1] == 2
             last = nt - (nt // S) * S
             rep = (nt // S) * (S - 1)
             est = _el_strip_cells(n, halo) * 2 * e
@@ -387,6 +389,8 @@ def kernel_names(physics, variant):
         def name(adj, v):
             if v > 100:
                 return f"el_march_kernel<float, {v - 100}, {'true' if adj else 'false'}>"
+            if v == 2:   # stresses in LDS, one launch per step
+                return f"el_tile_kernel<float, {'true' if adj else 'false'}>"
             tail = " (adjoint)" if adj else " (forward)"
             if v == 1:   # interior and other blocks in one launch per kernel
                 return "el_stress_kernel<float, true> + el_disp_kernel<float, true>" + tail
@@ -601,6 +605,26 @@ def cpu_baseline_elastic(wl, cores, nt_sample=300):
 
 
 # ---------------------------------------------------------------------------
+def hbm_triad(bytes_each=4 << 30, reps=20):
+    """The device's practical HBM ceiling: a STREAM triad a = b + s c on
+    float4 vectors (seismicwaves.jl_amd/probe/hbm_triad.hip), 3 arrays of
+    `bytes_each`, best and mean GB/s over `reps` launches of 8 blocks per CU
+    and of 1 block per 4 KiB."""
+    import ctypes
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "seismicwaves.jl_amd", "probe",
+                        "libswx_probe.so")
+    lib = ctypes.CDLL(path)
+    lib.swx_probe_hbm_triad.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for name, blocks in (("8_blocks_per_cu", 0), ("1_block_per_4KiB", bytes_each // 4096)):
+        r = (ctypes.c_double * 2)()
+        rc = lib.swx_probe_hbm_triad(bytes_each, reps, int(blocks), r)
+        if rc != 0:
+            raise RuntimeError(f"swx_probe_hbm_triad failed ({rc})")
+        out[name] = {"best_GBs": round(r[0], 1), "mean_GBs": round(r[1], 1)}
+    return {"probe": "hbm_triad", "bytes_per_array": bytes_each, "reps": reps, "peak_GBs": 8000.0, **out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -616,7 +640,12 @@ def main():
                     help="stop after the timed steps (for counter-collection runs, where every dispatch is "
                          "instrumented): no per-launch timing pass, no roofline")
     ap.add_argument("--one-shot-sample", type=int, default=2, help="shots timed on the 1-shot path")
+    ap.add_argument("--hbm-triad", action="store_true",
+                    help="only measure the HBM ceiling: a STREAM triad on 3 x 4 GiB (probe/libswx_probe.so)")
     args = ap.parse_args()
+    if args.hbm_triad:
+        print(json.dumps(hbm_triad()))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU: shots would be sharded across GPUs with one "
                                    "gradient all-reduce, which is described but not run here",

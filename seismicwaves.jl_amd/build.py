@@ -62,6 +62,25 @@ def build(verbose: bool = False, jobs: int = 4, debug: bool = False) -> str:
     return out
 
 
+PROBE_SRC = os.path.join(HERE, "probe", "hbm_triad.hip")
+PROBE_OUT = os.path.join(HERE, "probe", "libswx_probe.so")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """Builds the HBM bandwidth probe (probe/hbm_triad.hip) used by
+    bench.py --hbm-triad; it is not part of libswx."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = PROBE_OUT + ".tmp"
+    cmd = [hipcc, *HIPFLAGS, "-shared", "-o", tmp, PROBE_SRC]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed on hbm_triad.hip:\n" + r.stdout)
+    os.replace(tmp, PROBE_OUT)
+    return PROBE_OUT
+
+
 def _wait(item):
     src, p = item
     out, _ = p.communicate()
@@ -73,3 +92,5 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv, debug="--debug" in sys.argv))
+    if "--debug" not in sys.argv:
+        print(build_probe(verbose="-v" in sys.argv))

@@ -1067,6 +1067,317 @@ __global__ __launch_bounds__(64 * W) void el_march_kernel(ElArgs<T> a) {
   }
 }
 
+// Rows of displacement cells per block of the fused tile step (64 columns).
+template <typename T> struct ElTile;
+template <> struct ElTile<float> { static constexpr int rows = 16; };
+template <> struct ElTile<double> { static constexpr int rows = 8; };
+
+// Region of a tile held in LDS and read through the reference's 1-based
+// (i, j): element (i, j) at [(j - j0) * W + (i - i0)].
+template <typename T, int W>
+struct TileA {
+  const T *b;
+  int i0, j0;
+  __device__ __forceinline__ T operator()(int i, int j) const {
+    SWX_DASSERT(i - i0 >= 0 && i - i0 < W && j - j0 >= 0);
+    return b[(j - j0) * W + (i - i0)];
+  }
+};
+
+// One step, stresses and displacements in one launch, on a tile of 64 x R
+// displacement cells: forward or replay (ADJ = false), or adjoint (ADJ =
+// true: residual injection, and the correlation of adjoint step corr_step
+// on the tile's own stress cells, as el_stress_kernel does).  ucur is staged in LDS on the
+// tile plus 4 cells on every side; the stresses are computed into LDS on
+// the tile plus the 2 cells the displacement stencils reach on every side
+// (recomputed by neighbouring tiles, whose C-PML memory variables only the
+// owning tile writes), then the displacements of the tile are stepped.  The
+// stresses never go to HBM, except the never-updated edge cells that
+// accumulate moment-tensor injections (el_edge_inject_kernel, run first).
+// The expressions are those of el_stress_kernel and el_disp_kernel in the
+// same operand order, so the results are bitwise equal.
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
+  constexpr int R = ElTile<T>::rows;
+  constexpr int UW = TX + 8, UH = R + 8;   // ucur: columns I1-4 .. I1+TX+3, rows J1-4 .. J1+R+3
+  constexpr int SW = TX + 4, SH = R + 4;   // stresses: columns I1-2 .. I1+TX+1, rows J1-2 .. J1+R+1
+  __shared__ T lux[UH * UW], luz[UH * UW];
+  __shared__ T lsxx[SH * SW], lszz[SH * SW], lsxz[SH * SW];
+  const int b = xcd_tile_el(blockIdx.x, gridDim.x);
+  const int bx = b % a.tbx, t = b / a.tbx, by = t % a.tbz, s = t / a.tbz;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || (!ADJ && nts <= a.act_min)) return;
+  const bool corr = ADJ && a.corr_step > 0 && a.corr_step <= nts && a.corr_step <= a.corr_lim;
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const bool ft = a.freetop != 0;
+  const int64_t N = a.N;
+  const int I1 = bx * TX + 1, J1 = by * R + 1;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const T *ucx = a.ucur + (int64_t)s * 2 * N, *ucz = ucx + N;
+  T *un = a.unew + (int64_t)s * 2 * N;
+  const T *pso = a.psi_o + (int64_t)s * a.psi_stride;
+  T *psn = a.psi_n + (int64_t)s * a.psi_stride;
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0), cz = axis_coef(cb, h, 1);
+  // Interior tile: every stress cell of the tile + 2 and every displacement
+  // cell takes the plain 4th-order branch of each derivative helper, lies
+  // outside every C-PML strip, and no point of the shot lies on its rows.
+  bool lean = I1 - 2 >= max(3, h + 2) && I1 + TX + 1 <= min(nx - 2, nx - h - 2) && J1 - 2 >= max(3, h + 2) &&
+              J1 + R + 1 <= min(nz - 2, nz - h - 2);
+  if (lean && a.kind != 0) {
+    const int64_t o = (int64_t)s * (a.nrows + 1);
+    const int r0 = J1 - 3, r1 = J1 + R + 1;   // rows J1-2 .. J1+R+1 (0-based r0 .. r1-1)
+    lean = a.f0.rows[o + r0] == a.f0.rows[o + r1] && a.f1.rows[o + r0] == a.f1.rows[o + r1];
+  }
+
+  // ucur on the tile + 4 (zero outside the ux / uz grids)
+  for (int q = tid; q < UW * UH; q += TX * TZ) {
+    const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    lux[q] = (x >= 1 && x <= nx - 1 && j >= 1 && j <= nz) ? ucx[c] : T(0);
+    luz[q] = (x >= 1 && x <= nx && j >= 1 && j <= nz - 1) ? ucz[c] : T(0);
+  }
+  __syncthreads();
+  const TileA<T, UW> UX{lux, I1 - 4, J1 - 4}, UZ{luz, I1 - 4, J1 - 4};
+
+  // stresses on the tile + 2 (update_sxx_szz!, update_sxz!, :39-79, and the
+  // moment-tensor injection, :81-94)
+  for (int q = tid; q < SW * SH; q += TX * TZ) {
+    const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
+    const bool own = x >= I1 && x < I1 + TX && j >= J1 && j < J1 + R;
+    T vxx = 0, vzz = 0, vxz = 0;
+    T dux = 0, duz = 0, duzx = 0, duxz = 0;   // raw derivatives (adjoint correlation)
+    bool nrm = false, shr = false;
+    if (lean) {
+      // the interior branches of the helpers, in the same operand order
+      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+      const T l = a.lam[c], m = a.mu[c], mhh = a.muhh[c];
+      dux = d4<T>(UX(x - 2, j), UX(x - 1, j), UX(x, j), UX(x + 1, j), a.idx);
+      duz = d4<T>(UZ(x, j - 2), UZ(x, j - 1), UZ(x, j), UZ(x, j + 1), a.idz);
+      const T dxc = dux, dzc = duz;
+      vxx = (l + T(2) * m) * dxc + l * dzc;
+      vzz = l * dxc + (l + T(2) * m) * dzc;
+      duzx = d4<T>(UZ(x - 1, j), UZ(x, j), UZ(x + 1, j), UZ(x + 2, j), a.idx);
+      duxz = d4<T>(UX(x, j - 1), UX(x, j), UX(x, j + 1), UX(x, j + 2), a.idz);
+      const T c1 = duzx, c2 = duxz;
+      vxz = mhh * (c1 + c2);
+      nrm = shr = true;
+    } else if (x >= 1 && x <= nx && j >= 1 && j <= nz) {
+      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+      nrm = x >= 2 && x <= nx - 1 && j >= (ft ? 1 : 2) && j <= nz - 1;
+      if (nrm) {
+        const T l = a.lam[c], m = a.mu[c];
+        dux = d_ux_dx<T>(UX, x, j, nx, a.idx);
+        duz = d_uz_dz<T>(UX, UZ, l, m, x, j, nx, nz, ft, a.idx, a.idz);
+        T dxc = dux, dzc = duz;
+        int k = strip_half(x - 1, nx - 1, h);
+        if (k) {
+          const int64_t qq = a.psioff[4] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
+          const T pn = cx.b(k - 1) * pso[qq] + cx.a(k - 1) * dux;
+          if (own) psn[qq] = pn;
+          dxc = dux + pn;
+        }
+        k = strip_half(j - 1, nz - 1, h);
+        if (k) {
+          const int64_t qq = a.psioff[7] + (x - 1) + (int64_t)(k - 1) * nx;
+          const T pn = cz.b(k - 1) * pso[qq] + cz.a(k - 1) * duz;
+          if (own) psn[qq] = pn;
+          dzc = duz + pn;
+        }
+        vxx = (l + T(2) * m) * dxc + l * dzc;
+        vzz = j == 1 ? T(0) : l * dxc + (l + T(2) * m) * dzc;
+        if (a.kind == 1) {
+          const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
+          if (u >= 0) {
+            const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+            const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+            for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++) {
+              const int p = a.f0.epoint[e];
+              const T f = tf[(int64_t)nts * p];
+              vxx += M[3 * p + 0] * a.f0.ecoef[e] * f;
+              vzz += M[3 * p + 1] * a.f0.ecoef[e] * f;
+            }
+          }
+        }
+      } else if (a.kind == 1) {
+        // never updated: the injections accumulated so far (this step's
+        // included, el_edge_inject_kernel)
+        const T *sg = a.sig + (int64_t)s * 3 * N;
+        vxx = sg[c];
+        vzz = sg[N + c];
+      }
+      shr = x <= nx - 1 && j <= nz - 1;
+      if (shr) {
+        duzx = d_uz_dx<T>(UZ, x, j, nx, a.idx);
+        duxz = d_ux_dz<T>(UX, x, j, nz, ft, a.idz);
+        T c1 = duzx, c2 = duxz;
+        int k = strip_full(x, nx, h);
+        if (k) {
+          const int64_t qq = a.psioff[5] + (k - 1) + (int64_t)(j - 1) * (2 * h);
+          const T pn = cx.bh(k - 1) * pso[qq] + cx.ah(k - 1) * duzx;
+          if (own) psn[qq] = pn;
+          c1 = duzx + pn;
+        }
+        k = strip_full(j, nz, h);
+        if (k) {
+          const int64_t qq = a.psioff[6] + (x - 1) + (int64_t)(k - 1) * (nx - 1);
+          const T pn = cz.bh(k - 1) * pso[qq] + cz.ah(k - 1) * duxz;
+          if (own) psn[qq] = pn;
+          c2 = duxz + pn;
+        }
+        vxz = a.muhh[c] * (c1 + c2);
+        if (a.kind == 1) {
+          const int u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, j - 1, c);
+          if (u >= 0) {
+            const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+            const T *M = a.mom + 3 * (int64_t)a.off_mom[s];
+            for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++) {
+              const int p = a.f1.epoint[e];
+              vxz += M[3 * p + 2] * a.f1.ecoef[e] * tf[(int64_t)nts * p];
+            }
+          }
+        }
+      }
+    }
+    lsxx[q] = vxx;
+    lszz[q] = vzz;
+    lsxz[q] = vxz;
+    if (ADJ && corr && own && x <= nx && j <= nz) {
+      // correlate_gradients! of adjoint step corr_step (correlate_gradient_
+      // xPU.jl:1-82) on the own stress cells, as el_stress_kernel
+      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+      const int sl = a.corr_step - a.cbase + 1;   // slot of u^{corr_step}
+      const T *h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
+      const T *h1 = h0 + a.hslot;
+      const T *h2 = h1 + a.hslot;
+      T *g = a.grad + (int64_t)s * a.gstride;
+      if (x <= nx - 1) {
+        const T v = UX(x, j) * (h0[c] - T(2) * h1[c] + h2[c]) * a.idt2;
+        g[c] += (j == 1 && ft) ? v / T(2) : v;
+      }
+      if (j <= nz - 1) g[N + c] += UZ(x, j) * (h0[N + c] - T(2) * h1[N + c] + h2[N + c]) * a.idt2;
+      const F2<T> fx{h1, nx}, fz{h1 + N, nx};
+      if (nrm) {
+        const T exx = d_ux_dx(fx, x, j, nx, a.idx);
+        const T ezz = d_uz_dz(fx, fz, a.lam[c], a.mu[c], x, j, nx, nz, ft, a.idx, a.idz);
+        const T dv = exx + ezz, dva = dux + duz;
+        if (j == 1 && ft) {
+          g[2 * N + c] += dv * dva / T(2);
+          g[3 * N + c] += exx * dux + ezz * duz;
+        } else {
+          g[2 * N + c] += dv * dva;
+          g[3 * N + c] += T(2) * (exx * dux + ezz * duz);
+        }
+      }
+      if (shr) {
+        const T exz = (d_uz_dx(fz, x, j, nx, a.idx) + d_ux_dz(fx, x, j, nz, ft, a.idz)) / T(2);
+        const T exza = (duzx + duxz) / T(2);
+        g[4 * N + c] += T(2) * (exz * exza + exz * exza);
+      }
+    }
+  }
+  __syncthreads();
+  const TileA<T, SW> SXX{lsxx, I1 - 2, J1 - 2}, SZZ{lszz, I1 - 2, J1 - 2}, SXZ{lsxz, I1 - 2, J1 - 2};
+
+  // displacements of the tile (update_ux!, update_uz!, :1-37; force
+  // injection :96-106; forward history)
+  const int i = I1 + threadIdx.x;
+  if (i > nx) return;
+  const int sb = a.hmode == 1 ? (nts / a.S) * a.S : a.cbase;
+  for (int rr = threadIdx.y; rr < R; rr += TZ) {
+    const int j = J1 + rr;
+    if (j > nz) break;
+    const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+    const bool hx = i <= nx - 1, hz = j <= nz - 1;
+    T oldx = 0, oldz = 0, curx = 0, curz = 0, nux = 0, nuz = 0;
+    if (lean) {
+      oldx = un[c];
+      oldz = un[N + c];
+      curx = UX(i, j);
+      curz = UZ(i, j);
+      const T d1x = d4<T>(SXX(i - 1, j), SXX(i, j), SXX(i + 1, j), SXX(i + 2, j), a.idx);
+      const T d2x = d4<T>(SXZ(i, j - 2), SXZ(i, j - 1), SXZ(i, j), SXZ(i, j + 1), a.idz);
+      nux = T(2) * curx - oldx + a.rdx[c] * (d1x + d2x);
+      const T d1z = d4<T>(SXZ(i - 2, j), SXZ(i - 1, j), SXZ(i, j), SXZ(i + 1, j), a.idx);
+      const T d2z = d4<T>(SZZ(i, j - 1), SZZ(i, j), SZZ(i, j + 1), SZZ(i, j + 2), a.idz);
+      nuz = T(2) * curz - oldz + a.rdz[c] * (d1z + d2z);
+    }
+    if (hx && !lean) {
+      const T d1 = d_sxx_dx<T>(SXX, i, j, nx, a.idx);
+      const T d2 = d_sxz_dz<T>(SXZ, i, j, nz, ft, a.idz);
+      T c1 = d1, c2 = d2;
+      int k = strip_full(i, nx, h);
+      if (k) {
+        const int64_t q = a.psioff[0] + (k - 1) + (int64_t)(j - 1) * (2 * h);
+        const T pn = cx.bh(k - 1) * pso[q] + cx.ah(k - 1) * d1;
+        psn[q] = pn;
+        c1 = d1 + pn;
+      }
+      k = strip_half(j - 1, nz - 1, h);
+      if (k) {
+        const int64_t q = a.psioff[3] + (i - 1) + (int64_t)(k - 1) * (nx - 1);
+        const T pn = cz.b(k - 1) * pso[q] + cz.a(k - 1) * d2;
+        psn[q] = pn;
+        c2 = d2 + pn;
+      }
+      oldx = un[c];
+      curx = UX(i, j);
+      nux = T(2) * curx - oldx + a.rdx[c] * (c1 + c2);
+    }
+    if (hz && !lean) {
+      const T d1 = d_sxz_dx<T>(SXZ, i, j, nx, a.idx);
+      const T d2 = d_szz_dz<T>(SZZ, i, j, nz, ft, a.idz);
+      T c1 = d1, c2 = d2;
+      int k = strip_half(i - 1, nx - 1, h);
+      if (k) {
+        const int64_t q = a.psioff[1] + (k - 1) + (int64_t)(j - 1) * (2 * h + 2);
+        const T pn = cx.b(k - 1) * pso[q] + cx.a(k - 1) * d1;
+        psn[q] = pn;
+        c1 = d1 + pn;
+      }
+      k = strip_full(j, nz, h);
+      if (k) {
+        const int64_t q = a.psioff[2] + (i - 1) + (int64_t)(k - 1) * nx;
+        const T pn = cz.bh(k - 1) * pso[q] + cz.ah(k - 1) * d2;
+        psn[q] = pn;
+        c2 = d2 + pn;
+      }
+      oldz = un[N + c];
+      curz = UZ(i, j);
+      nuz = T(2) * curz - oldz + a.rdz[c] * (c1 + c2);
+    }
+    if (a.kind == 2 && !lean) {
+      const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
+      if (hx) {
+        const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
+        if (u >= 0)
+          for (int e = a.f0.eptr[u]; e < a.f0.eptr[u + 1]; e++)
+            nux += a.f0.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f0.epoint[e])] * a.rdx[c];
+      }
+      if (hz) {
+        const int u = find_point(a.f1.rows, a.f1.cell, a.nrows, s, j - 1, c);
+        if (u >= 0)
+          for (int e = a.f1.eptr[u]; e < a.f1.eptr[u + 1]; e++)
+            nuz += a.f1.ecoef[e] * tf[(int64_t)nts * (a.ncomp * a.f1.epoint[e] + 1)] * a.rdz[c];
+      }
+    }
+    if (hx) un[c] = nux;
+    if (hz) un[N + c] = nuz;
+    if (a.hmode && a.it > sb) {
+      T *hs = a.hist + (int64_t)s * 2 * N + c;
+      const int sl = a.it - sb + 1;
+      hs[(int64_t)sl * a.hslot] = nux;
+      hs[(int64_t)sl * a.hslot + N] = nuz;
+      if (a.hmode == 1 && a.it == sb + 1) {
+        hs[0] = oldx;
+        hs[N] = oldz;
+        hs[a.hslot] = curx;
+        hs[a.hslot + N] = curz;
+      }
+    }
+  }
+}
+
 // traces[it, comp, r] = sum over taps of coef * u (record_receivers2D_* and
 // the per-receiver reduction, elastic2D_iso_xPU.jl:108-118, :216-231).
 template <typename T>
@@ -1343,7 +1654,7 @@ class El2dEngine final : public Engine {
     SWX_REQUIRE(model_set_, SWX_ESTATE, "swx_set_model must be called before running");
     if (var_fwd_ < 0) {
       // tuning launches run on the state before it is cleared below
-      var_fwd_ = tune("SWX_EL2D_VARIANT_FWD", [&](int v) { step_fwd_v(v, 1, 0, 0, 0, nullptr); });
+      var_fwd_ = tune("SWX_EL2D_VARIANT_FWD", [&](int v) { step_fwd_v(v, 1, 0, 0, 0, nullptr); }, true);
     }
     zero_forward_state(true);
     if (save) SWX_HIP(hipMemsetAsync(hist_.get(), 0, hist_.bytes(), c_.stream));
@@ -1378,7 +1689,7 @@ class El2dEngine final : public Engine {
   void adjoint(bool per_shot) override {
     if (var_adj_ < 0) {
       const int cbt = (int)(ceil_div(ntmax_, S_) - 1) * S_;
-      var_adj_ = tune("SWX_EL2D_VARIANT_ADJ", [&](int v) { step_adj_v(v, ntmax_, cbt); });
+      var_adj_ = tune("SWX_EL2D_VARIANT_ADJ", [&](int v) { step_adj_v(v, ntmax_, cbt); }, false);
     }
     SWX_HIP(hipMemsetAsync(au_.get(), 0, au_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(asig_.get(), 0, asig_.bytes(), c_.stream));
@@ -1548,8 +1859,9 @@ class El2dEngine final : public Engine {
   }
 
   void step_fwd(int it, int act_min, int hmode, int cbase, T *traces) {
-    // with snapshots the stresses must be in HBM: no z-marching variant
-    step_fwd_v(c_.snap_every > 0 && var_fwd_ > 100 ? 1 : var_fwd_, it, act_min, hmode, cbase, traces);
+    // with snapshots the stresses must be in HBM: no fused variant
+    const bool fused = var_fwd_ == 2 || var_fwd_ > 100;
+    step_fwd_v(c_.snap_every > 0 && fused ? 1 : var_fwd_, it, act_min, hmode, cbase, traces);
   }
   void step_adj(int it, int cbase) { step_adj_v(var_adj_, it, cbase); }
 
@@ -1575,44 +1887,28 @@ class El2dEngine final : public Engine {
 
   // Variants: 0 = stress and displacement steps each as an interior-block
   // launch plus an other-block launch, 1 = the same steps with both kinds of
-  // blocks in one launch each, 102/104 = one z-marching launch on
-  // 128/256-column strips.  The tuner times 0 and 1 (and the z-marching
-  // variants when kMarchInTuner is set); the environment can pin any.
+  // blocks in one launch each, 2 = both steps in one launch on 64 x R tiles
+  // with the stresses in LDS, 102/104 = one z-marching launch on 128/256-
+  // column strips.  The tuner times 0, 1 and 2, and the z-marching variants
+  // when kMarchInTuner is set; the environment can pin any.
   static constexpr bool kMarchInTuner = false;
   template <typename F>
-  int tune(const char *env, F step) {
-    std::vector<int> cands = {0, 1};
+  int tune(const char *env, F step, bool forward) {
+    (void)forward;
+    std::vector<int> cands = {0, 1, 2};
     if (kMarchInTuner) {
       cands.push_back(102);
       cands.push_back(104);
     }
     if (const char *v = std::getenv(env)) {
       const int t = std::atoi(v);
-      if (t == 0 || t == 1 || t == 102 || t == 104) return t;
+      if (t == 0 || t == 1 || t == 2 || t == 102 || t == 104) return t;
     }
-    if (cands.size() == 1) return cands[0];
-    hipEvent_t e0, e1;
-    SWX_HIP(hipEventCreate(&e0));
-    SWX_HIP(hipEventCreate(&e1));
-    int best = cands.front();
-    float bestms = 1e30f;
-    for (int c : cands) {
-      step(c);
-      SWX_HIP(hipEventRecord(e0, c_.stream));
-      for (int r = 0; r < 3; r++) step(c);
-      SWX_HIP(hipEventRecord(e1, c_.stream));
-      SWX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < bestms) { bestms = ms; best = c; }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return best;
+    return cands[fastest_candidate(c_.stream, (int)cands.size(), [&](int k) { step(cands[k]); })];
   }
 
   void step_fwd_v(int v, int it, int act_min, int hmode, int cbase, T *traces) {
-    if (v > 100) {
+    if (v == 2 || v > 100) {
       ElArgs<T> a = base_args(it);
       a.ucur = U(u_, it - 1);
       a.unew = U(u_, it);
@@ -1632,7 +1928,17 @@ class El2dEngine final : public Engine {
         el_edge_inject_kernel<T><<<g, 64, 0, c_.stream>>>(a);
         SWX_HIP(hipGetLastError());
       }
-      launch_march<false>(a, v, 0);
+      if (v == 2) {
+        a.tbx = (int)ceil_div(nx_, TX);
+        a.tbz = (int)ceil_div(nz_, ElTile<T>::rows);
+        const unsigned g = (unsigned)((int64_t)a.tbx * a.tbz * nsh_);
+        c_.ktimer.begin(c_.stream);
+        el_tile_kernel<T, false><<<g, dim3(TX, TZ), 0, c_.stream>>>(a);
+        c_.ktimer.end(c_.stream, 0);
+        SWX_HIP(hipGetLastError());
+      } else {
+        launch_march<false>(a, v, 0);
+      }
       record(it, traces);
       return;
     }
@@ -1667,7 +1973,7 @@ class El2dEngine final : public Engine {
   }
 
   void step_adj_v(int v, int it, int cbase) {
-    if (v > 100) {
+    if (v == 2 || v > 100) {
       ElArgs<T> a = base_args(it);
       a.ucur = U(au_, it + 1);
       a.unew = U(au_, it);
@@ -1683,7 +1989,17 @@ class El2dEngine final : public Engine {
       a.tf = c_.pts.d_res.as<T>();
       a.off_tf = c_.pts.d_off_tr.as<int64_t>();
       a.ncomp = 2;
-      launch_march<true>(a, v, 1);
+      if (v == 2) {
+        a.tbx = (int)ceil_div(nx_, TX);
+        a.tbz = (int)ceil_div(nz_, ElTile<T>::rows);
+        const unsigned g = (unsigned)((int64_t)a.tbx * a.tbz * nsh_);
+        c_.ktimer.begin(c_.stream);
+        el_tile_kernel<T, true><<<g, dim3(TX, TZ), 0, c_.stream>>>(a);
+        c_.ktimer.end(c_.stream, 1);
+        SWX_HIP(hipGetLastError());
+      } else {
+        launch_march<true>(a, v, 1);
+      }
       return;
     }
     ElArgs<T> a = base_args(it);

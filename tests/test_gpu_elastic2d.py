@@ -123,10 +123,15 @@ def test_checkpoint_interval_bitwise_and_mute(oracle_mod):
         assert rel_l2(gm[k], gref[k]) <= 1e-10
 
 
-def test_point_edge_cases(oracle_mod):
+@pytest.mark.parametrize("fwd_variant", ["1", "2"])
+def test_point_edge_cases(oracle_mod, monkeypatch, fwd_variant):
     """Sources next to the grid edges (folded and mirrored taps, taps on
     never-updated stress cells), colocated receivers, receivers on the free
-    surface and on the last rows, nearest-point (non-sinc) mode."""
+    surface and on the last rows, nearest-point (non-sinc) mode; with the
+    two-launch steps and with the fused tile steps (stresses in LDS, edge
+    stress cells accumulating in HBM)."""
+    monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", fwd_variant)
+    monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", fwd_variant)
     n, h, dt, nt, f0, halo = (64, 50), (10.0, 10.0), 1e-3, 220, 10.0, 6
     lam, mu, rho = _media(n, 11)
     lam0, mu0, rho0 = [np.full(n, float(a.mean())) for a in (lam, mu, rho)]
@@ -162,7 +167,8 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     stress/displacement step as interior + other-block launches or as one
     launch each; with the XCD-aware tile map, with the C-PML/edge tiles
     first or with the round-robin block order; z-marching strips of 128/256
-    columns) gives the same seismograms and gradients bit for bit, on a grid
+    columns; the fused tile step with the stresses in LDS) gives the
+    same seismograms and gradients bit for bit, on a grid
     that needs several strips and a partial last one, and matches the
     oracle."""
     p = _problem(np.float64, kind=kind, nts=(200, 170), n=(301, 96), freetop=freetop)
@@ -172,7 +178,8 @@ def test_kernel_variants_agree_bitwise(oracle_mod, monkeypatch, kind, freetop):
     outs = []
     for v, w, xcd, order in (("0", "0", "1", "1"), ("1", "1", "1", "1"), ("1", "1", "1", "0"), ("0", "0", "1", "0"),
                              ("1", "1", "0", "0"), ("0", "0", "0", "0"), ("102", "104", "1", "1"),
-                             ("104", "102", "1", "1"), ("0", "104", "1", "1"), ("1", "0", "1", "1")):
+                             ("104", "102", "1", "1"), ("0", "104", "1", "1"), ("1", "0", "1", "1"),
+                             ("2", "1", "1", "1"), ("2", "2", "0", "0"), ("1", "2", "1", "1")):
         print(f"[el2d kernel variants {kind} forward {v} adjoint {w} xcd map {xcd} frame first {order}]", flush=True)
         monkeypatch.setenv("SWX_EL2D_VARIANT_FWD", v)
         monkeypatch.setenv("SWX_EL2D_VARIANT_ADJ", w)
