@@ -1084,6 +1084,149 @@ struct TileA {
   }
 };
 
+// correlate_gradients! of adjoint step corr_step (correlate_gradient_xPU.jl:
+// 1-82) at stress cell (x, j) of shot s, as el_stress_kernel: ux, uz the
+// adjoint field there, dux .. duxz its raw derivatives.
+template <typename T>
+__device__ __forceinline__ void el_tile_correlate(const ElArgs<T> &a, int s, int x, int j, T ux, T uz, bool nrm,
+                                                  bool shr, T dux, T duz, T duzx, T duxz) {
+  const int nx = a.nx, nz = a.nz;
+  const bool ft = a.freetop != 0;
+  const int64_t N = a.N;
+  const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+  const int sl = a.corr_step - a.cbase + 1;   // slot of u^{corr_step}
+  const T *h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
+  const T *h1 = h0 + a.hslot;
+  const T *h2 = h1 + a.hslot;
+  T *g = a.grad + (int64_t)s * a.gstride;
+  if (x <= nx - 1) {
+    const T v = ux * (h0[c] - T(2) * h1[c] + h2[c]) * a.idt2;
+    g[c] += (j == 1 && ft) ? v / T(2) : v;
+  }
+  if (j <= nz - 1) g[N + c] += uz * (h0[N + c] - T(2) * h1[N + c] + h2[N + c]) * a.idt2;
+  const F2<T> fx{h1, nx}, fz{h1 + N, nx};
+  if (nrm) {
+    const T exx = d_ux_dx(fx, x, j, nx, a.idx);
+    const T ezz = d_uz_dz(fx, fz, a.lam[c], a.mu[c], x, j, nx, nz, ft, a.idx, a.idz);
+    const T dv = exx + ezz, dva = dux + duz;
+    if (j == 1 && ft) {
+      g[2 * N + c] += dv * dva / T(2);
+      g[3 * N + c] += exx * dux + ezz * duz;
+    } else {
+      g[2 * N + c] += dv * dva;
+      g[3 * N + c] += T(2) * (exx * dux + ezz * duz);
+    }
+  }
+  if (shr) {
+    const T exz = (d_uz_dx(fz, x, j, nx, a.idx) + d_ux_dz(fx, x, j, nz, ft, a.idz)) / T(2);
+    const T exza = (duzx + duxz) / T(2);
+    g[4 * N + c] += T(2) * (exz * exza + exz * exza);
+  }
+}
+
+// The step of an interior tile (no boundary branch of any derivative
+// helper, no C-PML strip and no point in reach): every global load of the
+// tile (model factors of the stress cells, uold and dt^2/rho of the
+// displacement cells, ucur) is issued before the first barrier, then the
+// stresses and the displacements are formed with the helpers' interior
+// expressions in the same operand order (bitwise the general path).
+template <typename T, bool ADJ>
+__device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts, int I1, int J1, bool corr, T *lux,
+                                             T *luz, T *lsxx, T *lszz, T *lsxz) {
+  constexpr int R = ElTile<T>::rows;
+  constexpr int UW = TX + 8, UH = R + 8;
+  constexpr int SW = TX + 4, SH = R + 4;
+  constexpr int NT = TX * TZ;
+  constexpr int NSQ = (SW * SH + NT - 1) / NT;   // stress cells per thread
+  constexpr int NR = R / TZ;                     // displacement rows per thread
+  const int nx = a.nx;
+  const int64_t N = a.N;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const T *ucx = a.ucur + (int64_t)s * 2 * N, *ucz = ucx + N;
+  T *un = a.unew + (int64_t)s * 2 * N;
+  T ml[NSQ], mm[NSQ], mh[NSQ];
+#pragma unroll
+  for (int k = 0; k < NSQ; k++) {
+    const int q = tid + k * NT;
+    if (q < SW * SH) {
+      const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
+      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+      ml[k] = a.lam[c];
+      mm[k] = a.mu[c];
+      mh[k] = a.muhh[c];
+    }
+  }
+  const int i = I1 + threadIdx.x;
+  T ox[NR], oz[NR], rx[NR], rz[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const int j = J1 + threadIdx.y + k * TZ;
+    const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+    ox[k] = un[c];
+    oz[k] = un[N + c];
+    rx[k] = a.rdx[c];
+    rz[k] = a.rdz[c];
+  }
+  for (int q = tid; q < UW * UH; q += NT) {
+    const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    lux[q] = x <= nx - 1 ? ucx[c] : T(0);
+    luz[q] = ucz[c];
+  }
+  __syncthreads();
+  const TileA<T, UW> UX{lux, I1 - 4, J1 - 4}, UZ{luz, I1 - 4, J1 - 4};
+#pragma unroll
+  for (int k = 0; k < NSQ; k++) {
+    const int q = tid + k * NT;
+    if (q < SW * SH) {
+      const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
+      const T l = ml[k], m = mm[k], mhh = mh[k];
+      const T dux = d4<T>(UX(x - 2, j), UX(x - 1, j), UX(x, j), UX(x + 1, j), a.idx);
+      const T duz = d4<T>(UZ(x, j - 2), UZ(x, j - 1), UZ(x, j), UZ(x, j + 1), a.idz);
+      const T dxc = dux, dzc = duz;
+      lsxx[q] = (l + T(2) * m) * dxc + l * dzc;
+      lszz[q] = l * dxc + (l + T(2) * m) * dzc;
+      const T duzx = d4<T>(UZ(x - 1, j), UZ(x, j), UZ(x + 1, j), UZ(x + 2, j), a.idx);
+      const T duxz = d4<T>(UX(x, j - 1), UX(x, j), UX(x, j + 1), UX(x, j + 2), a.idz);
+      const T c1 = duzx, c2 = duxz;
+      lsxz[q] = mhh * (c1 + c2);
+      if (ADJ && corr && x >= I1 && x < I1 + TX && j >= J1 && j < J1 + R)
+        el_tile_correlate<T>(a, s, x, j, UX(x, j), UZ(x, j), true, true, dux, duz, duzx, duxz);
+    }
+  }
+  __syncthreads();
+  const TileA<T, SW> SXX{lsxx, I1 - 2, J1 - 2}, SZZ{lszz, I1 - 2, J1 - 2}, SXZ{lsxz, I1 - 2, J1 - 2};
+  const int sb = a.hmode == 1 ? (nts / a.S) * a.S : a.cbase;
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const int j = J1 + threadIdx.y + k * TZ;
+    const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+    const T curx = UX(i, j), curz = UZ(i, j);
+    const T d1x = d4<T>(SXX(i - 1, j), SXX(i, j), SXX(i + 1, j), SXX(i + 2, j), a.idx);
+    const T d2x = d4<T>(SXZ(i, j - 2), SXZ(i, j - 1), SXZ(i, j), SXZ(i, j + 1), a.idz);
+    const T c1x = d1x, c2x = d2x;
+    const T nux = T(2) * curx - ox[k] + rx[k] * (c1x + c2x);
+    const T d1z = d4<T>(SXZ(i - 2, j), SXZ(i - 1, j), SXZ(i, j), SXZ(i + 1, j), a.idx);
+    const T d2z = d4<T>(SZZ(i, j - 1), SZZ(i, j), SZZ(i, j + 1), SZZ(i, j + 2), a.idz);
+    const T c1z = d1z, c2z = d2z;
+    const T nuz = T(2) * curz - oz[k] + rz[k] * (c1z + c2z);
+    un[c] = nux;
+    un[N + c] = nuz;
+    if (a.hmode && a.it > sb) {
+      T *hs = a.hist + (int64_t)s * 2 * N + c;
+      const int sl = a.it - sb + 1;
+      hs[(int64_t)sl * a.hslot] = nux;
+      hs[(int64_t)sl * a.hslot + N] = nuz;
+      if (a.hmode == 1 && a.it == sb + 1) {
+        hs[0] = ox[k];
+        hs[N] = oz[k];
+        hs[a.hslot] = curx;
+        hs[a.hslot + N] = curz;
+      }
+    }
+  }
+}
+
 // One step, stresses and displacements in one launch, on a tile of 64 x R
 // displacement cells: forward or replay (ADJ = false), or adjoint (ADJ =
 // true: residual injection, and the correlation of adjoint step corr_step
@@ -1129,6 +1272,10 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
     const int r0 = J1 - 3, r1 = J1 + R + 1;   // rows J1-2 .. J1+R+1 (0-based r0 .. r1-1)
     lean = a.f0.rows[o + r0] == a.f0.rows[o + r1] && a.f1.rows[o + r0] == a.f1.rows[o + r1];
   }
+  if (lean) {
+    el_tile_lean<T, ADJ>(a, s, nts, I1, J1, corr, lux, luz, lsxx, lszz, lsxz);
+    return;
+  }
 
   // ucur on the tile + 4 (zero outside the ux / uz grids)
   for (int q = tid; q < UW * UH; q += TX * TZ) {
@@ -1148,21 +1295,7 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
     T vxx = 0, vzz = 0, vxz = 0;
     T dux = 0, duz = 0, duzx = 0, duxz = 0;   // raw derivatives (adjoint correlation)
     bool nrm = false, shr = false;
-    if (lean) {
-      // the interior branches of the helpers, in the same operand order
-      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
-      const T l = a.lam[c], m = a.mu[c], mhh = a.muhh[c];
-      dux = d4<T>(UX(x - 2, j), UX(x - 1, j), UX(x, j), UX(x + 1, j), a.idx);
-      duz = d4<T>(UZ(x, j - 2), UZ(x, j - 1), UZ(x, j), UZ(x, j + 1), a.idz);
-      const T dxc = dux, dzc = duz;
-      vxx = (l + T(2) * m) * dxc + l * dzc;
-      vzz = l * dxc + (l + T(2) * m) * dzc;
-      duzx = d4<T>(UZ(x - 1, j), UZ(x, j), UZ(x + 1, j), UZ(x + 2, j), a.idx);
-      duxz = d4<T>(UX(x, j - 1), UX(x, j), UX(x, j + 1), UX(x, j + 2), a.idz);
-      const T c1 = duzx, c2 = duxz;
-      vxz = mhh * (c1 + c2);
-      nrm = shr = true;
-    } else if (x >= 1 && x <= nx && j >= 1 && j <= nz) {
+    if (x >= 1 && x <= nx && j >= 1 && j <= nz) {
       const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
       nrm = x >= 2 && x <= nx - 1 && j >= (ft ? 1 : 2) && j <= nz - 1;
       if (nrm) {
@@ -1242,39 +1375,8 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
     lsxx[q] = vxx;
     lszz[q] = vzz;
     lsxz[q] = vxz;
-    if (ADJ && corr && own && x <= nx && j <= nz) {
-      // correlate_gradients! of adjoint step corr_step (correlate_gradient_
-      // xPU.jl:1-82) on the own stress cells, as el_stress_kernel
-      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
-      const int sl = a.corr_step - a.cbase + 1;   // slot of u^{corr_step}
-      const T *h0 = a.hist + (int64_t)(sl - 2) * a.hslot + (int64_t)s * 2 * N;
-      const T *h1 = h0 + a.hslot;
-      const T *h2 = h1 + a.hslot;
-      T *g = a.grad + (int64_t)s * a.gstride;
-      if (x <= nx - 1) {
-        const T v = UX(x, j) * (h0[c] - T(2) * h1[c] + h2[c]) * a.idt2;
-        g[c] += (j == 1 && ft) ? v / T(2) : v;
-      }
-      if (j <= nz - 1) g[N + c] += UZ(x, j) * (h0[N + c] - T(2) * h1[N + c] + h2[N + c]) * a.idt2;
-      const F2<T> fx{h1, nx}, fz{h1 + N, nx};
-      if (nrm) {
-        const T exx = d_ux_dx(fx, x, j, nx, a.idx);
-        const T ezz = d_uz_dz(fx, fz, a.lam[c], a.mu[c], x, j, nx, nz, ft, a.idx, a.idz);
-        const T dv = exx + ezz, dva = dux + duz;
-        if (j == 1 && ft) {
-          g[2 * N + c] += dv * dva / T(2);
-          g[3 * N + c] += exx * dux + ezz * duz;
-        } else {
-          g[2 * N + c] += dv * dva;
-          g[3 * N + c] += T(2) * (exx * dux + ezz * duz);
-        }
-      }
-      if (shr) {
-        const T exz = (d_uz_dx(fz, x, j, nx, a.idx) + d_ux_dz(fx, x, j, nz, ft, a.idz)) / T(2);
-        const T exza = (duzx + duxz) / T(2);
-        g[4 * N + c] += T(2) * (exz * exza + exz * exza);
-      }
-    }
+    if (ADJ && corr && own && x <= nx && j <= nz)
+      el_tile_correlate<T>(a, s, x, j, UX(x, j), UZ(x, j), nrm, shr, dux, duz, duzx, duxz);
   }
   __syncthreads();
   const TileA<T, SW> SXX{lsxx, I1 - 2, J1 - 2}, SZZ{lszz, I1 - 2, J1 - 2}, SXZ{lsxz, I1 - 2, J1 - 2};
@@ -1290,19 +1392,7 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
     const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
     const bool hx = i <= nx - 1, hz = j <= nz - 1;
     T oldx = 0, oldz = 0, curx = 0, curz = 0, nux = 0, nuz = 0;
-    if (lean) {
-      oldx = un[c];
-      oldz = un[N + c];
-      curx = UX(i, j);
-      curz = UZ(i, j);
-      const T d1x = d4<T>(SXX(i - 1, j), SXX(i, j), SXX(i + 1, j), SXX(i + 2, j), a.idx);
-      const T d2x = d4<T>(SXZ(i, j - 2), SXZ(i, j - 1), SXZ(i, j), SXZ(i, j + 1), a.idz);
-      nux = T(2) * curx - oldx + a.rdx[c] * (d1x + d2x);
-      const T d1z = d4<T>(SXZ(i - 2, j), SXZ(i - 1, j), SXZ(i, j), SXZ(i + 1, j), a.idx);
-      const T d2z = d4<T>(SZZ(i, j - 1), SZZ(i, j), SZZ(i, j + 1), SZZ(i, j + 2), a.idz);
-      nuz = T(2) * curz - oldz + a.rdz[c] * (d1z + d2z);
-    }
-    if (hx && !lean) {
+    if (hx) {
       const T d1 = d_sxx_dx<T>(SXX, i, j, nx, a.idx);
       const T d2 = d_sxz_dz<T>(SXZ, i, j, nz, ft, a.idz);
       T c1 = d1, c2 = d2;
@@ -1324,7 +1414,7 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
       curx = UX(i, j);
       nux = T(2) * curx - oldx + a.rdx[c] * (c1 + c2);
     }
-    if (hz && !lean) {
+    if (hz) {
       const T d1 = d_sxz_dx<T>(SXZ, i, j, nx, a.idx);
       const T d2 = d_szz_dz<T>(SZZ, i, j, nz, ft, a.idz);
       T c1 = d1, c2 = d2;
@@ -1346,7 +1436,7 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
       curz = UZ(i, j);
       nuz = T(2) * curz - oldz + a.rdz[c] * (c1 + c2);
     }
-    if (a.kind == 2 && !lean) {
+    if (a.kind == 2) {
       const T *tf = a.tf + a.off_tf[s] + (a.it - 1);
       if (hx) {
         const int u = find_point(a.f0.rows, a.f0.cell, a.nrows, s, j - 1, c);
@@ -1688,8 +1778,13 @@ class El2dEngine final : public Engine {
 
   void adjoint(bool per_shot) override {
     if (var_adj_ < 0) {
+      // timed on a step of the last segment that correlates (its stress
+      // launch correlates the step after it), so the candidates are timed
+      // with the correlation's loads and stores; the history slots it reads
+      // are in range, and the adjoint state and gradients are cleared below
       const int cbt = (int)(ceil_div(ntmax_, S_) - 1) * S_;
-      var_adj_ = tune("SWX_EL2D_VARIANT_ADJ", [&](int v) { step_adj_v(v, ntmax_, cbt); }, false);
+      const int itt = ntmax_ - 1 > cbt ? ntmax_ - 1 : ntmax_;
+      var_adj_ = tune("SWX_EL2D_VARIANT_ADJ", [&](int v) { step_adj_v(v, itt, cbt); }, false);
     }
     SWX_HIP(hipMemsetAsync(au_.get(), 0, au_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(asig_.get(), 0, asig_.bytes(), c_.stream));

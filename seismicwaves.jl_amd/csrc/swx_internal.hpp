@@ -288,30 +288,36 @@ void progress(const char *phase, int done, int total);
 int choose_segment(int nt_max, int check_freq, int64_t slot_bytes, int64_t ckpt_bytes, int64_t avail);
 
 // Launch-shape tuner: times launch(k) for the candidates k = 0 .. n-1 on
-// `stream`, after one warm-up launch each, in three interleaved rounds of
-// five launches (a clock ramp or a busy moment then cannot decide), keeps
-// each candidate's best round and returns the index of the fastest.  The
-// launches must be repeatable (the engines time a step that reads only
-// state it does not write, or the zero initial state).
+// `stream`, after one warm-up launch each, in three interleaved rounds (a
+// clock ramp or a busy moment then cannot decide) of at least five launches
+// and at least 2 ms each (short launches are otherwise lost in the timing
+// noise), keeps each candidate's best round (per launch) and returns the
+// index of the fastest.  The launches must be repeatable (the engines time
+// a step that reads only state it does not write, or the zero initial
+// state).
 template <typename F>
 int fastest_candidate(hipStream_t stream, int n, F launch) {
   if (n <= 1) return 0;
   hipEvent_t e0, e1;
   SWX_HIP(hipEventCreate(&e0));
   SWX_HIP(hipEventCreate(&e1));
-  std::vector<float> best(n, 1e30f);
-  for (int k = 0; k < n; k++) launch(k);
-  for (int round = 0; round < 3; round++) {
-    for (int k = 0; k < n; k++) {
-      SWX_HIP(hipEventRecord(e0, stream));
-      for (int r = 0; r < 5; r++) launch(k);
-      SWX_HIP(hipEventRecord(e1, stream));
-      SWX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      best[k] = std::min(best[k], ms);
-    }
+  auto timed = [&](int k, int reps) {
+    SWX_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; r++) launch(k);
+    SWX_HIP(hipEventRecord(e1, stream));
+    SWX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SWX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+  };
+  std::vector<int> reps(n, 5);
+  for (int k = 0; k < n; k++) {
+    const float one = timed(k, 1);   // warm-up, and the size of a round
+    reps[k] = std::max(5, std::min(2000, (int)(2.0f / std::max(one, 1e-4f)) + 1));
   }
+  std::vector<float> best(n, 1e30f);
+  for (int round = 0; round < 3; round++)
+    for (int k = 0; k < n; k++) best[k] = std::min(best[k], timed(k, reps[k]) / reps[k]);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   int k_best = 0;
