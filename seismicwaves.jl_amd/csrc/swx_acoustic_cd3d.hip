@@ -230,10 +230,127 @@ __device__ __forceinline__ void cd3d_lean_column(const Cd3dArgs<T> &a, int s, in
   }
 }
 
+// Interior z-march of cd3d_lean_column for a chunk inside the z-interior
+// (DEEP) with each plane of pcur staged through LDS: the block loads the
+// plane's 64 x TY tile and its one-cell x/y halo once (one load per value
+// instead of the five overlapping loads of cd3d_lean_column) into one of
+// two LDS planes used in turn, and takes the x/y neighbours from there.  The
+// staged values, old pressure and factor of planes k+1 and k+2 are in
+// flight while plane k is stepped; one barrier per plane.  The same
+// expressions in the same order as cd3d_lean_column, so the results are
+// bitwise equal.  Lean tiles keep every halo cell inside the grid.
+template <int TY>
+struct Cd3dHalo {
+  static constexpr int NH = 2 * TX + 2 * TY;                 // halo cells of a plane
+  static constexpr int PER = (NH + TX * TY - 1) / (TX * TY);  // per thread
+};
+
+template <typename T, int TY, bool CORR, bool DST>
+__device__ __forceinline__ void cd3d_lds_column(const Cd3dArgs<T> &a, T *sp, int s, int dslot, int i0, int j0,
+                                                int K0, int K1) {
+  using H = Cd3dHalo<TY>;
+  constexpr int W = TX + 2;
+  const int nx = a.n[0], ny = a.n[1];
+  const int64_t so = (int64_t)s * a.N;
+  const int64_t sxy = (int64_t)nx * ny;
+  const T *__restrict__ pc_ = a.pcur + so;
+  const T *__restrict__ po_ = a.pold + so;
+  T *__restrict__ pn_ = a.pnew + so;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int cxy = (j0 + ty) * nx + i0 + tx;
+  const int cslot = a.it - a.cbase;
+  // halo cell q of the plane: rows -1 and TY over the tile columns, then
+  // columns -1 and TX over the tile rows (LDS coordinates lx, lz; in-plane
+  // offset off from the tile's corner cell)
+  int hl[H::PER], hoff[H::PER];
+#pragma unroll
+  for (int r = 0; r < H::PER; r++) {
+    const int q = tid + r * TX * TY;
+    int lx = -1, ly = -1;
+    if (q < TX) { lx = q + 1; ly = 0; }
+    else if (q < 2 * TX) { lx = q - TX + 1; ly = TY + 1; }
+    else if (q < 2 * TX + TY) { lx = 0; ly = q - 2 * TX + 1; }
+    else if (q < H::NH) { lx = TX + 1; ly = q - 2 * TX - TY + 1; }
+    hl[r] = ly >= 0 ? ly * W + lx : -1;
+    hoff[r] = ly >= 0 ? (j0 + ly - 1) * nx + i0 + lx - 1 : 0;
+  }
+  struct Plane { T ctr, hal[H::PER], po, fa, g, d; };
+  auto load_plane = [&](int k) -> Plane {
+    Plane q;
+    const int64_t base = (int64_t)k * sxy;
+    q.ctr = pc_[base + cxy];
+#pragma unroll
+    for (int r = 0; r < H::PER; r++) q.hal[r] = hl[r] >= 0 ? pc_[base + hoff[r]] : T(0);
+    q.po = po_[base + cxy];
+    q.fa = a.fact[base + cxy];
+    q.g = CORR ? a.grad[so + base + cxy] : T(0);
+    q.d = CORR ? a.dbuf[(int64_t)cslot * a.dslot + so + base + cxy] : T(0);
+    return q;
+  };
+  auto stage = [&](int buf, const Plane &q) {
+    T *s_ = sp + buf * (TY + 2) * W;
+    s_[(ty + 1) * W + tx + 1] = q.ctr;
+#pragma unroll
+    for (int r = 0; r < H::PER; r++)
+      if (hl[r] >= 0) s_[hl[r]] = q.hal[r];
+  };
+  T pm = pc_[(int64_t)(K0 - 1) * sxy + cxy];
+  T pc;
+  // plane k sits in LDS plane (k - K0) & 1; cur holds its operands, nxt
+  // the staged values of plane k+1 (pp) and its operands
+  auto step = [&](int k, const Plane &cur, const Plane &nxt) {
+    const T *s_ = sp + ((k - K0) & 1) * (TY + 2) * W;
+    const T pl = s_[(ty + 1) * W + tx], pr = s_[(ty + 1) * W + tx + 2];
+    const T pd = s_[ty * W + tx + 1], pu = s_[(ty + 2) * W + tx + 1];
+    const T pp = nxt.ctr;
+    const int64_t c = (int64_t)k * sxy + cxy;
+    const T d2x = (pl - T(2) * pc + pr) * a.id2[0];
+    const T d2y = (pd - T(2) * pc + pu) * a.id2[1];
+    const T d2z = (pm - T(2) * pc + pp) * a.id2[2];
+    const T lx = d2x, ly = d2y, lz = d2z;
+    const T flap = cur.fa * (lx + ly + lz);
+    const T pnv = T(2) * pc - cur.po + flap;
+    pn_[c] = pnv;
+    if (DST) {
+      const T inj = 0;
+      a.dbuf[(int64_t)dslot * a.dslot + so + c] = flap + inj;
+    }
+    if (CORR) a.grad[so + c] = cur.g + pc * cur.d;
+    pm = pc;
+    pc = pp;
+  };
+  // Three plane buffers used in turn (unrolled by three, as in
+  // cd3d_lean_column); plane K1 (the last plane's z neighbour) is loaded
+  // too, which the DEEP bound keeps inside the grid.
+  Plane A = load_plane(K0);
+  Plane B = load_plane(K0 + 1);
+  stage(0, A);
+  pc = A.ctr;
+  __syncthreads();
+  for (int k = K0; k < K1; k += 3) {
+    Plane C = load_plane(min(k + 2, K1));
+    stage((k + 1 - K0) & 1, B);
+    step(k, A, B);
+    __syncthreads();
+    if (k + 1 >= K1) break;
+    A = load_plane(min(k + 3, K1));
+    stage((k + 2 - K0) & 1, C);
+    step(k + 1, B, C);
+    __syncthreads();
+    if (k + 2 >= K1) break;
+    B = load_plane(min(k + 4, K1));
+    stage((k + 3 - K0) & 1, A);
+    step(k + 2, C, A);
+    __syncthreads();
+  }
+}
+
 // TY: tile height (rows of the block); MODE 0 takes interior tiles through
 // cd3d_lean_column and the others through the general march in one launch,
 // MODE 1 runs only the interior tiles and MODE 2 only the others (the two
-// launched one after the other, so each gets its own register budget).
+// launched one after the other, so each gets its own register budget);
+// MODE 3 is MODE 0 with the chunks of interior tiles inside the z-interior
+// stepped by cd3d_lds_column.
 template <typename T, int TY, int MODE>
 __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   const int b = xcd_tile3(blockIdx.x, gridDim.x);
@@ -247,13 +364,24 @@ __global__ __launch_bounds__(TX *TY) void cd3d_step_kernel(Cd3dArgs<T> a) {
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], h = a.h;
   const int i = bx * TX + threadIdx.x, j = by * TY + threadIdx.y;
   const int K0 = bk * a.kch, K1 = min(K0 + a.kch, nz);
-  if (MODE == 1 || (a.lean && MODE == 0)) {
+  if (MODE == 1 || (a.lean && (MODE == 0 || MODE == 3))) {
     if (cd3d_is_lean<T, TY>(a, s, bx, by, K0, K1)) {
-      if (MODE < 2) {
+      if (MODE != 2) {
         const bool corr = a.corr && a.it + 1 <= nts && a.it + 1 <= a.corr_lim;
         const int dslot = cd3d_dslot(a, nts);
         const bool deep = K0 >= max(h, 1) && K1 - 1 <= nz - h - 2;
-        if (deep) {
+        if (MODE == 3 && deep) {
+          // two LDS planes of the tile and its x/y halo
+          __shared__ T sp[2 * (TY + 2) * (TX + 2)];
+          const int i0 = bx * TX, j0 = by * TY;
+          if (corr) {
+            if (dslot >= 0) cd3d_lds_column<T, TY, true, true>(a, sp, s, dslot, i0, j0, K0, K1);
+            else cd3d_lds_column<T, TY, true, false>(a, sp, s, dslot, i0, j0, K0, K1);
+          } else {
+            if (dslot >= 0) cd3d_lds_column<T, TY, false, true>(a, sp, s, dslot, i0, j0, K0, K1);
+            else cd3d_lds_column<T, TY, false, false>(a, sp, s, dslot, i0, j0, K0, K1);
+          }
+        } else if (deep) {
           if (corr) {
             if (dslot >= 0) cd3d_lean_column<T, true, true, true>(a, s, dslot, i, j, K0, K1);
             else cd3d_lean_column<T, true, true, false>(a, s, dslot, i, j, K0, K1);
@@ -509,12 +637,12 @@ class Cd3dEngine final : public Engine {
     }
     if (const char *v = std::getenv("SWX_CD3D_LEAN")) lean_ = std::atoi(v) != 0;
     // launch shape: 8 / 4 / 2 = one launch with 64 x 8 / 64 x 4 / 64 x 2
-    // tiles, 108 / 104 / 102 = interior tiles and the others in two launches
+    // tiles, 108 / 104 / 102 = interior tiles and the others in two launches,
+    // 208 / 204 / 202 = one launch with the LDS-staged interior march
     if (const char *v = std::getenv("SWX_CD3D_VARIANT")) {
       const int c = std::atoi(v);
-      SWX_REQUIRE(c % 100 == 8 || c % 100 == 4 || c % 100 == 2, SWX_EINVAL,
-                  "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
-      SWX_REQUIRE(c < 200, SWX_EINVAL, "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104 or 102");
+      SWX_REQUIRE((c % 100 == 8 || c % 100 == 4 || c % 100 == 2) && c > 0 && c < 300, SWX_EINVAL,
+                  "SWX_CD3D_VARIANT must be 8, 4, 2, 108, 104, 102, 208, 204 or 202");
       variant_ = c;
       variant_env_ = true;
     }
@@ -647,47 +775,51 @@ class Cd3dEngine final : public Engine {
     return a;
   }
 
+  // kind 0: one launch; 1: interior tiles and the others in two launches;
+  // 2: one launch with the LDS-staged march (cd3d_lds_column)
   template <int TY>
-  void launch_shape(Cd3dArgs<T> a, bool split) {
+  void launch_shape(Cd3dArgs<T> a, int kind) {
     a.nby = (int)ceil_div(n_[1], TY);
     const unsigned nb = (unsigned)((int64_t)a.nbx * a.nby * a.nbk * nsh_);
-    if (split && a.lean) {
+    if (kind == 1 && a.lean) {
       cd3d_step_kernel<T, TY, 1><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
       cd3d_step_kernel<T, TY, 2><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
+    } else if (kind == 2) {
+      cd3d_step_kernel<T, TY, 3><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
     } else {
       cd3d_step_kernel<T, TY, 0><<<nb, dim3(TX, TY), 0, c_.stream>>>(a);
     }
   }
 
+  void launch_variant(const Cd3dArgs<T> &a, int variant) {
+    const int ty = variant % 100, kind = variant / 100;
+    if (ty == 4) launch_shape<4>(a, kind);
+    else if (ty == 2) launch_shape<2>(a, kind);
+    else launch_shape<8>(a, kind);
+  }
+
   void launch(const Cd3dArgs<T> &a, int kind) {
     c_.ktimer.begin(c_.stream);
-    const int ty = variant_ % 100;
-    if (ty == 4) launch_shape<4>(a, variant_ > 100);
-    else if (ty == 2) launch_shape<2>(a, variant_ > 100);
-    else launch_shape<8>(a, variant_ > 100);
+    launch_variant(a, variant_);
     c_.ktimer.end(c_.stream, kind);
     SWX_HIP(hipGetLastError());
   }
 
-  // Launch shape (tile height 8 / 4 / 2, one or two launches) and then the
-  // planes per chunk, timed on step 1 of the zero initial state, which every
+  // Launch shape (tile height 8 / 4 / 2; one launch, two launches or the
+  // LDS-staged march) and then the planes per chunk, timed on step 1 of the zero initial state, which every
   // repetition recomputes identically.  Every choice gives bitwise the same
   // results; SWX_CD3D_VARIANT / SWX_CD3D_ZCHUNK pin them instead.
   void tune_shape(bool save, T *tr) {
     auto run = [&](int variant, int kch) {
       variant_ = variant;
       kch_ = kch;
-      Cd3dArgs<T> a = fwd_args(1, 0, save ? 1 : 0, 0, tr);
-      const int ty = variant % 100;
-      if (ty == 4) launch_shape<4>(a, variant > 100);
-      else if (ty == 2) launch_shape<2>(a, variant > 100);
-      else launch_shape<8>(a, variant > 100);
+      launch_variant(fwd_args(1, 0, save ? 1 : 0, 0, tr), variant);
       SWX_HIP(hipGetLastError());
     };
     if (!variant_env_) {
-      static const int vars[6] = {4, 8, 2, 104, 108, 102};
+      static const int vars[9] = {4, 8, 2, 104, 108, 102, 204, 208, 202};
       const int kch = kch_;
-      variant_ = vars[fastest_candidate(c_.stream, 6, [&](int c) { run(vars[c], kch); })];
+      variant_ = vars[fastest_candidate(c_.stream, 9, [&](int c) { run(vars[c], kch); })];
     }
     if (!kch_env_) {
       static const int chunks[3] = {24, 12, 48};

@@ -174,13 +174,16 @@ def _interior_problem(dtype, nt=150):
     return dict(params=params, vtrue=vtrue, v0=v0, shots=shots, osh=osh, h=h, dt=dt, halo=halo, nt=nt)
 
 
-@pytest.mark.parametrize("chunk", ["24", "7"])
-def test_interior_tiles_against_oracle(oracle_mod, monkeypatch, chunk):
+@pytest.mark.parametrize("chunk,variant", [("24", None), ("7", None), ("7", "204"), ("7", "202")])
+def test_interior_tiles_against_oracle(oracle_mod, monkeypatch, chunk, variant):
     """FP64 seismograms and gradient on a grid whose interior tiles take the
     test-free path; with 7 planes per block some chunks lie wholly inside the
-    z-interior and take the straight-line plane update."""
-    print(f"[cd3d interior tiles float64, z chunk {chunk}]", flush=True)
+    z-interior and take the straight-line plane update (launch shapes 204 and
+    202: staged through LDS)."""
+    print(f"[cd3d interior tiles float64, z chunk {chunk}, launch shape {variant}]", flush=True)
     monkeypatch.setenv("SWX_CD3D_ZCHUNK", chunk)
+    if variant:
+        monkeypatch.setenv("SWX_CD3D_VARIANT", variant)
     p = _interior_problem(np.float64)
     obs = oracle_mod.acoustic_cd3d_forward(p["vtrue"], p["h"], p["dt"], p["nt"], p["halo"], 1e-4, True, p["osh"])
     sw.swforward(p["params"], sw.VpAcousticCDMaterialProperties(p["vtrue"]), p["shots"], runparams=sw.RunParameters())
@@ -202,7 +205,7 @@ def test_interior_tiles_against_oracle(oracle_mod, monkeypatch, chunk):
 def test_interior_path_agrees_bitwise(monkeypatch, chunk):
     """The test-free interior path gives the general path's results bit for
     bit (seismograms and gradient, FP32), with 64 x 8, 64 x 4 and 64 x 2 tiles, in one
-    launch or two."""
+    launch or two, and with the LDS-staged march."""
     p = _interior_problem(np.float32)
     rng = np.random.default_rng(1)
     mis = [sw.L2Misfit(rng.standard_normal(sh.recs.seismograms.shape).astype(np.float32) * 1e-3) for sh in p["shots"]]
@@ -210,7 +213,7 @@ def test_interior_path_agrees_bitwise(monkeypatch, chunk):
     monkeypatch.setenv("SWX_CD3D_ZCHUNK", chunk)
     outs = []
     for variant, lean in (("8", "1"), ("8", "0"), ("4", "1"), ("4", "0"), ("2", "1"), ("108", "1"), ("104", "1"),
-                          ("102", "1")):
+                          ("102", "1"), ("208", "1"), ("204", "1"), ("202", "1")):
         print(f"[cd3d z chunk {chunk}, launch shape {variant}, interior path {lean}]", flush=True)
         monkeypatch.setenv("SWX_CD3D_VARIANT", variant)
         monkeypatch.setenv("SWX_CD3D_LEAN", lean)
