@@ -82,7 +82,7 @@ def workload_vd2d_batched():
                         "check_freq": 50, "halo": halo})
 
 
-def workload_cd2d_grad_1000():
+def workload_cd2d_grad_1000(nt=4000):
     """2D acoustic constant-density gradient: 1000 x 1000, dh = 5 m,
     dt = 0.5 ms, nt = 4000, seeded 8-layer model, start model smoothed with
     sigma = 100 m, one 12 Hz Ricker shot at (2500 m, 10 m), 400 receivers at
@@ -91,7 +91,7 @@ def workload_cd2d_grad_1000():
     from seismicwaves import synthetic as syn
     nx = nz = 1000
     h = (5.0, 5.0)
-    dt, nt, f0, halo = 5e-4, 4000, 12.0, 20
+    dt, f0, halo = 5e-4, 12.0, 20
     rng = np.random.default_rng(1)
     vtrue = syn.layered(nx, nz, h, rng)
     v0 = syn.gaussian_smooth(vtrue, 100.0 / h[0])
@@ -100,7 +100,7 @@ def workload_cd2d_grad_1000():
     params = sw.InputParametersAcoustic(nt, dt, (nx, nz), h, sw.CPMLBoundaryConditionParameters(halo=halo))
     shots = [sw.ScalarShot(sw.ScalarSources(np.array([[2500.0, 10.0]]), tf, f0),
                            sw.ScalarReceivers(rec, nt, dtype=np.float32))]
-    return dict(name="acoustic_cd_2d_gradient_1000x1000_nt4000", params=params, physics="cd2d",
+    return dict(name=f"acoustic_cd_2d_gradient_1000x1000_nt{nt}", params=params, physics="cd2d",
                 mp_true=sw.VpAcousticCDMaterialProperties(vtrue.astype(np.float32)),
                 mp0=sw.VpAcousticCDMaterialProperties(v0.astype(np.float32)), shots=shots, check_freq=50, run={},
                 config={"nx": nx, "nz": nz, "dh_m": 5.0, "dt_s": dt, "nt": nt, "shots": 1, "receivers": 400,
@@ -251,7 +251,12 @@ def workload_el2d_batched(nt=5000):
 WORKLOADS = {"vd2d_batched": workload_vd2d_batched, "cd2d_grad_1000": workload_cd2d_grad_1000,
              "vd2d_batched_small": workload_vd2d_batched_small, "el2d_batched": workload_el2d_batched,
              "vd2d_batched_nt10": workload_vd2d_batched_nt10, "cd3d_600x600x400": workload_cd3d_600x600x400,
-             "el2d_batched_small": lambda: workload_el2d_batched(nt=500)}
+             "el2d_batched_small": lambda: workload_el2d_batched(nt=500),
+             # counter-collection runs (every dispatch instrumented): the same
+             # grids, shots and receivers with short time loops
+             "el2d_batched_nt100": lambda: workload_el2d_batched(nt=100),
+             "cd3d_600x600x400_nt60": lambda: workload_cd3d_600x600x400(nt=60),
+             "cd2d_grad_1000_nt400": lambda: workload_cd2d_grad_1000(nt=400)}
 
 
 # ---------------------------------------------------------------------------
@@ -292,6 +297,9 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
     ntmax = max(nts)
     strip = _strip_cells(n, halo) * 2 * e
     fwd_b = fwd_n = adj_b = adj_n = 0
+    # VD shots whose model factors one block (4xx: two shots per block) or
+    # one group of the block order (code + 1000 k: 2^k shots) read together
+    vd_groups = tuple(2 if 400 <= v % 1000 < 500 else 1 << (v // 1000) for v in variant)
     for nt in nts:
         if physics == "vd2d":
             # the final nt - floor(nt/S)*S steps are stored during the
@@ -321,11 +329,12 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             adj_b += nt * 27 * e * N + nt * est
             adj_n += (1 if march_a else 2) * nt
             continue
-        if physics == "vd2d" and (400 <= variant[0] < 500 or 400 <= variant[1] < 500):
-            # two shots per block: the model factors m0, m1x, m1z (3e) are read
-            # once per shot pair and step (counted below per pair); per shot
-            # forward r/w p, vx, vz (6e) + w p-history, adjoint 14e
-            pf, pa = 400 <= variant[0] < 500, 400 <= variant[1] < 500
+        if physics == "vd2d" and (vd_groups[0] > 1 or vd_groups[1] > 1):
+            # two shots per block (4xx) or shot groups in the block order
+            # (code + 1000 k): the model factors m0, m1x, m1z (3e) are read
+            # once per group of shots and step (counted below per group); per
+            # shot forward r/w p, vx, vz (6e) + w p-history, adjoint 14e
+            pf, pa = vd_groups[0] > 1, vd_groups[1] > 1
             fwd_b += ((nt + rep) * (6 if pf else 9) + (last + rep)) * e * N
             fwd_b += (nt + rep) * strip
             fwd_n += nt + rep
@@ -358,14 +367,15 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
             adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
             adj_n += nt
     if physics == "vd2d":
-        # model factors once per shot pair and step the pair steps (its
-        # longer shot's forward and replay steps; adjoint steps)
-        for k in range(0, len(nts), 2):
-            pair = nts[k:k + 2]
-            if 400 <= variant[0] < 500:
-                fwd_b += 3 * e * N * max(nt + (nt // S) * (S - 1) for nt in pair)
-            if 400 <= variant[1] < 500:
-                adj_b += 3 * e * N * max(pair)
+        # model factors once per group and step the group steps (its
+        # longest shot's forward and replay steps; adjoint steps)
+        gf, ga = vd_groups
+        if gf > 1:
+            for k in range(0, len(nts), gf):
+                fwd_b += 3 * e * N * max(nt + (nt // S) * (S - 1) for nt in nts[k:k + gf])
+        if ga > 1:
+            for k in range(0, len(nts), ga):
+                adj_b += 3 * e * N * max(nts[k:k + ga])
     return {0: (fwd_b, fwd_n), 1: (adj_b, adj_n)}
 
 
@@ -374,6 +384,7 @@ def kernel_names(physics, variant):
     picked (variant codes of swx_plan_info.kernel_variant)."""
     if physics == "vd2d":
         def name(kind, v):
+            v %= 1000      # the thousands give the shot groups of the block order
             if v >= 400:   # two shots per block, one launch
                 return f"vd2d_{kind}_pair_kernel<float, {v - 400}>"
             if v >= 300:   # interior and other tiles in one launch
@@ -400,7 +411,9 @@ def kernel_names(physics, variant):
     if physics == "cd3d":
         # [launch shape, planes per z chunk]
         v = variant[0] or 4
-        if v > 100:   # interior tiles and the others in two launches
+        if v > 200:   # one launch, interior march staged through LDS
+            k = f"cd3d_step_kernel<float, {v - 200}, 3>"
+        elif v > 100:   # interior tiles and the others in two launches
             k = f"cd3d_step_kernel<float, {v - 100}, 1> + cd3d_step_kernel<float, {v - 100}, 2>"
         else:
             k = f"cd3d_step_kernel<float, {v}, 0>"
@@ -605,6 +618,28 @@ def cpu_baseline_elastic(wl, cores, nt_sample=300):
 
 
 # ---------------------------------------------------------------------------
+def warm_process(workload):
+    """Runs one gradient step of the same workload in a child process that
+    then exits, before the measurement.  On a freshly started box the first
+    process that runs a large workload runs the step kernels slower: the
+    64-shot VD gradient measured 68.1 GCell-updates/s in the first process
+    and 74.8 in the next one on the same box (adjoint launch 1176 -> 998 us,
+    forward 654 -> 631 us); touching the memory from a child process without
+    running the workload did not always reach the second state.  A device
+    that has run anything before, as a serving or inversion host has, is in
+    the second state.  Nothing of the measured work is done here: the child
+    builds its own plan and data."""
+    import subprocess
+    t0 = time.perf_counter()
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", workload, "--steps", "1", "--warmup", "0",
+           "--no-cpu", "--no-one-shot", "--timed-steps-only", "--no-warm-process"]
+    # the child's progress lines go to this process's stderr
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        return f"failed ({r.returncode})"
+    return f"one {workload} gradient step in a child process, {time.perf_counter() - t0:.0f} s"
+
+
 def hbm_triad(bytes_each=4 << 30, reps=20):
     """The device's practical HBM ceiling: a STREAM triad a = b + s c on
     float4 vectors (seismicwaves.jl_amd/probe/hbm_triad.hip), 3 arrays of
@@ -640,6 +675,8 @@ def main():
                     help="stop after the timed steps (for counter-collection runs, where every dispatch is "
                          "instrumented): no per-launch timing pass, no roofline")
     ap.add_argument("--one-shot-sample", type=int, default=2, help="shots timed on the 1-shot path")
+    ap.add_argument("--no-warm-process", action="store_true",
+                    help="measure without first running one step of the workload in a child process")
     ap.add_argument("--hbm-triad", action="store_true",
                     help="only measure the HBM ceiling: a STREAM triad on 3 x 4 GiB (probe/libswx_probe.so)")
     args = ap.parse_args()
@@ -652,10 +689,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+    warm = None
+    if not args.no_warm_process:
+        warm = warm_process(args.workload)
+        log("warm-up process:", warm)
     import seismicwaves as sw
     wl = WORKLOADS[args.workload]()
     p = wl["params"]
-    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
     log("workload", wl["name"])
     # observed data from the true model (outside the timed region)
     if args.forward_only:
@@ -734,6 +775,7 @@ def main():
                      "alg_bytes_per_launch": fam[dom]["alg_bytes_per_launch"]},
         "kernels": {kern[k]: fam[k] for k in fam},
         "device_ms_per_step": round(dev_ms, 1),
+        "warm_process": warm or "off",
     }
     if not args.no_one_shot and len(wl["shots"]) > 1:
         # the same kernels on a plan holding one shot, shots run in sequence

@@ -72,6 +72,7 @@ struct Vd2dArgs {
   const T *res;                  // adjoint residuals
   const int *nt_shot;
   int nsh;                       // shots of the batch (pair kernels)
+  int sgrp;                      // shots per group of the block order (vd2d_tile_of)
   int it, act_min;
   // pressure history of the segment: slot k holds p^{base+k}
   T *pbuf; int64_t pslot; int smode; int S; int cbase;
@@ -104,6 +105,26 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int per = nb >> 3;
   if (b >= per * 8) return b;
   return (b & 7) * per + (b >> 3);
+}
+
+// Tile (bx, bz) and shot s of block b of a tiles x shots grid.  The shots
+// are taken in groups of sgrp (a divisor of the batch): the blocks of one
+// tile for the shots of a group follow each other, then the next tile of
+// the group, and a group's tiles form one contiguous run.  Consecutive
+// blocks land on one XCD (xcd_tile), so the sgrp blocks of a tile read its
+// model factors (3 of the 9 values of a forward cell step) from that XCD's
+// L2 after the first; sgrp = 1 is the plain shot-after-shot order.  The
+// z-neighbour tile of the same shot stays nbx * sgrp blocks back, within
+// L2 reach for small groups.
+template <typename T>
+__device__ __forceinline__ void vd2d_tile_of(const Vd2dArgs<T> &a, int b, int &bx, int &bz, int &s) {
+  const int g = a.sgrp;
+  const int lo = b % g, t = b / g;
+  const int tiles = a.nbx * a.nbz;
+  const int tile = t % tiles;
+  s = (t / tiles) * g + lo;
+  bx = tile % a.nbx;
+  bz = tile / a.nbx;
 }
 
 // Is tile (I0, J0) of shot s an interior tile of the forward step: every
@@ -273,8 +294,8 @@ __device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, 
 // grid, steps them.
 template <typename T, int TZ>
 __global__ __launch_bounds__(NT) void vd2d_fwd_lean(Vd2dArgs<T> a) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  int bx, bz, s;
+  vd2d_tile_of(a, xcd_tile(blockIdx.x, gridDim.x), bx, bz, s);
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
   const int I0 = bx * TX, J0 = bz * TZ;
@@ -483,8 +504,8 @@ __device__ __forceinline__ void vd2d_fwd_general_tile(const Vd2dArgs<T> &a, int 
 // vd2d_fwd_lean.
 template <typename T, int TZ, bool FOLD>
 __global__ __launch_bounds__(NT) void vd2d_fwd_kernel(Vd2dArgs<T> a) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  int bx, bz, s;
+  vd2d_tile_of(a, xcd_tile(blockIdx.x, gridDim.x), bx, bz, s);
   const int nts = a.nt_shot[s];
   if (a.it > nts || nts <= a.act_min) return;
   const int I0 = bx * TX, J0 = bz * TZ;
@@ -1200,8 +1221,8 @@ __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, 
 // steps them.
 template <typename T, int TZ>
 __global__ __launch_bounds__(NT) void vd2d_adj_lean(Vd2dArgs<T> a) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  int bx, bz, s;
+  vd2d_tile_of(a, xcd_tile(blockIdx.x, gridDim.x), bx, bz, s);
   const int nts = a.nt_shot[s];
   if (a.it > nts) return;
   const int I0 = bx * TX, J0 = bz * TZ;
@@ -1430,8 +1451,8 @@ __device__ __forceinline__ void vd2d_adj_general_tile(const Vd2dArgs<T> &a, int 
 // inline (FOLD, one launch per step) or left to vd2d_adj_lean.
 template <typename T, int TZ, bool FOLD>
 __global__ __launch_bounds__(NT) void vd2d_adj_kernel(Vd2dArgs<T> a) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % a.nbx, bzs = b / a.nbx, bz = bzs % a.nbz, s = bzs / a.nbz;
+  int bx, bz, s;
+  vd2d_tile_of(a, xcd_tile(blockIdx.x, gridDim.x), bx, bz, s);
   const int nts = a.nt_shot[s];
   if (a.it > nts) return;
   const int I0 = bx * TX, J0 = bz * TZ;
@@ -1861,6 +1882,16 @@ class Vd2dEngine final : public Engine {
   // launch also writes into the other buffer, so it toggles flip_.
   int slot(int k) const { return ((k % 2) + 2 + flip_) % 2; }
   static int steps_per_launch(int code) { return code >= 200 && code < 300 ? 2 : 1; }
+  // a 2-D tile code and its shot-group codes (code + 1000 k, groups of 2^k
+  // shots, k = 1..3) whose group divides the batch
+  std::vector<int> with_groups(int code) const {
+    std::vector<int> v = {code};
+    const int base = code % 100;
+    if (code >= 400 || (code >= 100 && code < 300) || (base != 8 && base != 16 && base != 32)) return v;
+    for (int k = 1; k <= 3; k++)
+      if (nsh_ % (1 << k) == 0 && nsh_ > (1 << k)) v.push_back(code + 1000 * k);
+    return v;
+  }
 
   Vd2dArgs<T> args(const DevBuf *b, int it, int rpar, int wpar) const {
     Vd2dArgs<T> a{};
@@ -1880,6 +1911,7 @@ class Vd2dEngine final : public Engine {
     a.res = c_.pts.d_res.as<T>();
     a.nt_shot = c_.pts.d_nt.as<int>();
     a.nsh = nsh_;
+    a.sgrp = 1;
     a.it = it;
     a.pbuf = pbuf_.as<T>(); a.pslot = (int64_t)ns_ * N_;
     a.S = store_all_ ? (1 << 30) : S_;   // store-all: the final segment is the whole run
@@ -1909,7 +1941,11 @@ class Vd2dEngine final : public Engine {
 
   // Forward variants: tile heights 8/16/32 (2-D tiles) and 102/104
   // (z-marching strips of 128/256 columns); the autotuner picks one.
+  // Thousands of a 2-D tile code give the shot groups of the block order
+  // (vd2d_tile_of): code + 1000 k takes groups of 2^k shots.
   void launch_fwd(Vd2dArgs<T> a, int tz) {
+    a.sgrp = 1 << (tz / 1000);
+    tz %= 1000;
     if (tz == 102 || tz == 104) {
       const unsigned g = grid_march(a, tz - 100);
       if (tz == 102) vd2d_fwd_march<T, 2><<<g, 128, 0, c_.stream>>>(a);
@@ -1950,6 +1986,8 @@ class Vd2dEngine final : public Engine {
     SWX_HIP(hipGetLastError());
   }
   void launch_adj(Vd2dArgs<T> a, int tz) {
+    a.sgrp = 1 << (tz / 1000);
+    tz %= 1000;
     if (tz == 102 || tz == 104) {
       const unsigned g = grid_march(a, tz - 100);
       if (tz == 102) vd2d_adj_march<T, 2><<<g, 128, 0, c_.stream>>>(a);
@@ -2038,18 +2076,26 @@ class Vd2dEngine final : public Engine {
   // runs before the adjoint state and gradients are cleared) and returns the
   // fastest.  The candidates are timed in three interleaved rounds of five
   // launches and each keeps its best round, so a clock ramp or a busy
-  // moment during one round does not decide.  SWX_VD2D_TILE_FWD / _ADJ pin
-  // a value instead (also the variants the tuner does not time: the
+  // moment during one round does not decide.  The shot groups of the block
+  // order (code + 1000 k) are then timed for the fastest 2-D tile variant.
+  // SWX_VD2D_TILE_FWD / _ADJ pin a value instead (also the variants the
+  // tuner does not time: the
   // z-marching strips 102/104, two steps per launch 208/216/232 and two
   // shots per block 408/416, all measured slower on MI355X).
   template <typename F>
   int tune(const char *env, const std::vector<int> &cands, F launch) {
     if (const char *v = std::getenv(env)) {
-      const int t = std::atoi(v);
+      const int t = std::atoi(v), base = t % 1000, k = t / 1000;
       for (int c : cands)
-        if (c == t) return t;
+        if (c == base && (k == 0 || (k <= 3 && nsh_ % (1 << k) == 0))) return t;
       if (t == 102 || t == 104 || t == 208 || t == 216 || t == 232 || t == 408 || t == 416) return t;
     }
+    // the tile variant in the plain shot order, then its shot groups
+    const int tile = fastest(cands, launch);
+    return fastest(with_groups(tile), launch);
+  }
+  template <typename F>
+  int fastest(const std::vector<int> &cands, F launch) {
     if (cands.size() == 1) return cands.front();
     hipEvent_t e0, e1;
     SWX_HIP(hipEventCreate(&e0));

@@ -36,6 +36,27 @@ def test_vd_forward_and_adjoint_bytes():
     assert b[0] == ((nt - last) * 9 * E * N + last * 10 * E * N + rep * 10 * E * N + (nt + rep) * STRIP, nt + rep)
 
 
+def test_vd_shot_group_bytes():
+    """Shot groups of the block order (code + 1000 k, 2^k shots) read the
+    model factors (3e) once per group and step: per shot 6e forward (+1e
+    history) and 14e adjoint, plus 3e per group for each step of the
+    group's longest shot."""
+    nts, S = [500, 450, 520, 480], 50
+    b1 = bench.alg_bytes("vd2d", NS, 0, E, nts, S, (316, 316))
+    steps = [nt + (nt // S) * (S - 1) for nt in nts]
+    for k, code in ((1, 1316), (2, 2316)):
+        g = 1 << k
+        b = bench.alg_bytes("vd2d", NS, 0, E, nts, S, (code, code))
+        model_f = sum(3 * E * N * max(steps[i:i + g]) for i in range(0, len(nts), g))
+        model_a = sum(3 * E * N * max(nts[i:i + g]) for i in range(0, len(nts), g))
+        assert b[0][0] == b1[0][0] - sum(3 * E * N * st for st in steps) + model_f
+        assert b[1][0] == b1[1][0] - sum(3 * E * N * nt for nt in nts) + model_a
+        assert b[0][1] == b1[0][1] and b[1][1] == b1[1][1]
+    # a group of two is priced as two shots per block (4xx)
+    assert bench.alg_bytes("vd2d", NS, 0, E, nts, S, (1316, 1316)) == bench.alg_bytes("vd2d", NS, 0, E, nts, S,
+                                                                                        (416, 416))
+
+
 def test_cd_forward_and_adjoint_bytes():
     nt, S = 501, 50
     b = bench.alg_bytes("cd2d", NS, 0, E, [nt], S)

@@ -200,3 +200,34 @@ def test_pair_kernel_odd_shot_count(monkeypatch):
     assert np.array_equal(outs[0][0]["vp"], outs[1][0]["vp"])
     assert np.array_equal(outs[0][0]["rho"], outs[1][0]["rho"])
     assert all(np.array_equal(a, b) for a, b in zip(outs[0][1], outs[1][1]))
+
+
+def test_shot_group_orders_agree_bitwise(monkeypatch):
+    """The block orders that take the shots in groups of 2, 4 and 8 (tile
+    codes + 1000, 2000, 3000) give the plain order's seismograms and
+    gradients bit for bit, with 8 shots of uneven length (some end inside a
+    group while the others go on), in one launch or two per step; the
+    autotuner's pick (no variant pinned) agrees too."""
+    nts = (300, 257, 281, 240, 299, 263, 270, 251)
+    p = _problem(np.float32, nts=nts, n=(301, 150))
+    sw.swforward(p["params"], sw.VpRhoAcousticVDMaterialProperties(p["vtrue"].astype(np.float32),
+                                                                    p["rtrue"].astype(np.float32)),
+                 p["shots"], runparams=sw.RunParameters())
+    mis = [sw.L2Misfit(sh.recs.seismograms.copy()) for sh in p["shots"]]
+    mp = sw.VpRhoAcousticVDMaterialProperties(p["v0"].astype(np.float32), p["r0"].astype(np.float32))
+    outs = []
+    for v, w in (("316", "316"), ("1316", "2316"), ("2316", "3316"), ("3316", "1316"), ("1016", "2016"),
+                 ("2008", "1316"), ("3332", "16"), (None, None)):
+        print(f"[vd2d shot groups forward {v} adjoint {w}]", flush=True)
+        for env, val in (("SWX_VD2D_TILE_FWD", v), ("SWX_VD2D_TILE_ADJ", w)):
+            if val is None:
+                monkeypatch.delenv(env, raising=False)
+            else:
+                monkeypatch.setenv(env, val)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=40))
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]]))
+    assert np.max(np.abs(outs[0][0]["vp"])) > 0
+    for g, seis in outs[1:]:
+        assert np.array_equal(g["vp"], outs[0][0]["vp"]) and np.array_equal(g["rho"], outs[0][0]["rho"])
+        assert all(np.array_equal(a, b) for a, b in zip(seis, outs[0][1]))
