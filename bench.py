@@ -675,6 +675,9 @@ def main():
                     help="stop after the timed steps (for counter-collection runs, where every dispatch is "
                          "instrumented): no per-launch timing pass, no roofline")
     ap.add_argument("--one-shot-sample", type=int, default=2, help="shots timed on the 1-shot path")
+    ap.add_argument("--synthetic-observed", action="store_true",
+                    help="seeded noise as observed data instead of a forward run on the true model (for "
+                         "counter-collection runs: then the only step-kernel dispatches are the gradients')")
     ap.add_argument("--no-warm-process", action="store_true",
                     help="measure without first running one step of the workload in a child process")
     ap.add_argument("--hbm-triad", action="store_true",
@@ -713,8 +716,16 @@ def main():
         print(json.dumps({"workload": wl["name"], "forward_only": True, "steps": args.steps,
                           "ms_per_forward": round(ms, 3)}))
         return
-    for g in wl.get("groups", [wl["shots"]]):
-        sw.swforward(p, wl["mp_true"], g, runparams=sw.RunParameters(**wl["run"]))
+    if args.synthetic_observed:
+        # seeded noise in place of the true model's traces: the bytes a
+        # gradient moves do not depend on the data
+        rng = np.random.default_rng(7)
+        for g in wl.get("groups", [wl["shots"]]):
+            for sh in g:
+                sh.recs.seismograms[...] = 1e-3 * rng.standard_normal(sh.recs.seismograms.shape)
+    else:
+        for g in wl.get("groups", [wl["shots"]]):
+            sw.swforward(p, wl["mp_true"], g, runparams=sw.RunParameters(**wl["run"]))
     log("observed data ready")
     # one plan per call: the elastic workload runs its two source kinds as
     # two calls, as the reference requires
@@ -767,7 +778,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded layered model, Ricker sources, observed data from a forward run)",
-        "config": {"workload": wl["name"], **wl["config"], "segment_used": S, "batches": batches},
+        "config": {"workload": wl["name"], **wl["config"], "segment_used": S, "batches": batches,
+                   "kernel_variant": [int(v) for v in info.kernel_variant]},
         "roofline": {"bound": "hbm", "achieved": fam[dom]["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fam[dom]["achieved_GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(wl["name"], kern[dom]),

@@ -1144,17 +1144,19 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
   const int tid = threadIdx.y * TX + threadIdx.x;
   const T *ucx = a.ucur + (int64_t)s * 2 * N, *ucz = ucx + N;
   T *un = a.unew + (int64_t)s * 2 * N;
+  constexpr int NU = (UW * UH + NT - 1) / NT;    // staged ucur cells per thread
+  // Every load below is unconditional (indexes past the last cell of a
+  // region are clamped onto it and their values not used), so they are all
+  // in flight together before the first LDS store.
   T ml[NSQ], mm[NSQ], mh[NSQ];
 #pragma unroll
   for (int k = 0; k < NSQ; k++) {
-    const int q = tid + k * NT;
-    if (q < SW * SH) {
-      const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
-      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
-      ml[k] = a.lam[c];
-      mm[k] = a.mu[c];
-      mh[k] = a.muhh[c];
-    }
+    const int q = min(tid + k * NT, SW * SH - 1);
+    const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    ml[k] = a.lam[c];
+    mm[k] = a.mu[c];
+    mh[k] = a.muhh[c];
   }
   const int i = I1 + threadIdx.x;
   T ox[NR], oz[NR], rx[NR], rz[NR];
@@ -1167,11 +1169,25 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
     rx[k] = a.rdx[c];
     rz[k] = a.rdz[c];
   }
-  for (int q = tid; q < UW * UH; q += NT) {
+  // ucur on the tile + 4: inside the grid for an interior tile (column nx,
+  // outside the ux grid, only when h = 0: loaded, stored as zero)
+  T vx[NU], vz[NU];
+#pragma unroll
+  for (int k = 0; k < NU; k++) {
+    const int q = min(tid + k * NT, UW * UH - 1);
     const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
     const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
-    lux[q] = x <= nx - 1 ? ucx[c] : T(0);
-    luz[q] = ucz[c];
+    vx[k] = ucx[c];
+    vz[k] = ucz[c];
+  }
+#pragma unroll
+  for (int k = 0; k < NU; k++) {
+    const int q = tid + k * NT;
+    if (q < UW * UH) {
+      const int x = I1 - 4 + q % UW;
+      lux[q] = x <= nx - 1 ? vx[k] : T(0);
+      luz[q] = vz[k];
+    }
   }
   __syncthreads();
   const TileA<T, UW> UX{lux, I1 - 4, J1 - 4}, UZ{luz, I1 - 4, J1 - 4};
@@ -1277,12 +1293,29 @@ __global__ __launch_bounds__(TX *TZ) void el_tile_kernel(ElArgs<T> a) {
     return;
   }
 
-  // ucur on the tile + 4 (zero outside the ux / uz grids)
-  for (int q = tid; q < UW * UH; q += TX * TZ) {
-    const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
-    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
-    lux[q] = (x >= 1 && x <= nx - 1 && j >= 1 && j <= nz) ? ucx[c] : T(0);
-    luz[q] = (x >= 1 && x <= nx && j >= 1 && j <= nz - 1) ? ucz[c] : T(0);
+  // ucur on the tile + 4 (zero outside the ux / uz grids); the loads read
+  // a clamped in-grid cell where the value is not used, so they are all in
+  // flight before the first LDS store
+  {
+    constexpr int NU = (UW * UH + TX * TZ - 1) / (TX * TZ);
+    T vx[NU], vz[NU];
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+      const int q = min(tid + k * TX * TZ, UW * UH - 1);
+      const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
+      const int64_t c = (int64_t)(min(max(j, 1), nz) - 1) * nx + (min(max(x, 1), nx) - 1);
+      vx[k] = ucx[c];
+      vz[k] = ucz[c];
+    }
+#pragma unroll
+    for (int k = 0; k < NU; k++) {
+      const int q = tid + k * TX * TZ;
+      if (q < UW * UH) {
+        const int r = q / UW, x = I1 - 4 + (q - r * UW), j = J1 - 4 + r;
+        lux[q] = (x >= 1 && x <= nx - 1 && j >= 1 && j <= nz) ? vx[k] : T(0);
+        luz[q] = (x >= 1 && x <= nx && j >= 1 && j <= nz - 1) ? vz[k] : T(0);
+      }
+    }
   }
   __syncthreads();
   const TileA<T, UW> UX{lux, I1 - 4, J1 - 4}, UZ{luz, I1 - 4, J1 - 4};

@@ -182,6 +182,10 @@ class KernelTimer {
   void begin(hipStream_t s);
   // kind 0: forward-step kernel family, 1: adjoint-step kernel family
   void end(hipStream_t s, int kind);
+  // The event pair of one launch made with hipExtLaunchKernelGGL, which
+  // records them at the kernel's own start and end (no event packets
+  // around the launch); both null when timing is off.
+  void dispatch_events(int kind, hipEvent_t *start, hipEvent_t *stop);
   // Sums elapsed times of recorded pairs per kind (stream synchronized).
   void collect(double ms[2], int64_t n[2]);
   void reset() { used_ = 0; kinds_.clear(); }

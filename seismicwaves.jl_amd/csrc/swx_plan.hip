@@ -59,6 +59,20 @@ void KernelTimer::end(hipStream_t s, int kind) {
   used_ += 2;
 }
 
+void KernelTimer::dispatch_events(int kind, hipEvent_t *start, hipEvent_t *stop) {
+  *start = *stop = nullptr;
+  if (!enabled) return;
+  while (used_ + 2 > ev_.size()) {
+    hipEvent_t e;
+    SWX_HIP(hipEventCreate(&e));
+    ev_.push_back(e);
+  }
+  *start = ev_[used_];
+  *stop = ev_[used_ + 1];
+  kinds_.push_back(kind);
+  used_ += 2;
+}
+
 void KernelTimer::collect(double ms[2], int64_t n[2]) {
   ms[0] = ms[1] = 0;
   n[0] = n[1] = 0;
