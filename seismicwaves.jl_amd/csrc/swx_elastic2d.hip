@@ -1180,6 +1180,24 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
     vx[k] = ucx[c];
     vz[k] = ucz[c];
   }
+  // adjoint with the correlation: the forward field of step corr_step
+  // (history level h1) on the stress box, staged in LDS after the
+  // displacements, and the history levels h0, h2 and the five gradient
+  // fields at the tile's own cells (loaded after the first barrier)
+  const T *h0p = nullptr;
+  if (ADJ && corr) h0p = a.hist + (int64_t)(a.corr_step - a.cbase - 1) * a.hslot + (int64_t)s * 2 * N;
+  T hx[ADJ ? NSQ : 1], hz[ADJ ? NSQ : 1];
+  if (ADJ && corr) {
+    const T *h1p = h0p + a.hslot;
+#pragma unroll
+    for (int k = 0; k < NSQ; k++) {
+      const int q = min(tid + k * NT, SW * SH - 1);
+      const int r = q / SW, x = I1 - 2 + (q - r * SW), j = J1 - 2 + r;
+      const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+      hx[k] = h1p[c];
+      hz[k] = h1p[N + c];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NU; k++) {
     const int q = tid + k * NT;
@@ -1191,6 +1209,22 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
   }
   __syncthreads();
   const TileA<T, UW> UX{lux, I1 - 4, J1 - 4}, UZ{luz, I1 - 4, J1 - 4};
+  T c0x[ADJ ? NR : 1], c0z[ADJ ? NR : 1], c2x[ADJ ? NR : 1], c2z[ADJ ? NR : 1], cg[ADJ ? NR : 1][5];
+  if (ADJ && corr) {
+    const T *h2p = h0p + 2 * a.hslot;
+    const T *g = a.grad + (int64_t)s * a.gstride;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int j = J1 + threadIdx.y + k * TZ;
+      const int64_t c = (int64_t)(i - 1) + (int64_t)(j - 1) * nx;
+      c0x[k] = h0p[c];
+      c0z[k] = h0p[N + c];
+      c2x[k] = h2p[c];
+      c2z[k] = h2p[N + c];
+#pragma unroll
+      for (int f = 0; f < 5; f++) cg[k][f] = g[f * N + c];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NSQ; k++) {
     const int q = tid + k * NT;
@@ -1206,8 +1240,6 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
       const T duxz = d4<T>(UX(x, j - 1), UX(x, j), UX(x, j + 1), UX(x, j + 2), a.idz);
       const T c1 = duzx, c2 = duxz;
       lsxz[q] = mhh * (c1 + c2);
-      if (ADJ && corr && x >= I1 && x < I1 + TX && j >= J1 && j < J1 + R)
-        el_tile_correlate<T>(a, s, x, j, UX(x, j), UZ(x, j), true, true, dux, duz, duzx, duxz);
     }
   }
   __syncthreads();
@@ -1240,6 +1272,48 @@ __device__ __forceinline__ void el_tile_lean(const ElArgs<T> &a, int s, int nts,
         hs[a.hslot + N] = curz;
       }
     }
+  }
+  if (!(ADJ && corr)) return;
+  // correlate_gradients! of step corr_step at the tile's own stress cells
+  // (el_tile_correlate's expressions for an interior cell, same operand
+  // order): h1 goes to the stress planes of LDS, which the displacements
+  // have consumed
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < (ADJ ? NSQ : 0); k++) {
+    const int q = tid + k * NT;
+    if (q < SW * SH) {
+      lsxx[q] = hx[k];
+      lszz[q] = hz[k];
+    }
+  }
+  __syncthreads();
+  const TileA<T, SW> FX{lsxx, I1 - 2, J1 - 2}, FZ{lszz, I1 - 2, J1 - 2};
+  T *g = a.grad + (int64_t)s * a.gstride;
+#pragma unroll
+  for (int k = 0; k < (ADJ ? NR : 0); k++) {
+    const int x = i, j = J1 + threadIdx.y + k * TZ;
+    const int64_t c = (int64_t)(j - 1) * nx + (x - 1);
+    const T ux = UX(x, j), uz = UZ(x, j);
+    const T dux = d4<T>(UX(x - 2, j), UX(x - 1, j), UX(x, j), UX(x + 1, j), a.idx);
+    const T duz = d4<T>(UZ(x, j - 2), UZ(x, j - 1), UZ(x, j), UZ(x, j + 1), a.idz);
+    const T duzx = d4<T>(UZ(x - 1, j), UZ(x, j), UZ(x + 1, j), UZ(x + 2, j), a.idx);
+    const T duxz = d4<T>(UX(x, j - 1), UX(x, j), UX(x, j + 1), UX(x, j + 2), a.idz);
+    // the statements of el_tile_correlate (the x term is formed on its own
+    // before the sum, the z term inside it), so that contraction into FMAs
+    // happens at the same places
+    const T v = ux * (c0x[k] - T(2) * FX(x, j) + c2x[k]) * a.idt2;
+    g[c] = cg[k][0] + v;
+    g[N + c] = cg[k][1] + uz * (c0z[k] - T(2) * FZ(x, j) + c2z[k]) * a.idt2;
+    const T exx = d4<T>(FX(x - 2, j), FX(x - 1, j), FX(x, j), FX(x + 1, j), a.idx);
+    const T ezz = d4<T>(FZ(x, j - 2), FZ(x, j - 1), FZ(x, j), FZ(x, j + 1), a.idz);
+    const T dv = exx + ezz, dva = dux + duz;
+    g[2 * N + c] = cg[k][2] + dv * dva;
+    g[3 * N + c] = cg[k][3] + T(2) * (exx * dux + ezz * duz);
+    const T exz = (d4<T>(FZ(x - 1, j), FZ(x, j), FZ(x + 1, j), FZ(x + 2, j), a.idx) +
+                   d4<T>(FX(x, j - 1), FX(x, j), FX(x, j + 1), FX(x, j + 2), a.idz)) / T(2);
+    const T exza = (duzx + duxz) / T(2);
+    g[4 * N + c] = cg[k][4] + T(2) * (exz * exza + exz * exza);
   }
 }
 
