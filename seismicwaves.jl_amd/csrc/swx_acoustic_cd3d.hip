@@ -319,28 +319,37 @@ __device__ __forceinline__ void cd3d_lds_column(const Cd3dArgs<T> &a, T *sp, int
     pm = pc;
     pc = pp;
   };
-  // Three plane buffers used in turn (unrolled by three, as in
-  // cd3d_lean_column); plane K1 (the last plane's z neighbour) is loaded
-  // too, which the DEEP bound keeps inside the grid.
+  // Four plane buffers used in turn (the loop is unrolled by four): while
+  // plane k is stepped, the loads of planes k+1 .. k+3 are in flight (the
+  // kernel's register budget is set by the general march, so the deeper
+  // queue costs no occupancy).  Plane K1 (the last plane's z neighbour) is
+  // loaded too, which the DEEP bound keeps inside the grid.
   Plane A = load_plane(K0);
   Plane B = load_plane(K0 + 1);
+  Plane C = load_plane(min(K0 + 2, K1));
+  Plane D;
   stage(0, A);
   pc = A.ctr;
   __syncthreads();
-  for (int k = K0; k < K1; k += 3) {
-    Plane C = load_plane(min(k + 2, K1));
+  for (int k = K0; k < K1; k += 4) {
+    D = load_plane(min(k + 3, K1));
     stage((k + 1 - K0) & 1, B);
     step(k, A, B);
     __syncthreads();
     if (k + 1 >= K1) break;
-    A = load_plane(min(k + 3, K1));
+    A = load_plane(min(k + 4, K1));
     stage((k + 2 - K0) & 1, C);
     step(k + 1, B, C);
     __syncthreads();
     if (k + 2 >= K1) break;
-    B = load_plane(min(k + 4, K1));
-    stage((k + 3 - K0) & 1, A);
-    step(k + 2, C, A);
+    B = load_plane(min(k + 5, K1));
+    stage((k + 3 - K0) & 1, D);
+    step(k + 2, C, D);
+    __syncthreads();
+    if (k + 3 >= K1) break;
+    C = load_plane(min(k + 6, K1));
+    stage((k + 4 - K0) & 1, A);
+    step(k + 3, D, A);
     __syncthreads();
   }
 }
