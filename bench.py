@@ -479,11 +479,16 @@ class Runner:
         self._loaded = i
 
     def step(self):
+        """One gradient; `device_ms` is the time between stream events
+        around each forward and adjoint loop of it (no per-launch events)."""
         self.gsum[:] = 0
+        self.device_ms = 0.0
         for i, b in enumerate(self.batches):
             self._load(i)
             self.plan.forward(self._bufs[i][0], True)
+            self.device_ms += self.plan.info().ms_forward
             self.plan.adjoint_l2(self.grad, False, len(b))
+            self.device_ms += self.plan.info().ms_adjoint
             self.gsum += self.grad
 
     def timed_kernels(self):
@@ -736,10 +741,13 @@ def main():
             r.step()
     log("warmup done")
     t0 = time.perf_counter()
+    dev_ms = 0.0
     for _ in range(args.steps):
         for r in runs:
             r.step()
+            dev_ms += r.device_ms
     wall = time.perf_counter() - t0
+    dev_ms /= max(args.steps, 1)
     ms_step = wall / args.steps * 1e3
     log(f"timed {args.steps} steps: {ms_step:.1f} ms/step")
     if args.timed_steps_only:
@@ -748,10 +756,8 @@ def main():
         print(json.dumps({"workload": wl["name"], "steps": args.steps, "ms_per_step": round(ms_step, 3)}))
         return
     kms, kn = [0.0, 0.0], [0, 0]
-    dev_ms = 0.0
     for r in runs:
-        m, n, dv = r.timed_kernels()
-        dev_ms += dv
+        m, n, _ = r.timed_kernels()
         for k in range(2):
             kms[k] += m[k]
             kn[k] += n[k]
