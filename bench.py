@@ -360,9 +360,18 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
         else:
             # CD: r p^n, p^n-1, fact; w p^n+1 (4e); + w D (5e).  Adjoint with
             # the fused correlation: + r D, grad read-modify-write (7e).
-            fwd_b += (nt - last) * 4 * e * N + last * 5 * e * N + nt * strip
-            fwd_b += rep * 5 * e * N + rep * strip
-            fwd_n += nt + rep
+            if physics == "cd2d" and len(variant) > 1 and variant[1] == 2:
+                # two levels per forward / replay launch: r p^n, p^n-1, fact
+                # once per launch (3e), w each level (1e) and its D (1e);
+                # launches start at levels 1, 3, 5, ... of the forward and
+                # of every replayed segment
+                nl = (nt + 1) // 2 + kl * ((S + 1) // 2)
+                fwd_b += nl * 3 * e * N + (nt + rep) * e * N + (last + rep) * e * N + (nt + rep) * strip
+                fwd_n += nl
+            else:
+                fwd_b += (nt - last) * 4 * e * N + last * 5 * e * N + nt * strip
+                fwd_b += rep * 5 * e * N + rep * strip
+                fwd_n += nt + rep
             nseg = kl + 1
             adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
             adj_n += nt
@@ -418,8 +427,10 @@ def kernel_names(physics, variant):
         else:
             k = f"cd3d_step_kernel<float, {v}, 0>"
         return f"{k} (forward)", f"{k} (adjoint)"
-    # [rows per thread, 0]
+    # [rows per thread, levels per forward / replay launch]
     k = f"cd2d_step_kernel<float, {variant[0] or 2}>"
+    if len(variant) > 1 and variant[1] == 2:
+        return "cd2d_step2_kernel<float> (forward)", f"{k} (adjoint)"
     return f"{k} (forward)", f"{k} (adjoint)"
 
 

@@ -9,11 +9,14 @@
 //   correlate  src/models/acoustic/backends/shared/correlate_gradient_xPU.jl:1-10
 //
 // Device layout (x fastest, exactly the Julia layout), one slab per shot:
-//   p      [3][ns][nx*nz]  rotating time levels, p^k lives in slot k mod 3
-//                          (the reference swaps pold/pcur/pnew, :122-124)
-//   psi_x  [2][ns][2h*nz]  double buffered by step parity so a cell can
-//   psi_z  [2][ns][nx*2h]  recompute its left neighbour's new value
-//   xi_x   [ns][(2h+2)*nz], xi_z [ns][nx*(2h+2)]
+//   p      [4][ns][nx*nz]  time levels, p^k lives in slot k mod 4 (the
+//                          reference swaps pold/pcur/pnew, :122-124; a
+//                          fourth slot lets a launch step two levels)
+//   psi_x  [3][ns][2h*nz]  level k in buffer k mod 3, so a cell can
+//   psi_z  [3][ns][nx*2h]  recompute its left neighbour's new value
+//   xi_x   [3][ns][(2h+2)*nz], xi_z [3][ns][nx*(2h+2)]
+//   (the adjoint state keeps three levels of p, psi by parity and xi in
+//   place)
 //   D      [S][ns][nx*nz]  p^{it-2} - 2 p^{it-1} + p^{it} of the segment
 //                          being replayed (what correlate_gradient! needs)
 //   ckpt   [K-2][...]      p^{c-1}, p^c, psi, xi at c = k*S
@@ -44,8 +47,8 @@ struct Cd2dStepArgs {
   T *pnew;
   const T *psix_o; T *psix_n; int64_t spx;   // psi_x old/new, per-shot stride
   const T *psiz_o; T *psiz_n; int64_t spz;
-  T *xix; int64_t sxx;
-  T *xiz; int64_t sxz;
+  const T *xix_o; T *xix; int64_t sxx;   // xi_x of the previous / this level (may be one array)
+  const T *xiz_o; T *xiz; int64_t sxz;
   const T *cpml; int cstride;                // per-shot packed coefficients
   const int *nt_shot;
   int it;                    // time step index (1-based)
@@ -195,7 +198,7 @@ __device__ __forceinline__ void cd2d_cell(const Cd2dStepArgs<T> &a, int s, int n
       const T psn = cx.bh(kn) * a.psix_o[qn] + cx.ah(kn) * ((pc - pl) * a.idx);
       const T dpsi = (psx - psn) * a.idx;
       const int64_t qx = (int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + kx;
-      const T xi = cx.b(kx) * a.xix[qx] + cx.a(kx) * (d2x + dpsi);
+      const T xi = cx.b(kx) * a.xix_o[qx] + cx.a(kx) * (d2x + dpsi);
       a.xix[qx] = xi;
       lx = d2x + dpsi + xi;
     }
@@ -205,7 +208,7 @@ __device__ __forceinline__ void cd2d_cell(const Cd2dStepArgs<T> &a, int s, int n
       const T psn = cz.bh(kn) * a.psiz_o[qn] + cz.ah(kn) * ((pc - pd) * a.idz);
       const T dpsi = (psz - psn) * a.idz;
       const int64_t qz = (int64_t)s * a.sxz + (int64_t)kz * nx + i;
-      const T xi = cz.b(kz) * a.xiz[qz] + cz.a(kz) * (d2z + dpsi);
+      const T xi = cz.b(kz) * a.xiz_o[qz] + cz.a(kz) * (d2z + dpsi);
       a.xiz[qz] = xi;
       lz = d2z + dpsi + xi;
     }
@@ -283,6 +286,290 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step_kernel(Cd2dStepArgs<T> a) {
   for (int r = 0; r < R; r++) cd2d_cell(a, s, nts, i, J0 + threadIdx.y + r * TZ);
 }
 
+// ---------------------------------------------------------------------------
+// Two forward / replay levels per launch (cd2d_step2_kernel).
+//
+// A launch at it steps the levels it and it+1.  A block owns a TX x H tile
+// (H = TZ * 2 rows) and computes level it on the tile plus a one-cell ring
+// (its "region", kept in LDS with the level-it C-PML memory variables),
+// then level it+1 on the tile from the region.  The ring cells belong to
+// the neighbouring tiles, whose blocks compute them as well; only the
+// owning block writes a cell.  Because a block reads levels it-1 and it-2
+// of the ring while its neighbours write levels it and it+1, the forward
+// state keeps four pressure levels (slot k mod 4) and three levels of the
+// C-PML memory variables (buffer k mod 3).  Every cell update is the
+// expression of cd2d_cell / cd2d_lean_cells in the same operand order, so
+// the results are bitwise those of two one-level launches.
+
+constexpr int S2H = TZ * 2;          // tile rows
+constexpr int S2W = TX + 2;          // region columns (tile + ring)
+constexpr int S2R = S2H + 2;         // region rows
+constexpr int S2N = S2W * S2R;       // region cells
+constexpr int S2Q = (S2N + TX * TZ - 1) / (TX * TZ);   // region cells per thread
+
+// Replay-history slot of level itl for shot s (or -1).
+template <typename T>
+__device__ __forceinline__ int cd2d_dslot(const Cd2dStepArgs<T> &a, int nts, int itl) {
+  if (a.dmode == 1) {
+    const int seg0 = ((nts - 1) / a.S) * a.S;
+    return itl > seg0 ? itl - seg0 - 1 : -1;
+  }
+  return a.dmode == 2 ? itl - a.cbase - 1 : -1;
+}
+
+// Results of one cell update (cd2d_cell's, without the stores).
+template <typename T>
+struct Cd2dCell {
+  T pnv, dval, psx, psz, xix, xiz;
+  int kx, kz;
+  bool xup, zup;   // xi_x / xi_z updated
+};
+
+// cd2d_cell's update of cell (i, j) at level itl (forward: sources),
+// reading the current level through pc(di, dj), the old C-PML memory
+// variables through psxo(cell i, strip k) / pszo(cell j, strip k) and
+// xixo() / xizo() (the cell's own xi).
+template <typename T, typename PC, typename PSX, typename PSZ, typename XIX, typename XIZ>
+__device__ __forceinline__ Cd2dCell<T> cd2d_cell_eval(const Cd2dStepArgs<T> &a, int s, int nts, int itl, int i,
+                                                      int j, PC pc_at, T pov, T fa, PSX psxo, PSZ pszo, XIX xixo,
+                                                      XIZ xizo) {
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int64_t c = (int64_t)j * nx + i;
+  const bool interior = (i >= 1 && i <= nx - 2 && j >= 1 && j <= nz - 2);
+  const T pc = pc_at(0, 0);
+  const T pl = interior ? pc_at(-1, 0) : T(0), pr = interior ? pc_at(1, 0) : T(0);
+  const T pd = interior ? pc_at(0, -1) : T(0), pu = interior ? pc_at(0, 1) : T(0);
+  const T *cb = a.cpml + (int64_t)s * a.cstride;
+  const AxisCoef<T> cx = axis_coef(cb, h, 0);
+  const AxisCoef<T> cz = axis_coef(cb, h, 1);
+  Cd2dCell<T> o;
+  o.kx = -1;
+  if (i < h) o.kx = i;
+  else if (i >= nx - h - 1 && i <= nx - 2) o.kx = i + 1 - nx + 2 * h;
+  o.psx = 0;
+  if (o.kx >= 0) {
+    const T d = (pc_at(1, 0) - pc) * a.idx;
+    o.psx = cx.bh(o.kx) * psxo(i, o.kx) + cx.ah(o.kx) * d;
+  }
+  o.kz = -1;
+  if (j < h) o.kz = j;
+  else if (j >= nz - h - 1 && j <= nz - 2) o.kz = j + 1 - nz + 2 * h;
+  o.psz = 0;
+  if (o.kz >= 0) {
+    const T d = (pc_at(0, 1) - pc) * a.idz;
+    o.psz = cz.bh(o.kz) * pszo(j, o.kz) + cz.ah(o.kz) * d;
+  }
+  o.xup = o.zup = false;
+  o.xix = o.xiz = 0;
+  T pnv = 0, flap = 0, inj = 0;
+  if (interior) {
+    const T d2x = (pl - T(2) * pc + pr) * a.idx2;
+    const T d2z = (pd - T(2) * pc + pu) * a.idz2;
+    T lx = d2x, lz = d2z;
+    if (i < h || i >= nx - h) {
+      const int kn = o.kx - 1;
+      const T psn = cx.bh(kn) * psxo(i - 1, kn) + cx.ah(kn) * ((pc - pl) * a.idx);
+      const T dpsi = (o.psx - psn) * a.idx;
+      const T xi = cx.b(o.kx) * xixo() + cx.a(o.kx) * (d2x + dpsi);
+      o.xix = xi;
+      o.xup = true;
+      lx = d2x + dpsi + xi;
+    }
+    if (j < h || j >= nz - h) {
+      const int kn = o.kz - 1;
+      const T psn = cz.bh(kn) * pszo(j - 1, kn) + cz.ah(kn) * ((pc - pd) * a.idz);
+      const T dpsi = (o.psz - psn) * a.idz;
+      const T xi = cz.b(o.kz) * xizo() + cz.a(o.kz) * (d2z + dpsi);
+      o.xiz = xi;
+      o.zup = true;
+      lz = d2z + dpsi + xi;
+    }
+    flap = fa * (lx + lz);
+    pnv = T(2) * pc - pov + flap;
+  }
+  if (!interior) pnv = pov;
+  const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c);
+  if (u >= 0) {
+    const T *tf = a.srctf + a.pt.off_tf[s] + (itl - 1);
+    for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) {
+      const T v = tf[(int64_t)nts * a.pt.usrc_list[k]];
+      pnv += v;
+      inj += v;
+    }
+  }
+  o.pnv = pnv;
+  o.dval = interior ? flap + inj : (pov - T(2) * pc) + pnv;
+  return o;
+}
+
+// Stores of an owned cell at level itl: pressure, memory variables,
+// replay history and receiver traces.
+template <typename T>
+__device__ __forceinline__ void cd2d_cell_store(const Cd2dStepArgs<T> &a, const Cd2dCell<T> &o, int s, int nts,
+                                                int itl, int i, int j, T *pn, T *psxn, T *pszn, T *xixn, T *xizn) {
+  const int nx = a.nx, h = a.h;
+  const int64_t so = (int64_t)s * a.N;
+  const int64_t c = (int64_t)j * nx + i;
+  pn[so + c] = o.pnv;
+  if (o.kx >= 0) psxn[(int64_t)s * a.spx + (int64_t)j * (2 * h) + o.kx] = o.psx;
+  if (o.kz >= 0) pszn[(int64_t)s * a.spz + (int64_t)o.kz * nx + i] = o.psz;
+  if (o.xup) xixn[(int64_t)s * a.sxx + (int64_t)j * (2 * h + 2) + o.kx] = o.xix;
+  if (o.zup) xizn[(int64_t)s * a.sxz + (int64_t)o.kz * nx + i] = o.xiz;
+  if (a.traces) {
+    const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
+    if (r >= 0) {
+      T *tr = a.traces + a.pt.off_tr[s] + (itl - 1);
+      for (int k = a.pt.urec_ptr[r]; k < a.pt.urec_ptr[r + 1]; k++) tr[(int64_t)nts * a.pt.urec_list[k]] = o.pnv;
+    }
+  }
+  const int slot = cd2d_dslot(a, nts, itl);
+  if (slot >= 0) a.dbuf[(int64_t)slot * a.dslot + so + c] = o.dval;
+}
+
+// Arguments of the second level of a two-level launch: the slots of
+// levels it+1 (pressure, memory variables) next to those of level it in
+// Cd2dStepArgs (pold = it-2, pcur = it-1, pnew = it).
+template <typename T>
+struct Cd2dStep2Args {
+  Cd2dStepArgs<T> a;
+  T *pnew2;
+  T *psix_n2, *psiz_n2, *xix_n2, *xiz_n2;
+};
+
+// Is the region of tile (I0, J0) of shot s plain: every region cell inside
+// the grid, outside the C-PML strips, and no source or receiver on the
+// region's rows?
+template <typename T>
+__device__ __forceinline__ bool cd2d_step2_is_lean(const Cd2dStepArgs<T> &a, int s, int I0, int J0) {
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int i0 = I0 - 1, i1 = I0 + TX, j0 = J0 - 1, j1 = J0 + S2H;
+  if (!(i0 >= max(h, 1) && i1 <= nx - h - 2 && j0 >= max(h, 1) && j1 <= nz - h - 2)) return false;
+  const int64_t o = (int64_t)s * (a.pt.nrows + 1);
+  if (a.pt.srow[o + j0] != a.pt.srow[o + j1 + 1]) return false;
+  if (a.traces && a.pt.rrow[o + j0] != a.pt.rrow[o + j1 + 1]) return false;
+  return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2) {
+  const Cd2dStepArgs<T> &a = A2.a;
+  __shared__ T sA[S2R][S2W];    // level it on the region
+  __shared__ T sC[S2R][S2W];    // level it-1 on the region
+  __shared__ T sX[S2R][S2W];    // psi_x of level it (strip cells)
+  __shared__ T sZ[S2R][S2W];    // psi_z of level it
+  __shared__ T sXI[S2R][S2W];   // xi_x of level it
+  __shared__ T sZI[S2R][S2W];   // xi_z of level it
+  const int b = xcd_tile2(blockIdx.x, gridDim.x);
+  const int t = b / a.nbx;
+  const int bx = b - t * a.nbx, by = t % a.nby, s = t / a.nby;
+  const int nts = a.nt_shot[s];
+  if (a.it > nts || nts <= a.act_min) return;
+  const bool two = a.it + 1 <= nts;   // the shot also takes level it+1
+  const int nx = a.nx, nz = a.nz, h = a.h;
+  const int I0 = bx * TX, J0 = by * S2H;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int64_t so = (int64_t)s * a.N;
+  const T *pc_ = a.pcur + so;
+  const T *po_ = a.pold + so;
+  if (a.lean && cd2d_step2_is_lean(a, s, I0, J0)) {
+    // level it on the region: every load first (the region's current
+    // level plus one cell around it is read from L1 / L2 lines)
+    T pc[S2Q], pl[S2Q], pr[S2Q], pd[S2Q], pu[S2Q], pov[S2Q], fa[S2Q];
+#pragma unroll
+    for (int k = 0; k < S2Q; k++) {
+      const int q = min(tid + k * TX * TZ, S2N - 1);
+      const int ly = q / S2W, lx = q - ly * S2W;
+      const int c = (J0 - 1 + ly) * nx + (I0 - 1 + lx);
+      pc[k] = pc_[c];
+      pl[k] = pc_[c - 1];
+      pr[k] = pc_[c + 1];
+      pd[k] = pc_[c - nx];
+      pu[k] = pc_[c + nx];
+      pov[k] = po_[c];
+      fa[k] = a.fact[c];
+    }
+    const int slotA = cd2d_dslot(a, nts, a.it), slotB = cd2d_dslot(a, nts, a.it + 1);
+#pragma unroll
+    for (int k = 0; k < S2Q; k++) {
+      const int q = tid + k * TX * TZ;
+      if (q < S2N) {
+        const int ly = q / S2W, lx = q - ly * S2W;
+        const T d2x = (pl[k] - T(2) * pc[k] + pr[k]) * a.idx2;
+        const T d2z = (pd[k] - T(2) * pc[k] + pu[k]) * a.idz2;
+        const T lxx = d2x, lzz = d2z;
+        const T flap = fa[k] * (lxx + lzz);
+        const T pnv = T(2) * pc[k] - pov[k] + flap;
+        const T inj = 0;
+        sA[ly][lx] = pnv;
+        sC[ly][lx] = pc[k];
+        if (lx >= 1 && lx <= TX && ly >= 1 && ly <= S2H) {
+          const int64_t c = (int64_t)(J0 - 1 + ly) * nx + (I0 - 1 + lx);
+          a.pnew[so + c] = pnv;
+          if (slotA >= 0) a.dbuf[(int64_t)slotA * a.dslot + so + c] = flap + inj;
+        }
+      }
+    }
+    if (!two) return;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int lx = threadIdx.x + 1, ly = threadIdx.y + r * TZ + 1;
+      const int64_t c = (int64_t)(J0 - 1 + ly) * nx + (I0 - 1 + lx);
+      const T p = sA[ly][lx];
+      const T d2x = (sA[ly][lx - 1] - T(2) * p + sA[ly][lx + 1]) * a.idx2;
+      const T d2z = (sA[ly - 1][lx] - T(2) * p + sA[ly + 1][lx]) * a.idz2;
+      const T lxx = d2x, lzz = d2z;
+      const T flap = a.fact[c] * (lxx + lzz);
+      const T pnv = T(2) * p - sC[ly][lx] + flap;
+      const T inj = 0;
+      A2.pnew2[so + c] = pnv;
+      if (slotB >= 0) a.dbuf[(int64_t)slotB * a.dslot + so + c] = flap + inj;
+    }
+    return;
+  }
+  // General tile: level it on the region with cd2d_cell's branches (cells
+  // outside the grid are skipped: only interior cells read their
+  // neighbours, and those are inside the grid)
+  const T *psxo_ = a.psix_o + (int64_t)s * a.spx, *pszo_ = a.psiz_o + (int64_t)s * a.spz;
+  const T *xixo_ = a.xix_o + (int64_t)s * a.sxx, *xizo_ = a.xiz_o + (int64_t)s * a.sxz;
+  for (int q = tid; q < S2N; q += TX * TZ) {
+    const int ly = q / S2W, lx = q - ly * S2W;
+    const int i = I0 - 1 + lx, j = J0 - 1 + ly;
+    if (i < 0 || i >= nx || j < 0 || j >= nz) continue;
+    const int64_t c = (int64_t)j * nx + i;
+    auto pcat = [&](int di, int dj) { return pc_[c + di + (int64_t)dj * nx]; };
+    auto psxo = [&](int, int k) { return psxo_[(int64_t)j * (2 * h) + k]; };
+    auto pszo = [&](int, int k) { return pszo_[(int64_t)k * nx + i]; };
+    const Cd2dCell<T> o = cd2d_cell_eval<T>(
+        a, s, nts, a.it, i, j, pcat, po_[c], a.fact[c], psxo, pszo,
+        [&]() { return xixo_[(int64_t)j * (2 * h + 2) + (i < h ? i : i + 1 - nx + 2 * h)]; },
+        [&]() { return xizo_[(int64_t)(j < h ? j : j + 1 - nz + 2 * h) * nx + i]; });
+    sA[ly][lx] = o.pnv;
+    sC[ly][lx] = pc_[c];
+    sX[ly][lx] = o.psx;
+    sZ[ly][lx] = o.psz;
+    sXI[ly][lx] = o.xix;
+    sZI[ly][lx] = o.xiz;
+    if (lx >= 1 && lx <= TX && ly >= 1 && ly <= S2H)
+      cd2d_cell_store(a, o, s, nts, a.it, i, j, a.pnew, a.psix_n, a.psiz_n, a.xix, a.xiz);
+  }
+  if (!two) return;
+  __syncthreads();
+  // level it+1 on the tile, from the region in LDS
+  for (int r = 0; r < 2; r++) {
+    const int lx = threadIdx.x + 1, ly = threadIdx.y + r * TZ + 1;
+    const int i = I0 - 1 + lx, j = J0 - 1 + ly;
+    if (i >= nx || j >= nz) continue;
+    const int64_t c = (int64_t)j * nx + i;
+    auto pcat = [&](int di, int dj) { return sA[ly + dj][lx + di]; };
+    auto psxo = [&](int ii, int) { return sX[ly][ii - (I0 - 1)]; };
+    auto pszo = [&](int jj, int) { return sZ[jj - (J0 - 1)][lx]; };
+    const Cd2dCell<T> o = cd2d_cell_eval<T>(a, s, nts, a.it + 1, i, j, pcat, sC[ly][lx], a.fact[c], psxo, pszo,
+                                            [&]() { return sXI[ly][lx]; }, [&]() { return sZI[ly][lx]; });
+    cd2d_cell_store(a, o, s, nts, a.it + 1, i, j, A2.pnew2, A2.psix_n2, A2.psiz_n2, A2.xix_n2, A2.xiz_n2);
+  }
+}
+
 // grad += adjcur * D for the first step of a segment (the other steps are
 // fused into the adjoint stencil).
 template <typename T>
@@ -318,21 +605,35 @@ class Cd2dEngine final : public Engine {
       tuned_ = true;
     }
     if (const char *v = std::getenv("SWX_CD2D_LEAN")) lean_ = std::atoi(v) != 0;
+    // levels per forward / replay launch (1 or 2, cd2d_step2_kernel)
+    if (const char *v = std::getenv("SWX_CD2D_STEPS")) {
+      const int n = std::atoi(v);
+      SWX_REQUIRE(n == 1 || n == 2, SWX_EINVAL, "SWX_CD2D_STEPS must be 1 or 2");
+      steps_ = n;
+      steps_env_ = true;
+    }
     const size_t e = sizeof(T);
     fact_.alloc(N_ * e);
-    p_.alloc(3 * ns_ * N_ * e);
-    psix_.alloc(std::max<int64_t>(2 * ns_ * spx_, 1) * e);
-    psiz_.alloc(std::max<int64_t>(2 * ns_ * spz_, 1) * e);
-    xix_.alloc(ns_ * sxx_ * e);
-    xiz_.alloc(ns_ * sxz_ * e);
+    // forward state: four pressure levels (slot k mod 4) and the C-PML
+    // memory variables of three levels (buffer k mod 3), so that a launch
+    // stepping two levels never overwrites a level a neighbouring block
+    // still reads (cd2d_step2_kernel); the one-step kernel works on any
+    // distinct slots
+    p_.alloc(4 * ns_ * N_ * e);
+    psix_.alloc(std::max<int64_t>(3 * ns_ * spx_, 1) * e);
+    psiz_.alloc(std::max<int64_t>(3 * ns_ * spz_, 1) * e);
+    xix_.alloc(std::max<int64_t>(3 * ns_ * sxx_, 1) * e);
+    xiz_.alloc(std::max<int64_t>(3 * ns_ * sxz_, 1) * e);
     cpml_.alloc((int64_t)ns_ * cstride_ * e);
     state_bytes_ = fact_.bytes() + p_.bytes() + psix_.bytes() + psiz_.bytes() + xix_.bytes() + xiz_.bytes();
     if (d.gradient) {
       a_.alloc(3 * ns_ * N_ * e);
-      apsix_.alloc(psix_.bytes());
-      apsiz_.alloc(psiz_.bytes());
-      axix_.alloc(xix_.bytes());
-      axiz_.alloc(xiz_.bytes());
+      // adjoint state: three levels (slot k mod 3), C-PML memory variables
+      // double buffered by parity (psi) and updated in place (xi)
+      apsix_.alloc(std::max<int64_t>(2 * ns_ * spx_, 1) * e);
+      apsiz_.alloc(std::max<int64_t>(2 * ns_ * spz_, 1) * e);
+      axix_.alloc(ns_ * sxx_ * e);
+      axiz_.alloc(ns_ * sxz_ * e);
       grad_.alloc(ns_ * N_ * e);
       gout_.alloc(ns_ * N_ * e);
       state_bytes_ += a_.bytes() + apsix_.bytes() + apsiz_.bytes() + axix_.bytes() +
@@ -385,18 +686,26 @@ class Cd2dEngine final : public Engine {
     zero_forward_state();
     T *tr = c_.pts.d_traces.as<T>();
     if (!tuned_) tune_rows(save, tr);
-    for (int it = 1; it <= ntmax_; it++) {
+    if (!steps_tuned_) tune_steps(save, tr);
+    for (int it = 1; it <= ntmax_;) {
       progress("forward step", it, ntmax_);
-      step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
-      take_snapshots(c_, *this, it);
-      if (save && it % S_ == 0) {
-        const int k = it / S_;
-        if (k >= 1 && k <= K_ - 2) save_ckpt(k, it);
+      // two levels per launch: the state of both levels is in memory
+      // afterwards, so snapshots and checkpoints of either are taken after it
+      const int n = steps_ == 2 && it + 1 <= ntmax_ ? 2 : 1;
+      if (n == 2) step2_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
+      else step_fwd(it, /*act_min=*/0, save ? 1 : 0, /*cbase=*/0, tr);
+      for (int l = it; l < it + n; l++) {
+        take_snapshots(c_, *this, l);
+        if (save && l % S_ == 0) {
+          const int k = l / S_;
+          if (k >= 1 && k <= K_ - 2) save_ckpt(k, l);
+        }
       }
+      it += n;
     }
   }
 
-  // "pcur" (acou_forward.jl:55-57): level it is in slot it mod 3
+  // "pcur" (acou_forward.jl:55-57): level it is in slot it mod 4
   int snap_fields(int it, int s, SnapPiece *out) const override {
     out[0] = {P(it) + (int64_t)s * N_, (int64_t)(N_ * sizeof(T)), (int64_t)(N_ * sizeof(T)), 1};
     return 1;
@@ -420,8 +729,11 @@ class Cd2dEngine final : public Engine {
       for (int s = 0; s < nsh_; s++) need |= nt_[s] > cb + S_;
       if (need) {
         restore_ckpt(k, cb);
-        for (int it = cb + 1; it <= cb + S_; it++) {
-          step_fwd(it, cb + S_, 2, cb, nullptr);
+        for (int it = cb + 1; it <= cb + S_;) {
+          const int n = steps_ == 2 && it + 1 <= cb + S_ ? 2 : 1;
+          if (n == 2) step2_fwd(it, cb + S_, 2, cb, nullptr);
+          else step_fwd(it, cb + S_, 2, cb, nullptr);
+          it += n;
         }
       }
       const int top = std::min(cb + S_, ntmax_);
@@ -450,10 +762,16 @@ class Cd2dEngine final : public Engine {
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return dbuf_.bytes(); }
   int segment() const override { return S_; }
-  void variants(int32_t v[2]) const override { v[0] = rows_; v[1] = 0; }
+  // rows per thread of the one-level kernel; levels per forward launch
+  void variants(int32_t v[2]) const override { v[0] = rows_; v[1] = steps_; }
 
  private:
-  T *P(int k) const { return p_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
+  T *P(int k) const { return p_.as<T>() + (int64_t)(((k % 4) + 4) % 4) * ns_ * N_; }
+  // forward C-PML memory variables of level k
+  T *PSX(int k) const { return psix_.as<T>() + (int64_t)mod3(k) * ns_ * spx_; }
+  T *PSZ(int k) const { return psiz_.as<T>() + (int64_t)mod3(k) * ns_ * spz_; }
+  T *XIX(int k) const { return xix_.as<T>() + (int64_t)mod3(k) * ns_ * sxx_; }
+  T *XIZ(int k) const { return xiz_.as<T>() + (int64_t)mod3(k) * ns_ * sxz_; }
   T *A(int k) const { return a_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
 
   void zero_forward_state() {
@@ -526,11 +844,10 @@ class Cd2dEngine final : public Engine {
   Cd2dStepArgs<T> fwd_args(int it, int act_min, int dmode, int cbase, T *traces) const {
     Cd2dStepArgs<T> a = base_args(it);
     a.pold = P(it - 2); a.pcur = P(it - 1); a.pnew = P(it);
-    a.psix_o = psix_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spx_;
-    a.psix_n = psix_.as<T>() + (int64_t)(it & 1) * ns_ * spx_;
-    a.psiz_o = psiz_.as<T>() + (int64_t)((it - 1) & 1) * ns_ * spz_;
-    a.psiz_n = psiz_.as<T>() + (int64_t)(it & 1) * ns_ * spz_;
-    a.xix = xix_.as<T>(); a.xiz = xiz_.as<T>();
+    a.psix_o = PSX(it - 1); a.psix_n = PSX(it);
+    a.psiz_o = PSZ(it - 1); a.psiz_n = PSZ(it);
+    a.xix_o = XIX(it - 1); a.xix = XIX(it);
+    a.xiz_o = XIZ(it - 1); a.xiz = XIZ(it);
     a.act_min = act_min;
     a.dmode = c_.d.gradient ? dmode : 0;
     a.cbase = cbase;
@@ -543,6 +860,58 @@ class Cd2dEngine final : public Engine {
     launch(fwd_args(it, act_min, dmode, cbase, traces));
   }
 
+  // levels it and it+1 in one launch (cd2d_step2_kernel)
+  Cd2dStep2Args<T> step2_args(int it, int act_min, int dmode, int cbase, T *traces) const {
+    Cd2dStep2Args<T> b;
+    b.a = fwd_args(it, act_min, dmode, cbase, traces);
+    b.a.lean = lean_ ? 1 : 0;
+    b.a.nbx = (int)ceil_div(nx_, TX);
+    b.a.nby = (int)ceil_div(nz_, S2H);
+    b.pnew2 = P(it + 1);
+    b.psix_n2 = PSX(it + 1);
+    b.psiz_n2 = PSZ(it + 1);
+    b.xix_n2 = XIX(it + 1);
+    b.xiz_n2 = XIZ(it + 1);
+    return b;
+  }
+  void launch_step2(const Cd2dStep2Args<T> &b, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const unsigned nb = (unsigned)((int64_t)b.a.nbx * b.a.nby * nsh_);
+    hipExtLaunchKernelGGL(cd2d_step2_kernel<T>, dim3(nb), dim3(TX, TZ), 0, c_.stream, e0, e1, 0, b);
+  }
+  void step2_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
+    hipEvent_t e0, e1;
+    c_.ktimer.dispatch_events(0, &e0, &e1);
+    launch_step2(step2_args(it, act_min, dmode, cbase, traces), e0, e1);
+    SWX_HIP(hipGetLastError());
+  }
+
+  // One level per launch (two launches of the tuned row count) against two
+  // levels per launch, timed on levels 1 and 2 of the zero initial state,
+  // which every repetition recomputes identically.  Both give bitwise the
+  // same results; SWX_CD2D_STEPS pins the choice.
+  void tune_steps(bool save, T *tr) {
+    steps_tuned_ = true;
+    if (steps_env_ || ntmax_ < 2) return;
+    Cd2dStepArgs<T> a1 = fwd_args(1, 0, save ? 1 : 0, 0, tr), a2 = fwd_args(2, 0, save ? 1 : 0, 0, tr);
+    a1.lean = a2.lean = lean_ ? 1 : 0;
+    const Cd2dStep2Args<T> b = step2_args(1, 0, save ? 1 : 0, 0, tr);
+    auto one = [&](Cd2dStepArgs<T> x) {
+      if (rows_ == 4) launch_rows<4>(x);
+      else if (rows_ == 1) launch_rows<1>(x);
+      else launch_rows<2>(x);
+    };
+    const int k = fastest_candidate(c_.stream, 2, [&](int c) {
+      if (c == 0) {
+        one(a1);
+        one(a2);
+      } else {
+        launch_step2(b);
+      }
+      SWX_HIP(hipGetLastError());
+    });
+    steps_ = k == 0 ? 1 : 2;
+  }
+
   void step_adj(int it, int cbase) {
     Cd2dStepArgs<T> a = base_args(it);
     a.pold = A(it + 2); a.pcur = A(it + 1); a.pnew = A(it);
@@ -551,6 +920,7 @@ class Cd2dEngine final : public Engine {
     a.psiz_o = apsiz_.as<T>() + (int64_t)((it + 1) & 1) * ns_ * spz_;
     a.psiz_n = apsiz_.as<T>() + (int64_t)(it & 1) * ns_ * spz_;
     a.xix = axix_.as<T>(); a.xiz = axiz_.as<T>();
+    a.xix_o = a.xix; a.xiz_o = a.xiz;
     a.act_min = 0;
     a.dmode = 0;
     a.cbase = cbase;
@@ -570,10 +940,10 @@ class Cd2dEngine final : public Engine {
     Part parts[6] = {
         {P(cb - 1), fb},
         {P(cb), fb},
-        {psix_.as<T>() + (int64_t)(cb & 1) * ns_ * spx_, ns_ * spx_ * e},
-        {psiz_.as<T>() + (int64_t)(cb & 1) * ns_ * spz_, ns_ * spz_ * e},
-        {xix_.get(), ns_ * sxx_ * e},
-        {xiz_.get(), ns_ * sxz_ * e},
+        {PSX(cb), ns_ * spx_ * e},
+        {PSZ(cb), ns_ * spz_ * e},
+        {XIX(cb), ns_ * sxx_ * e},
+        {XIZ(cb), ns_ * sxz_ * e},
     };
     int64_t off = 0;
     for (auto &pt : parts) {
@@ -601,6 +971,8 @@ class Cd2dEngine final : public Engine {
   int cstride_;
   int rows_ = 2;
   bool tuned_ = false;   // rows_ chosen (by SWX_CD2D_ROWS or tune_rows)
+  int steps_ = 1;        // levels per forward / replay launch
+  bool steps_env_ = false, steps_tuned_ = false;
   bool lean_ = true;
   DevBuf fact_, p_, psix_, psiz_, xix_, xiz_, cpml_;
   DevBuf a_, apsix_, apsiz_, axix_, axiz_, grad_, gout_;
@@ -765,6 +1137,8 @@ void forward_noalloc(const swx_plan_desc &d, const T *fact, const swx_shot_desc 
   a.spx = spx; a.spz = spz; a.sxx = sxx; a.sxz = sxz;
   a.xix = reinterpret_cast<T *>(at(o_xix));
   a.xiz = reinterpret_cast<T *>(at(o_xiz));
+  a.xix_o = a.xix;
+  a.xiz_o = a.xiz;
   a.cpml = reinterpret_cast<const T *>(at(o_cpml)); a.cstride = cstride;
   a.nt_shot = pv.nt;
   a.S = 1;

@@ -57,6 +57,21 @@ def test_vd_shot_group_bytes():
                                                                                         (416, 416))
 
 
+def test_cd2d_two_levels_per_launch_bytes():
+    """Two CD levels per forward / replay launch read p^n, p^n-1 and fact
+    once per launch (3e) and write each level and its D: per pair of levels
+    5e (+2e D) instead of 8e (+2e); the adjoint is unchanged."""
+    nt, S = 501, 50
+    kl = (nt - 1) // S
+    last, rep = nt - kl * S, kl * S
+    one = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 1))
+    two = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 2))
+    nl = (nt + 1) // 2 + kl * (S // 2)
+    assert two[0] == (nl * 3 * E * N + (nt + rep) * E * N + (last + rep) * E * N + (nt + rep) * STRIP, nl)
+    assert two[1] == one[1]
+    assert one[0][1] == nt + rep and two[0][0] < one[0][0]
+
+
 def test_cd_forward_and_adjoint_bytes():
     nt, S = 501, 50
     b = bench.alg_bytes("cd2d", NS, 0, E, [nt], S)

@@ -291,3 +291,70 @@ def test_misfit_api_matches_gradient_misfit(oracle_mod):
     _, m2 = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
                           gradparams=sw.GradParameters(compute_misfit=True, check_freq=20))
     assert np.isclose(m1, m2, rtol=1e-12)
+
+
+def _edge_problem(dtype):
+    nx, nz, h, dt, nt, f0, halo = 90, 70, (10.0, 10.0), 1e-3, 241, 9.0, 8
+    rng = np.random.default_rng(23)
+    vtrue = 2000 + 300 * rng.random((nx, nz))
+    v0 = np.full((nx, nz), 2100.0)
+    tf = np.stack([syn.ricker_tf(nt, dt, f0, 1.2 / f0) * a for a in (1.0, 0.7, -0.5)], 1)
+    src = np.array([[0.0, 0.0], [890.0, 350.0], [400.0, 690.0]])
+    rec = np.array([[0.0, 0.0], [300.0, 0.0], [620.0, 690.0], [450.0, 300.0], [890.0, 350.0]])
+    params = sw.InputParametersAcoustic(nt, dtype(dt), (nx, nz), h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=True))
+    shots = [sw.ScalarShot(sw.ScalarSources(src, tf.astype(dtype), f0), sw.ScalarReceivers(rec, nt, dtype=dtype))]
+    return dict(params=params, vtrue=vtrue, v0=v0, shots=shots)
+
+
+@pytest.mark.parametrize("case", ["deep_points", "uneven_shots", "edge_points"])
+def test_two_levels_per_launch_agree_bitwise(oracle_mod, monkeypatch, case):
+    """Two forward / replay levels per launch (cd2d_step2_kernel, interior
+    and C-PML/edge/point tiles) give the one-level launches' seismograms,
+    snapshots and gradient bit for bit (FP32): with points on interior rows,
+    with a batch whose shots end on an odd and an even level, with sources
+    and receivers on the outermost cells and a free top; odd replay
+    intervals and snapshot spacing.  The FP64 two-level run matches the
+    oracle."""
+    print(f"[cd2d two levels per launch: {case}]", flush=True)
+    if case == "deep_points":
+        p = _deep_points_problem(np.float32)
+        obs = _observed(oracle_mod, p)
+    elif case == "uneven_shots":
+        p = _grad_problem(np.float32, nts=(300, 257, 281))
+        obs = _observed(oracle_mod, p)
+    else:
+        p = _edge_problem(np.float32)
+        sw.swforward(p["params"], sw.VpAcousticCDMaterialProperties(p["vtrue"].astype(np.float32)), p["shots"],
+                     runparams=sw.RunParameters())
+        obs = [sh.recs.seismograms.astype(np.float64) for sh in p["shots"]]
+    mis = [sw.L2Misfit(o.astype(np.float32)) for o in obs]
+    mp = sw.VpAcousticCDMaterialProperties(p["v0"].astype(np.float32))
+    outs = []
+    for steps, lean, cf in (("1", "1", 23), ("2", "1", 23), ("2", "0", 23), ("2", "1", 1), ("2", "1", 50)):
+        print(f"[cd2d levels per launch {steps}, interior path {lean}, check_freq {cf}]", flush=True)
+        monkeypatch.setenv("SWX_CD2D_STEPS", steps)
+        monkeypatch.setenv("SWX_CD2D_LEAN", lean)
+        g = sw.swgradient(p["params"], mp, p["shots"], mis, runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=cf))["vp"]
+        snaps = sw.swforward(p["params"], mp, p["shots"], runparams=sw.RunParameters(snapevery=7))
+        outs.append((g, [sh.recs.seismograms.copy() for sh in p["shots"]], snaps))
+    assert np.max(np.abs(outs[0][0])) > 0
+    for g, seis, snaps in outs[1:]:
+        assert np.array_equal(g, outs[0][0])
+        assert all(np.array_equal(a, b) for a, b in zip(seis, outs[0][1]))
+        for s1, s0 in zip(snaps, outs[0][2]):
+            assert sorted(s1) == sorted(s0)
+            for it in s0:
+                assert np.array_equal(s1[it]["pcur"], s0[it]["pcur"]), it
+    if case == "deep_points":
+        monkeypatch.setenv("SWX_CD2D_STEPS", "2")
+        monkeypatch.setenv("SWX_CD2D_LEAN", "1")
+        p = _deep_points_problem(np.float64)
+        gref, _, sref = _oracle_gradient(oracle_mod, p, obs, 23)
+        g = sw.swgradient(p["params"], sw.VpAcousticCDMaterialProperties(p["v0"]), p["shots"],
+                          [sw.L2Misfit(o) for o in obs], runparams=sw.RunParameters(),
+                          gradparams=sw.GradParameters(check_freq=23))["vp"]
+        assert rel_l2(g, gref) <= 1e-10
+        for sh, s in zip(p["shots"], sref):
+            assert rel_l2(sh.recs.seismograms, s) <= 1e-10
