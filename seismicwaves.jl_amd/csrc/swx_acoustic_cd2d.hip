@@ -472,36 +472,48 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
   const T *pc_ = a.pcur + so;
   const T *po_ = a.pold + so;
   if (a.lean && cd2d_step2_is_lean(a, s, I0, J0)) {
-    // level it on the region: every load first (the region's current
-    // level plus one cell around it is read from L1 / L2 lines)
-    T pc[S2Q], pl[S2Q], pr[S2Q], pd[S2Q], pu[S2Q], pov[S2Q], fa[S2Q];
+    // Every load first: level it-1 on the tile + 2 (staged in LDS), level
+    // it-2 and the factor on the region, the factor on the tile's cells.
+    constexpr int PW = TX + 4, PH = S2H + 4, PN = PW * PH;
+    constexpr int PQ = (PN + TX * TZ - 1) / (TX * TZ);
+    __shared__ T sP[PH][PW];      // level it-1: columns I0-2 .., rows J0-2 ..
+    T pp[PQ], pov[S2Q], fa[S2Q], faB[2];
+#pragma unroll
+    for (int k = 0; k < PQ; k++) {
+      const int q = min(tid + k * TX * TZ, PN - 1);
+      const int ly = q / PW, lx = q - ly * PW;
+      pp[k] = pc_[(J0 - 2 + ly) * nx + (I0 - 2 + lx)];
+    }
 #pragma unroll
     for (int k = 0; k < S2Q; k++) {
       const int q = min(tid + k * TX * TZ, S2N - 1);
       const int ly = q / S2W, lx = q - ly * S2W;
       const int c = (J0 - 1 + ly) * nx + (I0 - 1 + lx);
-      pc[k] = pc_[c];
-      pl[k] = pc_[c - 1];
-      pr[k] = pc_[c + 1];
-      pd[k] = pc_[c - nx];
-      pu[k] = pc_[c + nx];
       pov[k] = po_[c];
       fa[k] = a.fact[c];
     }
+#pragma unroll
+    for (int r = 0; r < 2; r++) faB[r] = a.fact[(J0 + threadIdx.y + r * TZ) * nx + I0 + threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < PQ; k++) {
+      const int q = tid + k * TX * TZ;
+      if (q < PN) sP[q / PW][q % PW] = pp[k];
+    }
     const int slotA = cd2d_dslot(a, nts, a.it), slotB = cd2d_dslot(a, nts, a.it + 1);
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < S2Q; k++) {
       const int q = tid + k * TX * TZ;
       if (q < S2N) {
         const int ly = q / S2W, lx = q - ly * S2W;
-        const T d2x = (pl[k] - T(2) * pc[k] + pr[k]) * a.idx2;
-        const T d2z = (pd[k] - T(2) * pc[k] + pu[k]) * a.idz2;
+        const T pc = sP[ly + 1][lx + 1];
+        const T d2x = (sP[ly + 1][lx] - T(2) * pc + sP[ly + 1][lx + 2]) * a.idx2;
+        const T d2z = (sP[ly][lx + 1] - T(2) * pc + sP[ly + 2][lx + 1]) * a.idz2;
         const T lxx = d2x, lzz = d2z;
         const T flap = fa[k] * (lxx + lzz);
-        const T pnv = T(2) * pc[k] - pov[k] + flap;
+        const T pnv = T(2) * pc - pov[k] + flap;
         const T inj = 0;
         sA[ly][lx] = pnv;
-        sC[ly][lx] = pc[k];
         if (lx >= 1 && lx <= TX && ly >= 1 && ly <= S2H) {
           const int64_t c = (int64_t)(J0 - 1 + ly) * nx + (I0 - 1 + lx);
           a.pnew[so + c] = pnv;
@@ -519,8 +531,8 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
       const T d2x = (sA[ly][lx - 1] - T(2) * p + sA[ly][lx + 1]) * a.idx2;
       const T d2z = (sA[ly - 1][lx] - T(2) * p + sA[ly + 1][lx]) * a.idz2;
       const T lxx = d2x, lzz = d2z;
-      const T flap = a.fact[c] * (lxx + lzz);
-      const T pnv = T(2) * p - sC[ly][lx] + flap;
+      const T flap = faB[r] * (lxx + lzz);
+      const T pnv = T(2) * p - sP[ly + 1][lx + 1] + flap;
       const T inj = 0;
       A2.pnew2[so + c] = pnv;
       if (slotB >= 0) a.dbuf[(int64_t)slotB * a.dslot + so + c] = flap + inj;
@@ -530,6 +542,12 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
   // General tile: level it on the region with cd2d_cell's branches (cells
   // outside the grid are skipped: only interior cells read their
   // neighbours, and those are inside the grid)
+  T faB[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int i = I0 + threadIdx.x, j = J0 + threadIdx.y + r * TZ;
+    faB[r] = (i < nx && j < nz) ? a.fact[(int64_t)j * nx + i] : T(0);
+  }
   const T *psxo_ = a.psix_o + (int64_t)s * a.spx, *pszo_ = a.psiz_o + (int64_t)s * a.spz;
   const T *xixo_ = a.xix_o + (int64_t)s * a.sxx, *xizo_ = a.xiz_o + (int64_t)s * a.sxz;
   for (int q = tid; q < S2N; q += TX * TZ) {
@@ -564,7 +582,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
     auto pcat = [&](int di, int dj) { return sA[ly + dj][lx + di]; };
     auto psxo = [&](int ii, int) { return sX[ly][ii - (I0 - 1)]; };
     auto pszo = [&](int jj, int) { return sZ[jj - (J0 - 1)][lx]; };
-    const Cd2dCell<T> o = cd2d_cell_eval<T>(a, s, nts, a.it + 1, i, j, pcat, sC[ly][lx], a.fact[c], psxo, pszo,
+    const Cd2dCell<T> o = cd2d_cell_eval<T>(a, s, nts, a.it + 1, i, j, pcat, sC[ly][lx], faB[r], psxo, pszo,
                                             [&]() { return sXI[ly][lx]; }, [&]() { return sZI[ly][lx]; });
     cd2d_cell_store(a, o, s, nts, a.it + 1, i, j, A2.pnew2, A2.psix_n2, A2.psiz_n2, A2.xix_n2, A2.xiz_n2);
   }
