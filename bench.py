@@ -360,7 +360,7 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
         else:
             # CD: r p^n, p^n-1, fact; w p^n+1 (4e); + w D (5e).  Adjoint with
             # the fused correlation: + r D, grad read-modify-write (7e).
-            if physics == "cd2d" and len(variant) > 1 and variant[1] == 2:
+            if physics == "cd2d" and len(variant) > 1 and variant[1] % 10 == 2:
                 # two levels per forward / replay launch: r p^n, p^n-1, fact
                 # once per launch (3e), w each level (1e) and its D (1e);
                 # launches start at levels 1, 3, 5, ... of the forward and
@@ -373,8 +373,27 @@ def alg_bytes(physics, n, halo, e, nts, S, variant=(0, 0)):
                 fwd_b += rep * 5 * e * N + rep * strip
                 fwd_n += nt + rep
             nseg = kl + 1
-            adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
-            adj_n += nt
+            if physics == "cd2d" and len(variant) > 1 and variant[1] // 10 == 2:
+                # two adjoint levels per launch: r a^{n+1}, a^{n+2}, fact once
+                # (3e), w both levels (2e), the gradient read and written once
+                # (2e) when either level correlates and one history level per
+                # correlating level (1e); the top level of every segment does
+                # not correlate (its correlation is in the next segment's
+                # separate kernel), a lone level takes a one-level launch (7e,
+                # or 4e at a segment top)
+                for k in range(nseg):
+                    L = min((k + 1) * S, nt) - k * S
+                    for top in range(L, 0, -2):
+                        if top >= 2:
+                            ncorr = 2 if top < L else 1
+                            adj_b += 5 * e * N + 2 * e * N + ncorr * e * N + 2 * strip
+                            adj_n += 1
+                        else:
+                            adj_b += (7 if top < L else 4) * e * N + strip
+                            adj_n += 1
+            else:
+                adj_b += (nt - nseg) * 7 * e * N + nseg * 4 * e * N + nt * strip
+                adj_n += nt
     if physics == "vd2d":
         # model factors once per group and step the group steps (its
         # longest shot's forward and replay steps; adjoint steps)
@@ -427,11 +446,12 @@ def kernel_names(physics, variant):
         else:
             k = f"cd3d_step_kernel<float, {v}, 0>"
         return f"{k} (forward)", f"{k} (adjoint)"
-    # [rows per thread, levels per forward / replay launch]
+    # [rows per thread, levels per launch: 10 x adjoint + forward / replay]
     k = f"cd2d_step_kernel<float, {variant[0] or 2}>"
-    if len(variant) > 1 and variant[1] == 2:
-        return "cd2d_step2_kernel<float> (forward)", f"{k} (adjoint)"
-    return f"{k} (forward)", f"{k} (adjoint)"
+    two = variant[1] if len(variant) > 1 else 0
+    fwd = "cd2d_step2_kernel<float, false>" if two % 10 == 2 else k
+    adj = "cd2d_step2_kernel<float, true>" if two // 10 == 2 else k
+    return f"{fwd} (forward)", f"{adj} (adjoint)"
 
 
 def traffic_from_profiles(workload, kernel):

@@ -15,8 +15,7 @@
 //   psi_x  [3][ns][2h*nz]  level k in buffer k mod 3, so a cell can
 //   psi_z  [3][ns][nx*2h]  recompute its left neighbour's new value
 //   xi_x   [3][ns][(2h+2)*nz], xi_z [3][ns][nx*(2h+2)]
-//   (the adjoint state keeps three levels of p, psi by parity and xi in
-//   place)
+//   (the adjoint state is laid out the same way)
 //   D      [S][ns][nx*nz]  p^{it-2} - 2 p^{it-1} + p^{it} of the segment
 //                          being replayed (what correlate_gradient! needs)
 //   ckpt   [K-2][...]      p^{c-1}, p^c, psi, xi at c = k*S
@@ -325,11 +324,12 @@ struct Cd2dCell {
   bool xup, zup;   // xi_x / xi_z updated
 };
 
-// cd2d_cell's update of cell (i, j) at level itl (forward: sources),
+// cd2d_cell's update of cell (i, j) at level itl (forward: sources;
+// ADJ: the prescaled residuals),
 // reading the current level through pc(di, dj), the old C-PML memory
 // variables through psxo(cell i, strip k) / pszo(cell j, strip k) and
 // xixo() / xizo() (the cell's own xi).
-template <typename T, typename PC, typename PSX, typename PSZ, typename XIX, typename XIZ>
+template <typename T, bool ADJ, typename PC, typename PSX, typename PSZ, typename XIX, typename XIZ>
 __device__ __forceinline__ Cd2dCell<T> cd2d_cell_eval(const Cd2dStepArgs<T> &a, int s, int nts, int itl, int i,
                                                       int j, PC pc_at, T pov, T fa, PSX psxo, PSZ pszo, XIX xixo,
                                                       XIZ xizo) {
@@ -388,13 +388,22 @@ __device__ __forceinline__ Cd2dCell<T> cd2d_cell_eval(const Cd2dStepArgs<T> &a, 
     pnv = T(2) * pc - pov + flap;
   }
   if (!interior) pnv = pov;
-  const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c);
-  if (u >= 0) {
-    const T *tf = a.srctf + a.pt.off_tf[s] + (itl - 1);
-    for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) {
-      const T v = tf[(int64_t)nts * a.pt.usrc_list[k]];
-      pnv += v;
-      inj += v;
+  if (!ADJ) {
+    const int u = find_point(a.pt.srow, a.pt.usrc_cell, a.pt.nrows, s, j, c);
+    if (u >= 0) {
+      const T *tf = a.srctf + a.pt.off_tf[s] + (itl - 1);
+      for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) {
+        const T v = tf[(int64_t)nts * a.pt.usrc_list[k]];
+        pnv += v;
+        inj += v;
+      }
+    }
+  } else {
+    const int u = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
+    if (u >= 0) {
+      const T *rs = a.res + a.pt.off_tr[s] + (itl - 1);
+      const T f = fa;
+      for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++) pnv += rs[(int64_t)nts * a.pt.urec_list[k]] * f;
     }
   }
   o.pnv = pnv;
@@ -426,9 +435,9 @@ __device__ __forceinline__ void cd2d_cell_store(const Cd2dStepArgs<T> &a, const 
   if (slot >= 0) a.dbuf[(int64_t)slot * a.dslot + so + c] = o.dval;
 }
 
-// Arguments of the second level of a two-level launch: the slots of
-// levels it+1 (pressure, memory variables) next to those of level it in
-// Cd2dStepArgs (pold = it-2, pcur = it-1, pnew = it).
+// Arguments of the second level of a two-level launch: the slots of the
+// second level (it+1 forward, it-1 adjoint: pressure, memory variables)
+// next to those of level it in Cd2dStepArgs.
 template <typename T>
 struct Cd2dStep2Args {
   Cd2dStepArgs<T> a;
@@ -445,38 +454,74 @@ __device__ __forceinline__ bool cd2d_step2_is_lean(const Cd2dStepArgs<T> &a, int
   const int i0 = I0 - 1, i1 = I0 + TX, j0 = J0 - 1, j1 = J0 + S2H;
   if (!(i0 >= max(h, 1) && i1 <= nx - h - 2 && j0 >= max(h, 1) && j1 <= nz - h - 2)) return false;
   const int64_t o = (int64_t)s * (a.pt.nrows + 1);
-  if (a.pt.srow[o + j0] != a.pt.srow[o + j1 + 1]) return false;
+  const int *rowp = a.res == nullptr ? a.pt.srow : a.pt.rrow;
+  if (rowp[o + j0] != rowp[o + j1 + 1]) return false;
   if (a.traces && a.pt.rrow[o + j0] != a.pt.rrow[o + j1 + 1]) return false;
   return true;
 }
 
-template <typename T>
+// Forward (ADJ = false): levels it and it+1 from it-1, it-2.  Adjoint
+// (ADJ = true): adjoint levels it and it-1 from it+1, it+2, with the
+// residual injection of each level and the correlations of both launches
+// the one-level kernel would make (level it correlates the adjoint level
+// it+1, level it-1 the adjoint level it, each with its replay history
+// slot), done by the owner of each cell in the same order.
+template <typename T, bool ADJ>
 __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2) {
   const Cd2dStepArgs<T> &a = A2.a;
-  __shared__ T sA[S2R][S2W];    // level it on the region
-  __shared__ T sC[S2R][S2W];    // level it-1 on the region
-  __shared__ T sX[S2R][S2W];    // psi_x of level it (strip cells)
-  __shared__ T sZ[S2R][S2W];    // psi_z of level it
-  __shared__ T sXI[S2R][S2W];   // xi_x of level it
-  __shared__ T sZI[S2R][S2W];   // xi_z of level it
+  __shared__ T sA[S2R][S2W];    // first level on the region
+  __shared__ T sC[S2R][S2W];    // its current level (it-1 forward, it+1 adjoint) on the region
+  __shared__ T sX[S2R][S2W];    // psi_x of the first level (strip cells)
+  __shared__ T sZ[S2R][S2W];    // psi_z of the first level
+  __shared__ T sXI[S2R][S2W];   // xi_x of the first level
+  __shared__ T sZI[S2R][S2W];   // xi_z of the first level
   const int b = xcd_tile2(blockIdx.x, gridDim.x);
   const int t = b / a.nbx;
   const int bx = b - t * a.nbx, by = t % a.nby, s = t / a.nby;
   const int nts = a.nt_shot[s];
-  if (a.it > nts || nts <= a.act_min) return;
-  const bool two = a.it + 1 <= nts;   // the shot also takes level it+1
+  if (nts <= a.act_min) return;
+  const int itA = a.it, itB = ADJ ? a.it - 1 : a.it + 1;
+  // a shot takes a level while it is within its nt (an adjoint shot whose
+  // last level is it-1 starts from zero fields at level it)
+  const bool actA = itA <= nts, actB = itB <= nts;
+  if (!actA && !actB) return;
   const int nx = a.nx, nz = a.nz, h = a.h;
   const int I0 = bx * TX, J0 = by * S2H;
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int64_t so = (int64_t)s * a.N;
   const T *pc_ = a.pcur + so;
   const T *po_ = a.pold + so;
-  if (a.lean && cd2d_step2_is_lean(a, s, I0, J0)) {
-    // Every load first: level it-1 on the tile + 2 (staged in LDS), level
-    // it-2 and the factor on the region, the factor on the tile's cells.
+  // correlations (adjoint): level itl correlates the adjoint level itl+1
+  // with history slot itl - cbase while itl+1 is within nt and the segment
+  const bool corrA = ADJ && a.corr && actA && itA + 1 <= nts && itA + 1 <= a.corr_lim;
+  const bool corrB = ADJ && a.corr && actB && itB + 1 <= nts && itB + 1 <= a.corr_lim;
+  T gv[2], dA[2], dB[2];
+  if (ADJ && (corrA || corrB)) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      const int i = min(I0 + (int)threadIdx.x, nx - 1), j = min(J0 + (int)threadIdx.y + r * TZ, nz - 1);
+      const int64_t c = (int64_t)j * nx + i;
+      gv[r] = a.grad[so + c];
+      dA[r] = corrA ? a.dbuf[(int64_t)(itA - a.cbase) * a.dslot + so + c] : T(0);
+      dB[r] = corrB ? a.dbuf[(int64_t)(itB - a.cbase) * a.dslot + so + c] : T(0);
+    }
+  }
+  // owner's correlation of tile cell r: the current level of each of the
+  // two one-level launches (pcA, then the first level pcB)
+  auto correlate = [&](int r, int64_t c, T pcA, T pcB) {
+    if (!(corrA || corrB)) return;
+    T g = gv[r];
+    if (corrA) g = g + pcA * dA[r];
+    if (corrB) g = g + pcB * dB[r];
+    a.grad[so + c] = g;
+  };
+  if (actA && a.lean && cd2d_step2_is_lean(a, s, I0, J0)) {
+    // Every load first: the current level on the tile + 2 (staged in LDS),
+    // the previous level and the factor on the region, the factor on the
+    // tile's cells.
     constexpr int PW = TX + 4, PH = S2H + 4, PN = PW * PH;
     constexpr int PQ = (PN + TX * TZ - 1) / (TX * TZ);
-    __shared__ T sP[PH][PW];      // level it-1: columns I0-2 .., rows J0-2 ..
+    __shared__ T sP[PH][PW];      // current level: columns I0-2 .., rows J0-2 ..
     T pp[PQ], pov[S2Q], fa[S2Q], faB[2];
 #pragma unroll
     for (int k = 0; k < PQ; k++) {
@@ -499,7 +544,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
       const int q = tid + k * TX * TZ;
       if (q < PN) sP[q / PW][q % PW] = pp[k];
     }
-    const int slotA = cd2d_dslot(a, nts, a.it), slotB = cd2d_dslot(a, nts, a.it + 1);
+    const int slotA = cd2d_dslot(a, nts, itA), slotB = cd2d_dslot(a, nts, itB);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < S2Q; k++) {
@@ -521,7 +566,7 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
         }
       }
     }
-    if (!two) return;
+    if (!actB) return;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -536,11 +581,12 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
       const T inj = 0;
       A2.pnew2[so + c] = pnv;
       if (slotB >= 0) a.dbuf[(int64_t)slotB * a.dslot + so + c] = flap + inj;
+      correlate(r, c, sP[ly + 1][lx + 1], p);
     }
     return;
   }
-  // General tile: level it on the region with cd2d_cell's branches (cells
-  // outside the grid are skipped: only interior cells read their
+  // General tile: the first level on the region with cd2d_cell's branches
+  // (cells outside the grid are skipped: only interior cells read their
   // neighbours, and those are inside the grid)
   T faB[2];
 #pragma unroll
@@ -554,12 +600,17 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
     const int ly = q / S2W, lx = q - ly * S2W;
     const int i = I0 - 1 + lx, j = J0 - 1 + ly;
     if (i < 0 || i >= nx || j < 0 || j >= nz) continue;
+    if (!actA) {
+      // an adjoint shot that starts at the second level: zero fields
+      sA[ly][lx] = sC[ly][lx] = sX[ly][lx] = sZ[ly][lx] = sXI[ly][lx] = sZI[ly][lx] = T(0);
+      continue;
+    }
     const int64_t c = (int64_t)j * nx + i;
     auto pcat = [&](int di, int dj) { return pc_[c + di + (int64_t)dj * nx]; };
     auto psxo = [&](int, int k) { return psxo_[(int64_t)j * (2 * h) + k]; };
     auto pszo = [&](int, int k) { return pszo_[(int64_t)k * nx + i]; };
-    const Cd2dCell<T> o = cd2d_cell_eval<T>(
-        a, s, nts, a.it, i, j, pcat, po_[c], a.fact[c], psxo, pszo,
+    const Cd2dCell<T> o = cd2d_cell_eval<T, ADJ>(
+        a, s, nts, itA, i, j, pcat, po_[c], a.fact[c], psxo, pszo,
         [&]() { return xixo_[(int64_t)j * (2 * h + 2) + (i < h ? i : i + 1 - nx + 2 * h)]; },
         [&]() { return xizo_[(int64_t)(j < h ? j : j + 1 - nz + 2 * h) * nx + i]; });
     sA[ly][lx] = o.pnv;
@@ -569,11 +620,11 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
     sXI[ly][lx] = o.xix;
     sZI[ly][lx] = o.xiz;
     if (lx >= 1 && lx <= TX && ly >= 1 && ly <= S2H)
-      cd2d_cell_store(a, o, s, nts, a.it, i, j, a.pnew, a.psix_n, a.psiz_n, a.xix, a.xiz);
+      cd2d_cell_store(a, o, s, nts, itA, i, j, a.pnew, a.psix_n, a.psiz_n, a.xix, a.xiz);
   }
-  if (!two) return;
+  if (!actB) return;
   __syncthreads();
-  // level it+1 on the tile, from the region in LDS
+  // the second level on the tile, from the region in LDS
   for (int r = 0; r < 2; r++) {
     const int lx = threadIdx.x + 1, ly = threadIdx.y + r * TZ + 1;
     const int i = I0 - 1 + lx, j = J0 - 1 + ly;
@@ -582,9 +633,10 @@ __global__ __launch_bounds__(TX *TZ) void cd2d_step2_kernel(Cd2dStep2Args<T> A2)
     auto pcat = [&](int di, int dj) { return sA[ly + dj][lx + di]; };
     auto psxo = [&](int ii, int) { return sX[ly][ii - (I0 - 1)]; };
     auto pszo = [&](int jj, int) { return sZ[jj - (J0 - 1)][lx]; };
-    const Cd2dCell<T> o = cd2d_cell_eval<T>(a, s, nts, a.it + 1, i, j, pcat, sC[ly][lx], faB[r], psxo, pszo,
-                                            [&]() { return sXI[ly][lx]; }, [&]() { return sZI[ly][lx]; });
-    cd2d_cell_store(a, o, s, nts, a.it + 1, i, j, A2.pnew2, A2.psix_n2, A2.psiz_n2, A2.xix_n2, A2.xiz_n2);
+    const Cd2dCell<T> o = cd2d_cell_eval<T, ADJ>(a, s, nts, itB, i, j, pcat, sC[ly][lx], faB[r], psxo, pszo,
+                                                 [&]() { return sXI[ly][lx]; }, [&]() { return sZI[ly][lx]; });
+    cd2d_cell_store(a, o, s, nts, itB, i, j, A2.pnew2, A2.psix_n2, A2.psiz_n2, A2.xix_n2, A2.xiz_n2);
+    correlate(r, c, sC[ly][lx], sA[ly][lx]);
   }
 }
 
@@ -645,13 +697,12 @@ class Cd2dEngine final : public Engine {
     cpml_.alloc((int64_t)ns_ * cstride_ * e);
     state_bytes_ = fact_.bytes() + p_.bytes() + psix_.bytes() + psiz_.bytes() + xix_.bytes() + xiz_.bytes();
     if (d.gradient) {
-      a_.alloc(3 * ns_ * N_ * e);
-      // adjoint state: three levels (slot k mod 3), C-PML memory variables
-      // double buffered by parity (psi) and updated in place (xi)
-      apsix_.alloc(std::max<int64_t>(2 * ns_ * spx_, 1) * e);
-      apsiz_.alloc(std::max<int64_t>(2 * ns_ * spz_, 1) * e);
-      axix_.alloc(ns_ * sxx_ * e);
-      axiz_.alloc(ns_ * sxz_ * e);
+      a_.alloc(4 * ns_ * N_ * e);
+      // adjoint state: the same level layout as the forward state
+      apsix_.alloc(psix_.bytes());
+      apsiz_.alloc(psiz_.bytes());
+      axix_.alloc(xix_.bytes());
+      axiz_.alloc(xiz_.bytes());
       grad_.alloc(ns_ * N_ * e);
       gout_.alloc(ns_ * N_ * e);
       state_bytes_ += a_.bytes() + apsix_.bytes() + apsiz_.bytes() + axix_.bytes() +
@@ -732,6 +783,7 @@ class Cd2dEngine final : public Engine {
   void adjoint(bool per_shot) override {
     (void)per_shot;
     const int64_t e = sizeof(T);
+    if (!adj_steps_tuned_) tune_adj_steps();
     SWX_HIP(hipMemsetAsync(a_.get(), 0, a_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(apsix_.get(), 0, apsix_.bytes(), c_.stream));
     SWX_HIP(hipMemsetAsync(apsiz_.get(), 0, apsiz_.bytes(), c_.stream));
@@ -755,12 +807,15 @@ class Cd2dEngine final : public Engine {
         }
       }
       const int top = std::min(cb + S_, ntmax_);
-      for (int it = top; it >= cb + 1; it--) {
-        step_adj(it, cb);
+      for (int it = top; it >= cb + 1;) {
+        const int n = adj_steps_ == 2 && it - 1 >= cb + 1 ? 2 : 1;
+        if (n == 2) step2_adj(it, cb);
+        else step_adj(it, cb);
+        it -= n;
       }
       // correlation of the first step of the segment
       dim3 g((unsigned)std::min<int64_t>(ceil_div(N_, 256), 1024), nsh_);
-      cd2d_correlate_kernel<T><<<g, 256, 0, c_.stream>>>(grad_.as<T>(), a_.as<T>() + mod3(cb + 1) * ns_ * N_,
+      cd2d_correlate_kernel<T><<<g, 256, 0, c_.stream>>>(grad_.as<T>(), A(cb + 1),
                                                           dbuf_.as<T>(), N_, c_.pts.d_nt.as<int>(), cb);
       SWX_HIP(hipGetLastError());
     }
@@ -780,8 +835,9 @@ class Cd2dEngine final : public Engine {
   int64_t bytes_checkpoints() const override { return ck_.bytes(); }
   int64_t bytes_replay() const override { return dbuf_.bytes(); }
   int segment() const override { return S_; }
-  // rows per thread of the one-level kernel; levels per forward launch
-  void variants(int32_t v[2]) const override { v[0] = rows_; v[1] = steps_; }
+  // rows per thread of the one-level kernel; levels per launch, 10 x
+  // adjoint + forward / replay
+  void variants(int32_t v[2]) const override { v[0] = rows_; v[1] = 10 * adj_steps_ + steps_; }
 
  private:
   T *P(int k) const { return p_.as<T>() + (int64_t)(((k % 4) + 4) % 4) * ns_ * N_; }
@@ -790,7 +846,11 @@ class Cd2dEngine final : public Engine {
   T *PSZ(int k) const { return psiz_.as<T>() + (int64_t)mod3(k) * ns_ * spz_; }
   T *XIX(int k) const { return xix_.as<T>() + (int64_t)mod3(k) * ns_ * sxx_; }
   T *XIZ(int k) const { return xiz_.as<T>() + (int64_t)mod3(k) * ns_ * sxz_; }
-  T *A(int k) const { return a_.as<T>() + (int64_t)mod3(k) * ns_ * N_; }
+  T *A(int k) const { return a_.as<T>() + (int64_t)(((k % 4) + 4) % 4) * ns_ * N_; }
+  T *APSX(int k) const { return apsix_.as<T>() + (int64_t)mod3(k) * ns_ * spx_; }
+  T *APSZ(int k) const { return apsiz_.as<T>() + (int64_t)mod3(k) * ns_ * spz_; }
+  T *AXIX(int k) const { return axix_.as<T>() + (int64_t)mod3(k) * ns_ * sxx_; }
+  T *AXIZ(int k) const { return axiz_.as<T>() + (int64_t)mod3(k) * ns_ * sxz_; }
 
   void zero_forward_state() {
     SWX_HIP(hipMemsetAsync(p_.get(), 0, p_.bytes(), c_.stream));
@@ -892,14 +952,15 @@ class Cd2dEngine final : public Engine {
     b.xiz_n2 = XIZ(it + 1);
     return b;
   }
+  template <bool ADJ>
   void launch_step2(const Cd2dStep2Args<T> &b, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const unsigned nb = (unsigned)((int64_t)b.a.nbx * b.a.nby * nsh_);
-    hipExtLaunchKernelGGL(cd2d_step2_kernel<T>, dim3(nb), dim3(TX, TZ), 0, c_.stream, e0, e1, 0, b);
+    hipExtLaunchKernelGGL(cd2d_step2_kernel<T, ADJ>, dim3(nb), dim3(TX, TZ), 0, c_.stream, e0, e1, 0, b);
   }
   void step2_fwd(int it, int act_min, int dmode, int cbase, T *traces) {
     hipEvent_t e0, e1;
     c_.ktimer.dispatch_events(0, &e0, &e1);
-    launch_step2(step2_args(it, act_min, dmode, cbase, traces), e0, e1);
+    launch_step2<false>(step2_args(it, act_min, dmode, cbase, traces), e0, e1);
     SWX_HIP(hipGetLastError());
   }
 
@@ -923,22 +984,20 @@ class Cd2dEngine final : public Engine {
         one(a1);
         one(a2);
       } else {
-        launch_step2(b);
+        launch_step2<false>(b);
       }
       SWX_HIP(hipGetLastError());
     });
     steps_ = k == 0 ? 1 : 2;
   }
 
-  void step_adj(int it, int cbase) {
+  Cd2dStepArgs<T> adj_args(int it, int cbase) const {
     Cd2dStepArgs<T> a = base_args(it);
     a.pold = A(it + 2); a.pcur = A(it + 1); a.pnew = A(it);
-    a.psix_o = apsix_.as<T>() + (int64_t)((it + 1) & 1) * ns_ * spx_;
-    a.psix_n = apsix_.as<T>() + (int64_t)(it & 1) * ns_ * spx_;
-    a.psiz_o = apsiz_.as<T>() + (int64_t)((it + 1) & 1) * ns_ * spz_;
-    a.psiz_n = apsiz_.as<T>() + (int64_t)(it & 1) * ns_ * spz_;
-    a.xix = axix_.as<T>(); a.xiz = axiz_.as<T>();
-    a.xix_o = a.xix; a.xiz_o = a.xiz;
+    a.psix_o = APSX(it + 1); a.psix_n = APSX(it);
+    a.psiz_o = APSZ(it + 1); a.psiz_n = APSZ(it);
+    a.xix_o = AXIX(it + 1); a.xix = AXIX(it);
+    a.xiz_o = AXIZ(it + 1); a.xiz = AXIZ(it);
     a.act_min = 0;
     a.dmode = 0;
     a.cbase = cbase;
@@ -946,7 +1005,57 @@ class Cd2dEngine final : public Engine {
     a.corr = 1;
     a.corr_lim = cbase + S_;
     a.res = c_.pts.d_res.as<T>();
-    launch(a);
+    return a;
+  }
+  void step_adj(int it, int cbase) { launch(adj_args(it, cbase)); }
+
+  // adjoint levels it and it-1 in one launch
+  Cd2dStep2Args<T> adj2_args(int it, int cbase) const {
+    Cd2dStep2Args<T> b;
+    b.a = adj_args(it, cbase);
+    b.a.lean = lean_ ? 1 : 0;
+    b.a.nbx = (int)ceil_div(nx_, TX);
+    b.a.nby = (int)ceil_div(nz_, S2H);
+    b.pnew2 = A(it - 1);
+    b.psix_n2 = APSX(it - 1);
+    b.psiz_n2 = APSZ(it - 1);
+    b.xix_n2 = AXIX(it - 1);
+    b.xiz_n2 = AXIZ(it - 1);
+    return b;
+  }
+  void step2_adj(int it, int cbase) {
+    hipEvent_t e0, e1;
+    c_.ktimer.dispatch_events(1, &e0, &e1);
+    launch_step2<true>(adj2_args(it, cbase), e0, e1);
+    SWX_HIP(hipGetLastError());
+  }
+
+  // Adjoint levels per launch, timed like tune_steps on the top two levels
+  // of the first segment before the adjoint state and the gradient are
+  // cleared (what the timed launches write is discarded).
+  void tune_adj_steps() {
+    adj_steps_tuned_ = true;
+    adj_steps_ = steps_;
+    if (steps_env_ || ntmax_ < 2 || S_ < 2) return;
+    const int it = std::min(S_, ntmax_);
+    Cd2dStepArgs<T> a1 = adj_args(it, 0), a2 = adj_args(it - 1, 0);
+    a1.lean = a2.lean = lean_ ? 1 : 0;
+    const Cd2dStep2Args<T> b = adj2_args(it, 0);
+    auto one = [&](Cd2dStepArgs<T> x) {
+      if (rows_ == 4) launch_rows<4>(x);
+      else if (rows_ == 1) launch_rows<1>(x);
+      else launch_rows<2>(x);
+    };
+    const int k = fastest_candidate(c_.stream, 2, [&](int c) {
+      if (c == 0) {
+        one(a1);
+        one(a2);
+      } else {
+        launch_step2<true>(b);
+      }
+      SWX_HIP(hipGetLastError());
+    });
+    adj_steps_ = k == 0 ? 1 : 2;
   }
 
   // checkpoint layout (all shots): p^{c-1} | p^c | psi_x | psi_z | xi_x | xi_z
@@ -990,7 +1099,8 @@ class Cd2dEngine final : public Engine {
   int rows_ = 2;
   bool tuned_ = false;   // rows_ chosen (by SWX_CD2D_ROWS or tune_rows)
   int steps_ = 1;        // levels per forward / replay launch
-  bool steps_env_ = false, steps_tuned_ = false;
+  int adj_steps_ = 1;    // levels per adjoint launch
+  bool steps_env_ = false, steps_tuned_ = false, adj_steps_tuned_ = false;
   bool lean_ = true;
   DevBuf fact_, p_, psix_, psiz_, xix_, xiz_, cpml_;
   DevBuf a_, apsix_, apsiz_, axix_, axiz_, grad_, gout_;

@@ -64,12 +64,34 @@ def test_cd2d_two_levels_per_launch_bytes():
     nt, S = 501, 50
     kl = (nt - 1) // S
     last, rep = nt - kl * S, kl * S
-    one = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 1))
-    two = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 2))
+    one = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 11))
+    two = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 12))
     nl = (nt + 1) // 2 + kl * (S // 2)
     assert two[0] == (nl * 3 * E * N + (nt + rep) * E * N + (last + rep) * E * N + (nt + rep) * STRIP, nl)
     assert two[1] == one[1]
     assert one[0][1] == nt + rep and two[0][0] < one[0][0]
+
+
+def test_cd2d_two_adjoint_levels_per_launch_bytes():
+    """Two CD adjoint levels per launch: per pair 3e read + 2e written, the
+    gradient read and written once (2e) and one history level per
+    correlating level; the top level of each segment does not correlate and
+    an odd segment ends with a one-level launch (7e)."""
+    S = 5
+    for nt in (10, 11, 13):
+        one = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 11))
+        two = bench.alg_bytes("cd2d", NS, 0, E, [nt], S, (2, 21))
+        want_b = want_n = 0
+        for k in range((nt - 1) // S + 1):
+            L = min((k + 1) * S, nt) - k * S
+            pairs, lone = L // 2, L % 2
+            want_n += pairs + lone
+            # the top pair correlates one level, the others two
+            want_b += pairs * 7 * E * N + max(2 * pairs - 1, 0) * E * N + pairs * 2 * STRIP
+            want_b += lone * ((7 if L > 1 else 4) * E * N + STRIP)
+        assert two[1] == (want_b, want_n), nt
+        assert two[0] == one[0]
+        assert two[1][0] < one[1][0]
 
 
 def test_cd_forward_and_adjoint_bytes():
