@@ -358,3 +358,36 @@ def test_two_levels_per_launch_agree_bitwise(oracle_mod, monkeypatch, case):
         assert rel_l2(g, gref) <= 1e-10
         for sh, s in zip(p["shots"], sref):
             assert rel_l2(sh.recs.seismograms, s) <= 1e-10
+
+
+@pytest.mark.parametrize("n,halo,freetop", [((37, 7), 2, True), ((65, 9), 0, False), ((5, 4), 0, True),
+                                            ((130, 17), 4, False)])
+def test_two_levels_per_launch_small_grids(oracle_mod, monkeypatch, n, halo, freetop):
+    """Grids narrower or lower than one tile (partial tiles, rings outside
+    the grid, every cell an edge or C-PML cell), FP64: one and two levels
+    per launch agree bit for bit and match the oracle."""
+    print(f"[cd2d two levels, small grid {n} halo {halo} freetop {freetop}]", flush=True)
+    h, dt, nt, f0 = (10.0, 10.0), 1e-3, 61, 9.0
+    rng = np.random.default_rng(sum(n) + halo)
+    vtrue = 2000 + 300 * rng.random(n)
+    v0 = np.full(n, 2100.0)
+    tf = syn.ricker_tf(nt, dt, f0, 1.2 / f0)[:, None]
+    src = np.array([[(n[0] // 2) * h[0], (n[1] // 2) * h[1]]])
+    rec = np.array([[0.0, 0.0], [(n[0] - 1) * h[0], (n[1] - 1) * h[1]], [(n[0] // 3) * h[0], (n[1] // 2) * h[1]]])
+    shot = dict(src_pos=src, srctf=tf, f0=f0, rec_pos=rec)
+    obs = oracle_mod.acoustic_cd_forward(vtrue, h, dt, nt, halo, 1e-4, freetop, [shot])[0]
+    gref, _, sref = oracle_mod.acoustic_cd_gradient(v0, h, dt, nt, halo, 1e-4, freetop, [dict(shot, observed=obs)],
+                                                    check_freq=3)
+    params = sw.InputParametersAcoustic(nt, dt, n, h,
+                                        sw.CPMLBoundaryConditionParameters(halo=halo, rcoef=1e-4, freeboundtop=freetop))
+    outs = []
+    for steps in ("1", "2"):
+        monkeypatch.setenv("SWX_CD2D_STEPS", steps)
+        shots = [sw.ScalarShot(sw.ScalarSources(src, tf, f0), sw.ScalarReceivers(rec, nt))]
+        g = sw.swgradient(params, sw.VpAcousticCDMaterialProperties(v0), shots, [sw.L2Misfit(obs)],
+                          runparams=sw.RunParameters(erroronPPW=False), gradparams=sw.GradParameters(check_freq=3))
+        outs.append((g["vp"], shots[0].recs.seismograms.copy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert np.max(np.abs(gref["vp"])) > 0
+    assert rel_l2(outs[1][0], gref["vp"]) <= 1e-10
+    assert rel_l2(outs[1][1], sref[0]) <= 1e-10

@@ -91,11 +91,15 @@ def _forward(params, mp, shots):
     return [sh.recs.seismograms.copy() for sh in shots]
 
 
+@pytest.mark.parametrize("steps", ["1", "2"])
 @pytest.mark.parametrize("seed", range(8))
-def test_cd2d_random(oracle_mod, seed):
+def test_cd2d_random(oracle_mod, monkeypatch, seed, steps):
+    """With one and with two levels per launch (SWX_CD2D_STEPS)."""
+    monkeypatch.setenv("SWX_CD2D_STEPS", steps)
     rng = np.random.default_rng(100 + seed)
     n, halo, freetop, nts, cf = _config(rng, (130, 40), (220, 140), (0, 4, 9, 13))
-    print(f"[random cd2d seed {seed}: n {n} halo {halo} freetop {freetop} nts {nts} check_freq {cf}]", flush=True)
+    print(f"[random cd2d seed {seed}, {steps} level(s) per launch: n {n} halo {halo} freetop {freetop} nts {nts} "
+          f"check_freq {cf}]", flush=True)
     vt = _field(rng, n, 1800.0, 2600.0)
     v0 = syn.gaussian_smooth(vt, 4.0)
     shots, osh = _scalar_shots(rng, n, nts, np.float64)
