@@ -685,14 +685,16 @@ def hbm_triad(bytes_each=4 << 30, reps=20):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "seismicwaves.jl_amd", "probe",
                         "libswx_probe.so")
     lib = ctypes.CDLL(path)
-    lib.swx_probe_hbm_triad.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.swx_probe_hbm_triad_mode.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_double)]
     out = {}
-    for name, blocks in (("8_blocks_per_cu", 0), ("1_block_per_4KiB", bytes_each // 4096)):
-        r = (ctypes.c_double * 2)()
-        rc = lib.swx_probe_hbm_triad(bytes_each, reps, int(blocks), r)
-        if rc != 0:
-            raise RuntimeError(f"swx_probe_hbm_triad failed ({rc})")
-        out[name] = {"best_GBs": round(r[0], 1), "mean_GBs": round(r[1], 1)}
+    for mode, tag in ((0, ""), (1, "_nt_stores"), (3, "_nt_loads_stores")):
+        for name, blocks in (("8_blocks_per_cu", 0), ("1_block_per_4KiB", bytes_each // 4096)):
+            r = (ctypes.c_double * 2)()
+            rc = lib.swx_probe_hbm_triad_mode(bytes_each, reps, int(blocks), mode, r)
+            if rc != 0:
+                raise RuntimeError(f"swx_probe_hbm_triad_mode failed ({rc})")
+            out[name + tag] = {"best_GBs": round(r[0], 1), "mean_GBs": round(r[1], 1)}
     return {"probe": "hbm_triad", "bytes_per_array": bytes_each, "reps": reps, "peak_GBs": 8000.0, **out}
 
 

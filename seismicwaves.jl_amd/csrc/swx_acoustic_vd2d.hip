@@ -265,11 +265,11 @@ __device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, 
     const T pv = lean_p(tx + 1, lz, rp[r], rm0[r]);
     s_p[lz][tx + 1] = pv;
     const int c = (J0 + lz - 1) * nx + I0 + tx;
-    pn[c] = pv;
+    st_nt(pn + c, pv);
     if (sb >= 0) {
       T *hs = a.pbuf + so;
-      hs[(int64_t)(a.it - sb) * a.pslot + c] = pv;
-      if (a.smode == 1 && a.it == sb + 1) hs[c] = rp[r];
+      st_nt(hs + (int64_t)(a.it - sb) * a.pslot + c, pv);
+      if (a.smode == 1 && a.it == sb + 1) st_nt(hs + c, rp[r]);
     }
   }
 #pragma unroll
@@ -283,9 +283,11 @@ __device__ __forceinline__ void vd2d_fwd_lean_tile(const Vd2dArgs<T> &a, int s, 
     const int lz = ty + BY * r;
     const int c = (J0 + lz) * nx + I0 + tx;
     const T dx = d4(s_p[lz + 1][tx], s_p[lz + 1][tx + 1], s_p[lz + 1][tx + 2], s_p[lz + 1][tx + 3]) * a.idx;
-    vxn[c] = s_vx[lz + 1][tx + 3] - rm1x[r] * dx;
+    const T vx = s_vx[lz + 1][tx + 3] - rm1x[r] * dx;
+    st_nt(vxn + c, vx);
     const T dz = d4(s_p[lz][tx + 1], s_p[lz + 1][tx + 1], s_p[lz + 2][tx + 1], s_p[lz + 3][tx + 1]) * a.idz;
-    vzn[c] = s_vz[lz + 3][tx + 1] - rm1z[r] * dz;
+    const T vz = s_vz[lz + 3][tx + 1] - rm1z[r] * dz;
+    st_nt(vzn + c, vz);
   }
 }
 
@@ -441,10 +443,10 @@ __device__ __forceinline__ void vd2d_fwd_general_tile(const Vd2dArgs<T> &a, int 
       for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pv += tf[(int64_t)nts * a.pt.usrc_list[k]];
     }
     if (owned) {
-      a.p_n[so + c] = pv;
+      st_nt(a.p_n + so + c, pv);
       if (sb >= 0) {
-        a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
-        if (a.smode == 1 && a.it == sb + 1) a.pbuf[so + c] = pold;
+        st_nt(a.pbuf + (int64_t)(a.it - sb) * a.pslot + so + c, pv);
+        if (a.smode == 1 && a.it == sb + 1) st_nt(a.pbuf + so + c, pold);
       }
       if (any_rec) {
         // record_receivers! (acoustic2D_VD_xPU.jl:9-15)
@@ -483,7 +485,7 @@ __device__ __forceinline__ void vd2d_fwd_general_tile(const Vd2dArgs<T> &a, int 
         a.psx_n[q] = ps;
         d = d + ps;
       }
-      a.vx_n[so + c] = s_vx[lz + 1][tx + 3] - rm1x[r] * d;
+      st_nt(a.vx_n + so + c, s_vx[lz + 1][tx + 3] - rm1x[r] * d);
     }
     if (gj <= nz - 2) {
       T d = d4(s_p[lz][tx + 1], s_p[lz + 1][tx + 1], s_p[lz + 2][tx + 1], s_p[lz + 3][tx + 1]) * a.idz;
@@ -494,7 +496,7 @@ __device__ __forceinline__ void vd2d_fwd_general_tile(const Vd2dArgs<T> &a, int 
         a.psz_n[q] = ps;
         d = d + ps;
       }
-      a.vz_n[so + c] = s_vz[lz + 3][tx + 1] - rm1z[r] * d;
+      st_nt(a.vz_n + so + c, s_vz[lz + 3][tx + 1] - rm1z[r] * d);
     }
   }
 }
@@ -957,10 +959,10 @@ __global__ __launch_bounds__(64 * W) void vd2d_fwd_march(Vd2dArgs<T> a) {
       for (int k = a.pt.usrc_ptr[u]; k < a.pt.usrc_ptr[u + 1]; k++) pv += tf[(int64_t)nts * a.pt.usrc_list[k]];
     }
     if (owned) {
-      a.p_n[so + c] = pv;
+      st_nt(a.p_n + so + c, pv);
       if (sb >= 0) {
-        a.pbuf[(int64_t)(a.it - sb) * a.pslot + so + c] = pv;
-        if (a.smode == 1 && a.it == sb + 1) a.pbuf[so + c] = pold;
+        st_nt(a.pbuf + (int64_t)(a.it - sb) * a.pslot + so + c, pv);
+        if (a.smode == 1 && a.it == sb + 1) st_nt(a.pbuf + so + c, pold);
       }
       if (any_rec) {
         const int r = find_point(a.pt.rrow, a.pt.urec_cell, a.pt.nrows, s, j, c);
@@ -1177,8 +1179,8 @@ __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, 
     const int c = (J0 + lz) * nx + I0 + tx;
     const T vx = new_vx(tx + 2, lz, rvx[r], rm1x[r]);
     const T vz = new_vz(tx, lz + 2, rvz[r], rm1z[r]);
-    vxn[c] = vx;
-    vzn[c] = vz;
+    st_nt(vxn + c, vx);
+    st_nt(vzn + c, vz);
     s_vx[lz][tx + 2] = vx;
     s_vz[lz + 2][tx] = vz;
   }
@@ -1206,13 +1208,16 @@ __device__ __forceinline__ void vd2d_adj_lean_tile(const Vd2dArgs<T> &a, int s, 
     const T dvx = d4(s_vx[lz][tx], s_vx[lz][tx + 1], s_vx[lz][tx + 2], s_vx[lz][tx + 3]) * a.idx;
     const T dvz = d4(s_vz[lz][tx], s_vz[lz + 1][tx], s_vz[lz + 2][tx], s_vz[lz + 3][tx]) * a.idz;
     pv = pv - rm0[r] * (dvx + dvz);
-    pn[c] = pv;
+    st_nt(pn + c, pv);
     const T pcv = s_pc[lz + 1][tx + 1];
-    a.g0[so + c] = rg0[r] - (pv * (pcv - rpo[r]) * a.idt);
+    const T g0 = rg0[r] - (pv * (pcv - rpo[r]) * a.idt);
+    st_nt(a.g0 + so + c, g0);
     const T dpx = d4(s_pc[lz + 1][tx], pcv, s_pc[lz + 1][tx + 2], s_pc[lz + 1][tx + 3]) * a.idx;
-    a.g1x[so + c] = rg1x[r] + s_vx[lz][tx + 2] * dpx;
+    const T g1x = rg1x[r] + s_vx[lz][tx + 2] * dpx;
+    st_nt(a.g1x + so + c, g1x);
     const T dpz = d4(s_pc[lz][tx + 1], pcv, s_pc[lz + 2][tx + 1], s_pc[lz + 3][tx + 1]) * a.idz;
-    a.g1z[so + c] = rg1z[r] + s_vz[lz + 2][tx] * dpz;
+    const T g1z = rg1z[r] + s_vz[lz + 2][tx] * dpz;
+    st_nt(a.g1z + so + c, g1z);
   }
 }
 
@@ -1357,7 +1362,7 @@ __device__ __forceinline__ void vd2d_adj_general_tile(const Vd2dArgs<T> &a, int 
       d = d + ps;
     }
     const T v = vold - m1 * d;
-    if (owned) a.vx_n[so + (int64_t)gj2 * nx + gi2] = v;
+    if (owned) st_nt(a.vx_n + so + (int64_t)gj2 * nx + gi2, v);
     return v;
   };
   auto new_vz = [&](int lx, int lz, T vold, T m1, bool owned) -> T {
@@ -1372,7 +1377,7 @@ __device__ __forceinline__ void vd2d_adj_general_tile(const Vd2dArgs<T> &a, int 
       d = d + ps;
     }
     const T v = vold - m1 * d;
-    if (owned) a.vz_n[so + (int64_t)gj2 * nx + gi2] = v;
+    if (owned) st_nt(a.vz_n + so + (int64_t)gj2 * nx + gi2, v);
     return v;
   };
 #pragma unroll
@@ -1430,18 +1435,18 @@ __device__ __forceinline__ void vd2d_adj_general_tile(const Vd2dArgs<T> &a, int 
       for (int k = a.pt.urec_ptr[u]; k < a.pt.urec_ptr[u + 1]; k++)
         pv += rs[(int64_t)nts * a.pt.urec_list[k]] * rm0[r];
     }
-    a.p_n[so + c] = pv;
+    st_nt(a.p_n + so + c, pv);
     // correlate_gradient_m0! (correlate_gradient_xPU.jl:12-21)
     const T pcv = s_pc[lz + 1][tx + 1];
-    a.g0[so + c] = rg0[r] - (pv * (pcv - rpo[r]) * a.idt);
+    st_nt(a.g0 + so + c, rg0[r] - (pv * (pcv - rpo[r]) * a.idt));
     // correlate_gradient_m1! (acoustic2D_VD_xPU.jl:180-198), 4th order, no C-PML
     if (gi <= nx - 2) {
       const T dpx = d4(s_pc[lz + 1][tx], pcv, s_pc[lz + 1][tx + 2], s_pc[lz + 1][tx + 3]) * a.idx;
-      a.g1x[so + c] = rg1x[r] + s_vx[lz][tx + 2] * dpx;
+      st_nt(a.g1x + so + c, rg1x[r] + s_vx[lz][tx + 2] * dpx);
     }
     if (gj <= nz - 2) {
       const T dpz = d4(s_pc[lz][tx + 1], pcv, s_pc[lz + 2][tx + 1], s_pc[lz + 3][tx + 1]) * a.idz;
-      a.g1z[so + c] = rg1z[r] + s_vz[lz + 2][tx] * dpz;
+      st_nt(a.g1z + so + c, rg1z[r] + s_vz[lz + 2][tx] * dpz);
     }
   }
 }

@@ -9,6 +9,14 @@
 
 namespace swx {
 
+// Store of a value the next launch reads back from HBM (a new time level,
+// replay history, a gradient field): non-temporal, so the stream of new
+// levels does not evict the halo lines neighbouring tiles reuse from L2.
+template <typename T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 // Index of the unique point group at linear cell `cell` (grid row `row`) of
 // shot s, or -1.  rowptr/cells are a PointTables row index (sorted cells).
 __device__ __forceinline__ int find_point(const int *rowptr, const int64_t *cells, int64_t nrows, int s,
