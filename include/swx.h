@@ -156,9 +156,12 @@ typedef struct {
   int64_t n_kernel[2];
   /* Launch shapes the plan's autotuner chose (or the environment pinned):
    * VD [forward, adjoint] tile codes (8/16/32 interior + other launches,
-   * 308/316/332 one launch, 102/104 z-marching strips); elastic [forward,
-   * adjoint] variant codes; CD 2D [rows per thread, 0]; CD 3D [launch shape
-   * 8/4/2/108/104/102, planes per z chunk]; 0 where there is one shape. */
+   * 308/316/332 one launch, 102/104 z-marching strips; + 1000 k: shots in
+   * groups of 2^k in the block order); elastic [forward, adjoint] variant
+   * codes (0/1 stress and displacement launches, 2 one fused launch); CD 2D
+   * [rows per thread, 10 x adjoint + forward levels per launch]; CD 3D
+   * [launch shape 8/4/2/108/104/102/208/204/202, planes per z chunk]; 0
+   * where there is one shape. */
   int32_t kernel_variant[2];
 } swx_plan_info;
 
