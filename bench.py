@@ -698,6 +698,29 @@ def hbm_triad(bytes_each=4 << 30, reps=20):
     return {"probe": "hbm_triad", "bytes_per_array": bytes_each, "reps": reps, "peak_GBs": 8000.0, **out}
 
 
+def launch_gap(launches=2000, reps=5):
+    """What a HIP graph of a time loop saves: a chain of `launches` dependent
+    three-point passes over n floats (seismicwaves.jl_amd/probe/hbm_triad.hip,
+    swx_probe_launch_chain), stream-launched and replayed from a captured
+    graph; us per pass (best of `reps` chains) and of one pass alone."""
+    import ctypes
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "seismicwaves.jl_amd", "probe",
+                        "libswx_probe.so")
+    lib = ctypes.CDLL(path)
+    lib.swx_probe_launch_chain.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double)]
+    rows = []
+    for n in (1 << 12, 1 << 18, 1 << 20, 1 << 22, 1 << 24):
+        r = (ctypes.c_double * 4)()
+        rc = lib.swx_probe_launch_chain(n, launches, reps, r)
+        if rc != 0:
+            raise RuntimeError(f"swx_probe_launch_chain failed ({rc})")
+        rows.append({"floats": n, "stream_us_per_pass": round(r[0], 2), "graph_us_per_pass": round(r[1], 2),
+                     "one_pass_us": round(r[2], 2), "capture_instantiate_ms": round(r[3], 1)})
+        print(json.dumps(rows[-1]), flush=True)
+    return {"probe": "launch_chain", "launches": launches, "reps": reps, "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -720,7 +743,13 @@ def main():
                     help="measure without first running one step of the workload in a child process")
     ap.add_argument("--hbm-triad", action="store_true",
                     help="only measure the HBM ceiling: a STREAM triad on 3 x 4 GiB (probe/libswx_probe.so)")
+    ap.add_argument("--launch-gap", action="store_true",
+                    help="only measure launch gaps: dependent passes stream-launched against a replayed HIP graph "
+                         "(probe/libswx_probe.so)")
     args = ap.parse_args()
+    if args.launch_gap:
+        print(json.dumps(launch_gap()))
+        return
     if args.hbm_triad:
         print(json.dumps(hbm_triad()))
         return

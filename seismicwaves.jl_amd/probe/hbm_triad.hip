@@ -4,6 +4,7 @@
 // not part of the library's ABI (bench.py --hbm-triad loads it).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 
@@ -87,5 +88,93 @@ extern "C" int swx_probe_hbm_triad_mode(int64_t bytes_each, int reps, int blocks
   (void)hipFree(c);
   out[0] = best;
   out[1] = reps > 0 ? sum / reps : 0;
+  return err;
+}
+
+// Launch-gap probe: a chain of dependent three-point smoothing passes over
+// `n` floats (pass i reads buffer i mod 2 and writes the other), launched
+// back to back on one stream, and the same chain captured once into a HIP
+// graph and replayed.  It measures what a graph of a time loop would save
+// against stream launches for kernels of the step kernels' size.
+namespace {
+__global__ __launch_bounds__(256) void smooth3(float *__restrict__ dst, const float *__restrict__ src, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float l = src[i > 0 ? i - 1 : i], r = src[i + 1 < n ? i + 1 : i];
+  dst[i] = 0.5f * src[i] + 0.25f * (l + r);
+}
+}  // namespace
+
+// out[0]: best us per pass over `reps` stream-launched chains of `launches`
+// passes; out[1]: the same for replays of the captured chain; out[2]: best
+// us of one pass timed alone between events; out[3]: ms to capture and
+// instantiate the graph.  Returns 0, or a HIP error code.
+extern "C" int swx_probe_launch_chain(int64_t n, int launches, int reps, double *out) {
+  if (n <= 0 || launches <= 0 || reps <= 0) return 4;
+  float *buf[2] = {nullptr, nullptr};
+  if (hipMalloc(&buf[0], n * 4) != hipSuccess) return 2;
+  if (hipMalloc(&buf[1], n * 4) != hipSuccess) { (void)hipFree(buf[0]); return 2; }
+  (void)hipMemset(buf[0], 0, n * 4);
+  (void)hipMemset(buf[1], 0, n * 4);
+  hipStream_t st;
+  if (hipStreamCreate(&st) != hipSuccess) { (void)hipFree(buf[0]); (void)hipFree(buf[1]); return 1; }
+  const unsigned g = (unsigned)((n + 255) / 256);
+  auto chain = [&]() {
+    for (int i = 0; i < launches; i++) smooth3<<<g, 256, 0, st>>>(buf[(i + 1) & 1], buf[i & 1], n);
+  };
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto timed = [&](auto &&body) {
+    (void)hipEventRecord(e0, st);
+    body();
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return (double)ms;
+  };
+  int err = 0;
+  chain();   // warm-up
+  (void)hipStreamSynchronize(st);
+  double one = 1e30, str = 1e30, gra = 1e30;
+  for (int r = 0; r < 20; r++) {
+    const double ms = timed([&] { smooth3<<<g, 256, 0, st>>>(buf[1], buf[0], n); });
+    one = ms < one ? ms : one;
+  }
+  for (int r = 0; r < reps; r++) {
+    const double ms = timed(chain);
+    str = ms < str ? ms : str;
+  }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) err = 5;
+  if (!err) {
+    chain();
+    if (hipStreamEndCapture(st, &graph) != hipSuccess) err = 5;
+  }
+  if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) err = 6;
+  const double inst_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!err) {
+    (void)hipGraphLaunch(exec, st);   // warm-up
+    (void)hipStreamSynchronize(st);
+    for (int r = 0; r < reps; r++) {
+      const double ms = timed([&] { (void)hipGraphLaunch(exec, st); });
+      gra = ms < gra ? ms : gra;
+    }
+  }
+  if (!err && hipGetLastError() != hipSuccess) err = 3;
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(buf[0]);
+  (void)hipFree(buf[1]);
+  out[0] = str * 1e3 / launches;
+  out[1] = err ? 0.0 : gra * 1e3 / launches;
+  out[2] = one * 1e3;
+  out[3] = inst_ms;
   return err;
 }
